@@ -1,0 +1,65 @@
+/*
+ * dsm_oracle.h -- CPU restatement of the DSM strain-Green's-tensor (SGT)
+ * engines SGTsh / SGTpsv of IPGP/DSM-Kernel.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This oracle is the parity checker for the
+ * HIP implementation and the timed CPU baseline.  Only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it;
+ * the product library never links or calls it.
+ *
+ * Every routine restates the reference routine of the same name, with the
+ * reference's argument list (all arguments by pointer, Fortran column-major
+ * arrays, 1-based index arithmetic), so that each call site of the
+ * restated driver can be switched to the reference routine compiled from
+ * its own sources (oracle/_ref) -- see sh_oracle.c, "function table".
+ */
+#ifndef DSM_ORACLE_H
+#define DSM_ORACLE_H
+
+#include <stdint.h>
+#include <complex.h>
+
+#define ORC_MAXZONE 64
+
+/* Parsed inputs: the stdin parameter file (src/SGTsh/others.f90:1-60),
+ * DSM.config (others.f90:62-92) and the polynomial model file
+ * (SGTsh-MPI.f90:83-134, SGTpsv-MPI.f90:100-105).  Polynomials are stored
+ * coefficient-fastest, as the reference's rrho(4,nzone). */
+typedef struct {
+    int32_t nzone;
+    int32_t maxlmax;
+    int32_t imin, imax;
+    int32_t rsgtswitch, tsgtswitch, synnswitch, psgtswitch;
+    double vrmin[ORC_MAXZONE], vrmax[ORC_MAXZONE];
+    double rho[ORC_MAXZONE][4], vpv[ORC_MAXZONE][4], vph[ORC_MAXZONE][4];
+    double vsv[ORC_MAXZONE][4], vsh[ORC_MAXZONE][4], eta[ORC_MAXZONE][4];
+    double qmu[ORC_MAXZONE], qkappa[ORC_MAXZONE];
+    double re, ratc, ratl, omegai_seed;   /* DSM.config values, verbatim */
+    double tlen;
+    double rmin_, rmax_, rdelta_;         /* receiver radii (km) */
+    double r0min;                         /* source radius (km) */
+    double thetamin, thetamax, thetadelta;/* epicentral distances (deg) */
+} orc_input_t;
+
+/* Sizes of one SH run (after the fluid/inner-core removal and the
+ * receiver trim of SGTsh-MPI.f90:66-162). */
+typedef struct {
+    int32_t r_n, theta_n, nn, nzone;
+    double r0, spo;
+    int32_t spn, intir0;
+} orc_sh_info_t;
+
+/* use_ref: 0 = restated C routines, 1 = the reference's own routines
+ * loaded from oracle/_ref/libref_sh.so (see orc_sh_load_ref). */
+int orc_sh_info(const orc_input_t *in, orc_sh_info_t *info);
+int orc_sh_load_ref(const char *path);
+int orc_sh_mix(uint64_t mask);
+/* One frequency of SGTsh (SGTsh-MPI.f90:389-613).  Outputs are the FP64
+ * accumulators before the complex64 cast, Fortran layout
+ * tsgt(10,r_n,theta_n), rsgt(5,r_n,theta_n), synn(5,theta_n).
+ * llog receives the reference's log value llog-1 (SGTsh-MPI.f90:559). */
+int orc_sh_omega(const orc_input_t *in, int32_t i, int32_t use_ref,
+                 double complex *tsgt, double complex *rsgt,
+                 double complex *synn, int32_t *llog);
+
+#endif
