@@ -1,0 +1,173 @@
+// hostcheck.cpp -- TEST BUILD ONLY (libdsmsgt_hostcheck.so, never linked
+// into libdsmsgt.so): runs the product's SH formulation on the CPU -- the
+// same set-up code, the same per-lane bodies of the HIP kernels
+// (sh_lane.h, legendre_lane.h) and the same contraction table (sh_jobs.h),
+// with the MFMA contraction replaced by plain loops -- so that the
+// reformulation of the reference's per-theta loops can be checked against
+// the oracle without a GPU.
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "dsm_sgt.h"
+#include "fe_setup.h"
+#include "legendre_lane.h"
+#include "sh_jobs.h"
+#include "sh_lane.h"
+
+using namespace dsm;
+
+extern "C" int dsm_hostcheck_sh(const dsm_sgt_input_t *in, int32_t i, double *tsgt, double *rsgt, double *synn,
+                                int32_t *llog_out)
+{
+    ShSetup s;
+    std::string err;
+    int rc = sh_setup(*in, s, err);
+    if (rc) return rc;
+    ShOmega o;
+    sh_omega_setup(s, i, o);
+    const int nn = s.nn, R = s.r_n, T = s.theta_n, L = s.maxlmax;
+    const int lcap = ((L + 1 + 15) / 16) * 16;
+    std::vector<int> need(s.iista.begin(), s.iista.end());
+    need.push_back(nn);
+    std::sort(need.begin(), need.end(), std::greater<int>());
+    need.erase(std::unique(need.begin(), need.end()), need.end());
+    std::vector<int> rec_slot(3 * R);
+    for (int k = 0; k < 3 * R; k++)
+        rec_slot[k] = (int)(std::find(need.begin(), need.end(), s.iista[k]) - need.begin());
+    std::vector<double2> band(4 * (size_t)nn);
+    for (int j = 0; j < nn; j++) {
+        band[0 * nn + j] = make_double2(o.a0[2 * j].real(), o.a0[2 * j].imag());
+        band[1 * nn + j] = make_double2(o.a0[2 * j + 1].real(), o.a0[2 * j + 1].imag());
+        band[2 * nn + j] = make_double2(o.a2[2 * j].real(), o.a2[2 * j].imag());
+        band[3 * nn + j] = make_double2(o.a2[2 * j + 1].real(), o.a2[2 * j + 1].imag());
+    }
+    const int nneed = (int)need.size();
+    const int nlb = (L + 64) / 64, nthr = nlb * 64;
+    std::vector<double2> scratch((size_t)(nn + 10 * (nn - s.ns + 1)) * nthr);
+    std::vector<double2> xn((size_t)nneed * SH_NSOL * lcap);
+    std::vector<double2> E((size_t)SH_NQ * R * lcap), Es((size_t)SH_NQS * lcap);
+    std::vector<double> amp(lcap), rinv(R);
+    for (int r = 0; r < R; r++) rinv[r] = 1.0 / s.r_[r];
+    int slot0 = 0;
+    ShSrcConst src{};
+    for (int k = 0; k < 4; k++) {
+        src.t4[k] = s.t[s.ins - 1 + k];
+        src.h1[k] = s.h1[s.ins - 1 + k];
+        src.h2[k] = s.h2[s.ins - 1 + k];
+        src.h3[k] = s.h3[s.ins - 1 + k];
+        src.h4[k] = s.h4[s.ins - 1 + k];
+    }
+    for (int k = 0; k < 8; k++) {
+        src.gt[k] = s.gt[k];
+        src.gh1[k] = s.gh1[k];
+        src.gh2[k] = s.gh2[k];
+        src.gh3[k] = s.gh3[k];
+        src.gh4[k] = s.gh4[k];
+    }
+    src.r0 = s.r0;
+    src.mu0 = s.mu0;
+    double2 om[2] = {make_double2(o.comega2.real(), o.comega2.imag()),
+                     make_double2(o.coef_spn.real(), o.coef_spn.imag())};
+    ShSolveArgs sa{};
+    sa.band = band.data();
+    sa.slot_list = &slot0;
+    sa.nlb = nlb;
+    sa.l0 = 0;
+    sa.lend = L;
+    sa.nn = nn;
+    sa.k = s.ns;
+    sa.nneed = nneed;
+    sa.need = need.data();
+    sa.scratch = scratch.data();
+    sa.sstride = nthr;
+    sa.xn = xn.data();
+    sa.lcap = lcap;
+    sa.om = om;
+    sa.src = &src;
+    ShCoefArgs ca{};
+    ca.xn = xn.data();
+    ca.slot_list = &slot0;
+    ca.nlb = nlb;
+    ca.l0 = 0;
+    ca.lend = L;
+    ca.lcap = lcap;
+    ca.nneed = nneed;
+    ca.r_n = R;
+    ca.rec_slot = rec_slot.data();
+    ca.w0 = s.w0.data();
+    ca.w1 = s.w1.data();
+    ca.rinv = rinv.data();
+    ca.E = E.data();
+    ca.Es = Es.data();
+    ca.amp = amp.data();
+    for (int b = 0; b < nlb; b++)
+        for (int t = 0; t < 64; t++) {
+            sh_solve_lane(sa, b, t);
+            sh_coef_lane(ca, b, t);
+        }
+    // l cut-off (calamp replay, SH form)
+    int ismall = 0, llog = L, nl = L + 1;
+    double maxamp = -1.0;
+    for (int l = 0; l <= L; l++) {
+        if (ismall > 20) {
+            llog = l;
+            nl = l;
+            break;
+        }
+        for (int m = -2; m <= 2; m++) {
+            if (m == 0 || std::abs(m) > l) continue;
+            const double a = std::abs(m) == 2 ? amp[l] : 0.0;
+            double ratio = 0.0;
+            if (a > maxamp) maxamp = a;
+            if (a != 0.0 && maxamp != 0.0) ratio = a / maxamp;
+            if (ratio < s.ratl && l > o.lsuf) ismall++;
+            else ismall = 0;
+        }
+    }
+    *llog_out = llog - 1;
+    // theta functions
+    const int tpad = T;
+    std::vector<double> th(T), tab((size_t)12 * lcap * tpad);
+    for (int k = 0; k < T; k++) th[k] = s.theta_deg[k] / 180.0 * DSM_PI;
+    for (int k = 0; k < T; k++) legendre_sh_column(th.data(), T, tpad, L, lcap, tab.data(), k);
+    // contraction
+    std::memset(tsgt, 0, sizeof(double) * 2 * 10 * R * T);
+    std::memset(rsgt, 0, sizeof(double) * 2 * 5 * R * T);
+    std::memset(synn, 0, sizeof(double) * 2 * 5 * T);
+    const size_t plane = (size_t)lcap * tpad;
+    for (int jb = 0; jb < SH_NJOBS; jb++) {
+        const ShJobDef &d = SH_JOBS[jb];
+        if (d.kind == 2) continue;  // SYNN below
+        const int Re = d.kind == 2 ? 1 : R, nc = d.kind == 0 ? 10 : 5;
+        double *out = d.kind == 0 ? tsgt : (d.kind == 1 ? rsgt : synn);
+        for (int r = 0; r < Re; r++)
+            for (int t = 0; t < T; t++) {
+                double re = 0.0, im = 0.0;
+                for (int j = 0; j < d.nb; j++) {
+                    const double *Tb = tab.data() + (size_t)((d.mu - 1) * 6 + d.b[j]) * plane + t;
+                    const double2 *Eq = d.kind == 2 ? &Es[(size_t)(d.q0 + j) * lcap]
+                                                    : &E[((size_t)(d.q0 + j) * R + r) * lcap];
+                    for (int l = 0; l < nl; l++) {
+                        re += Eq[l].x * Tb[(size_t)l * tpad];
+                        im += Eq[l].y * Tb[(size_t)l * tpad];
+                    }
+                }
+                const size_t o2 = d.kind == 2 ? ((size_t)t * nc + d.comp) : (((size_t)t * R + r) * nc + d.comp);
+                out[2 * o2] = re;
+                out[2 * o2 + 1] = im;
+            }
+    }
+    ShSynnArgs ya{};
+    ya.xn = xn.data();
+    ya.nneed = nneed;
+    ya.lcap = lcap;
+    ya.nl = &nl;
+    ya.slot_out = &slot0;
+    ya.theta_rad = th.data();
+    ya.theta_n = T;
+    ya.nslot = 1;
+    ya.out = (double2 *)synn;
+    for (int k = 0; k < T; k++) sh_synn_lane(ya, 0, k);
+    return 0;
+}

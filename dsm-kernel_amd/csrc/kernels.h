@@ -1,0 +1,47 @@
+// kernels.h -- host-side launchers of the HIP kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "sh_lane.h"
+
+namespace dsm {
+
+// SH lane bodies and argument structs: sh_lane.h
+void launch_sh_solve(const ShSolveArgs &a, int nblocks, hipStream_t s);
+void launch_sh_coef(const ShCoefArgs &a, int nblocks, hipStream_t s);
+void launch_sh_synn(const ShSynnArgs &a, hipStream_t s);
+
+// ---- Legendre / vector-harmonic tables ------------------------------------
+// FP64 upward recurrence for P_l^m, m<=3 (trial.f90:85-117), one lane per
+// distance; writes the real basis functions of the synthesis for
+// l in [0, lmax] and |m| = 1, 2: tab[mu-1][b][l][theta_pad].
+void launch_legendre_sh(const double *theta_rad, int theta_n, int theta_pad, int lmax, int lcap,
+                        double *tab, hipStream_t s);
+
+// ---- zero the l tail of the coefficient rows ------------------------------
+void launch_zero_tail(double2 *E, int nslot_rows, int rows_per_slot, const int *slots,
+                      const int *nl, int lcap, hipStream_t s);
+
+// ---- synthesis contraction -------------------------------------------------
+// out[(slot,r)][theta][comp] += sum_{j,l} E[slot][q0+j][r][l] * T[mu][b_j][l][theta]
+// as a real GEMM on FP64 MFMA (complex A rows split into re/im rows).
+struct SynthJob {
+    int q0, nb, mu, b[4];
+    int comp, ncomp;       // component index within the record, record width
+    double2 *out;          // catalog of this kind: [pos][r_eff][theta_n][ncomp]
+};
+struct SynthArgs {
+    const double2 *E;      // [slot][nq][r_eff][lcap]
+    int nq, r_eff;
+    const double *tab;     // [mu-1][6][lcap][theta_pad]
+    int theta_n, theta_pad, lcap;
+    int nslot;
+    const int *nl;         // [slot] number of l terms
+    const int *slot_out;   // [slot] -> catalog frequency position
+    const SynthJob *jobs;  // device array
+    int njobs;
+};
+void launch_synth_f64(const SynthArgs &a, hipStream_t s);
+
+}  // namespace dsm

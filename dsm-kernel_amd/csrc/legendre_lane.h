@@ -1,0 +1,87 @@
+// legendre_lane.h -- per-distance body of the Legendre / theta-basis table
+// (legendre.hip), shared with the host test build (hostcheck.cpp).
+#pragma once
+
+#include "dev_common.h"
+
+namespace dsm {
+
+#ifndef DSM_PI_DEFINED
+#define DSM_PI_DEFINED
+static constexpr double DSM_PI = 3.1415926535897932;
+#endif
+
+// Upward recurrence of calplm (src/SGTsh/trial.f90:85-117).
+__host__ __device__ inline void calplm(int l, int m, double x, double *p)
+{
+    if (l == m) {
+        double pmm = 1.0;
+        if (m > 0) {
+            const double somx2 = sqrt((1.0 - x) * (1.0 + x));
+            double fact = 1.0;
+            for (int i = 1; i <= m; i++) {
+                pmm = -pmm * fact * somx2;
+                fact = fact + 2.0;
+            }
+        }
+        p[2] = 0.0;
+        p[1] = 0.0;
+        p[0] = pmm;
+    } else {
+        p[2] = p[1];
+        p[1] = p[0];
+        if (l == m + 1) p[0] = x * (double)(2 * m + 1) * p[1];
+        else p[0] = (x * (double)(2 * l - 1) * p[1] - (double)(l + m - 1) * p[2]) / (double)(l - m);
+    }
+}
+
+// SH basis, |m| = mu in {1,2}, b = 0..5:
+//   Y0 = c P/s, Y1 = c dP, Y2 = c (dP/s - x/(1-x^2) P),
+//   Y3 = c (x/s dP - mu^2/(1-x^2) P + l(l+1) P),
+//   Z23 = -mu^2 Y0/s + cot Y1,  Z33 = cot Y0 - Y1/s,
+// with c = sqrt((2l+1)/(4 pi) (l-mu)!/(l+mu)!), dP = mu x/s P_l^mu + P_l^{mu+1}
+// (trial.f90:1-81).  Zero at the poles (trial.f90:151-164) and for l < mu.
+__host__ __device__ inline void legendre_sh_column(const double *theta_rad, int theta_n, int theta_pad, int lmax,
+                                                   int lcap, double *tab, int it)
+{
+    if (it >= theta_pad) return;
+    const size_t plane = (size_t)lcap * theta_pad;
+    if (it >= theta_n) {  // padding columns
+        for (int b = 0; b < 12; b++)
+            for (int l = 0; l < lcap; l++) tab[(size_t)b * plane + (size_t)l * theta_pad + it] = 0.0;
+        return;
+    }
+    const double th = theta_rad[it];
+    const bool pole = (th == 0.0 || th == DSM_PI);
+    const double x = cos(th), s = sin(th);
+    const double cot = x / s;
+    double plm[4][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    for (int l = 0; l <= lmax; l++) {
+        if (!pole)
+            for (int m = 0; m <= (l < 3 ? l : 3); m++) calplm(l, m, x, plm[m]);
+        const double xl2 = (double)l * (double)(l + 1);
+        for (int mu = 1; mu <= 2; mu++) {
+            double y[6] = {0, 0, 0, 0, 0, 0};
+            if (!pole && l >= mu) {
+                double fact = 1.0;
+                for (int i = l - mu + 1; i <= l + mu; i++) fact = fact * (double)i;
+                const double c = sqrt((double)(2 * l + 1) / (4.0 * DSM_PI) / fact);
+                const double P = plm[mu][0], P1 = plm[mu + 1][0];
+                const double dP = (double)mu * x / s * P + P1;
+                const double mu2 = (double)(mu * mu);
+                y[0] = c * P / s;
+                y[1] = c * dP;
+                y[2] = c * (dP / s - x / (1 - x * x) * P);
+                y[3] = c * (x / s * dP - mu2 / (1 - x * x) * P + xl2 * P);
+                y[4] = -mu2 * y[0] / s + cot * y[1];
+                y[5] = cot * y[0] - y[1] / s;
+            }
+            for (int b = 0; b < 6; b++)
+                tab[(size_t)((mu - 1) * 6 + b) * plane + (size_t)l * theta_pad + it] = y[b];
+        }
+    }
+    for (int l = lmax + 1; l < lcap; l++)
+        for (int b = 0; b < 12; b++) tab[(size_t)b * plane + (size_t)l * theta_pad + it] = 0.0;
+}
+
+}  // namespace dsm

@@ -1,0 +1,164 @@
+"""Python host side of the MI355X SGT engine (ctypes over libdsmsgt.so).
+
+Mirrors the reference's driver interface for this path: the same three
+input files (stdin parameter file, DSM.config, polynomial model) and the
+same catalog records (TSGT / RSGT / SYNN, complex64, component fastest).
+The compute runs in the HIP library; there is no CPU fallback: creating an
+Engine without the built library or without a GPU raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from .inputs import DsmSgtInput, read_inputs, read_model, read_dsm_config, read_params, make_input  # noqa: F401
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "libdsmsgt.so")
+
+SH, PSV = 1, 2
+FP64, FP32, BF16X3 = 0, 1, 2
+TSGT, RSGT, SYNN = 0, 1, 2
+
+ERRORS = {1: "EINVAL", 2: "ESTATE", 3: "EHIP", 4: "ENOMEM", 5: "EIO", 6: "EUNSUPPORTED"}
+
+
+class Info(C.Structure):
+    _fields_ = [("mode", C.c_int32), ("r_n", C.c_int32), ("theta_n", C.c_int32), ("nn", C.c_int32),
+                ("num_tsgt", C.c_int32), ("num_rsgt", C.c_int32), ("num_synn", C.c_int32),
+                ("intir0", C.c_int32), ("r0", C.c_double)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("solve_ms", C.c_double), ("legendre_ms", C.c_double), ("synth_ms", C.c_double),
+                ("other_ms", C.c_double), ("solve_launches", C.c_int64), ("synth_launches", C.c_int64),
+                ("systems", C.c_int64), ("solve_bytes", C.c_double), ("synth_flops", C.c_double),
+                ("synth_bytes", C.c_double)]
+
+
+class DsmSgtError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+# every entry point declared in include/dsm_sgt.h
+EXPORTS = ("dsm_sgt_create", "dsm_sgt_get_info", "dsm_sgt_get_grid", "dsm_sgt_run", "dsm_sgt_fetch",
+           "dsm_sgt_fetch_fp64", "dsm_sgt_lcut", "dsm_sgt_write_catalog", "dsm_sgt_get_stats",
+           "dsm_sgt_reset_stats", "dsm_sgt_clear", "dsm_sgt_last_error", "dsm_sgt_destroy")
+
+
+def lib():
+    """Load libdsmsgt.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it first (__graft_entry__.build())")
+        L = C.CDLL(LIB_PATH)
+        L.dsm_sgt_create.argtypes = [C.POINTER(DsmSgtInput), C.c_int32, C.c_int32, C.c_int32,
+                                     C.POINTER(C.c_void_p)]
+        L.dsm_sgt_get_info.argtypes = [C.c_void_p, C.POINTER(Info)]
+        L.dsm_sgt_get_grid.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        L.dsm_sgt_run.argtypes = [C.c_void_p, C.c_int32, C.c_int32]
+        L.dsm_sgt_fetch.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_void_p]
+        L.dsm_sgt_fetch_fp64.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_void_p]
+        L.dsm_sgt_lcut.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_int32)]
+        L.dsm_sgt_write_catalog.argtypes = [C.c_void_p, C.c_int32, C.c_char_p, C.c_char_p]
+        L.dsm_sgt_get_stats.argtypes = [C.c_void_p, C.POINTER(Stats)]
+        L.dsm_sgt_reset_stats.argtypes = [C.c_void_p]
+        L.dsm_sgt_clear.argtypes = [C.c_void_p]
+        L.dsm_sgt_last_error.argtypes = [C.c_void_p]
+        L.dsm_sgt_last_error.restype = C.c_char_p
+        L.dsm_sgt_destroy.argtypes = [C.c_void_p]
+        L.dsm_sgt_destroy.restype = None
+        _lib = L
+    return _lib
+
+
+class Engine:
+    """One engine (handle) on one GPU: `Engine(inp, mode=SH)` runs the
+    omega-independent set-up; `run(i0, i1)` computes frequencies; `fetch*`
+    return catalog records as numpy arrays shaped (theta_n, num)."""
+
+    def __init__(self, inp, mode=SH, precision=FP64, device=0):
+        L = lib()
+        self._h = C.c_void_p()
+        rc = L.dsm_sgt_create(C.byref(inp), mode, precision, device, C.byref(self._h))
+        if rc:
+            msg = L.dsm_sgt_last_error(self._h).decode() if self._h else ""
+            if self._h:
+                L.dsm_sgt_destroy(self._h)
+            self._h = None
+            raise DsmSgtError(rc, msg)
+        self.info = Info()
+        self._check(L.dsm_sgt_get_info(self._h, C.byref(self.info)))
+        self.radii = np.zeros(self.info.r_n)
+        self.theta = np.zeros(self.info.theta_n)
+        self._check(L.dsm_sgt_get_grid(self._h, self.radii.ctypes.data, self.theta.ctypes.data))
+        self.mode = mode
+
+    def _check(self, rc):
+        if rc:
+            raise DsmSgtError(rc, lib().dsm_sgt_last_error(self._h).decode())
+
+    def num(self, kind):
+        return (self.info.num_tsgt, self.info.num_rsgt, self.info.num_synn)[kind]
+
+    def run(self, i0, i1):
+        self._check(lib().dsm_sgt_run(self._h, i0, i1))
+
+    def fetch(self, i, kind, ir=0):
+        out = np.zeros((self.info.theta_n, self.num(kind)), np.complex64)
+        self._check(lib().dsm_sgt_fetch(self._h, i, kind, ir, out.ctypes.data))
+        return out
+
+    def fetch_fp64(self, i, kind, ir=0):
+        out = np.zeros((self.info.theta_n, self.num(kind)), np.complex128)
+        self._check(lib().dsm_sgt_fetch_fp64(self._h, i, kind, ir, out.ctypes.data))
+        return out
+
+    def fetch_all_fp64(self, i):
+        """(tsgt[theta][r][num_tsgt], rsgt[theta][r][num_rsgt], synn[theta][num_synn])"""
+        R = self.info.r_n
+        t = np.stack([self.fetch_fp64(i, TSGT, ir) for ir in range(R)], axis=1)
+        r = np.stack([self.fetch_fp64(i, RSGT, ir) for ir in range(R)], axis=1)
+        s = self.fetch_fp64(i, SYNN)
+        return t, r, s
+
+    def lcut(self, i):
+        v = C.c_int32()
+        self._check(lib().dsm_sgt_lcut(self._h, i, C.byref(v)))
+        return v.value
+
+    def write_catalog(self, i, output_dir, model_name):
+        self._check(lib().dsm_sgt_write_catalog(self._h, i, output_dir.encode(), model_name.encode()))
+
+    def stats(self):
+        s = Stats()
+        self._check(lib().dsm_sgt_get_stats(self._h, C.byref(s)))
+        return s
+
+    def reset_stats(self):
+        self._check(lib().dsm_sgt_reset_stats(self._h))
+
+    def clear(self):
+        self._check(lib().dsm_sgt_clear(self._h))
+
+    def close(self):
+        if self._h:
+            lib().dsm_sgt_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()

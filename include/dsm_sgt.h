@@ -1,0 +1,140 @@
+/*
+ * dsm_sgt.h -- C-ABI of the MI355X strain-Green's-tensor (SGT) engine.
+ *
+ * Drop-in for the per-frequency loop body of IPGP/DSM-Kernel's SGTsh and
+ * SGTpsv drivers.  The reference has no plugin/operator API for this path:
+ * the work is inlined in the driver loops
+ *     src/SGTsh/SGTsh-MPI.f90:389-613     (SH, one frequency per iteration)
+ *     src/SGTpsv/SGTpsv-MPI.f90:535-1153  (P-SV, one frequency per iteration)
+ * and the host side around them (input parsing, grid, element integrals)
+ * is the set-up code SGTsh-MPI.f90:27-348 / SGTpsv-MPI.f90:37-477.  These
+ * entry points replace that loop body at frequency-batch granularity; the
+ * caller keeps the reference's stdin grammar, files and catalog layout.
+ *
+ * All arrays are plain pointers; complex numbers are interleaved
+ * (re, im) pairs, i.e. Fortran complex(4)/complex(8), C float/double
+ * _Complex.  No exceptions or process exits cross this boundary: every
+ * call returns a status (0 = ok, see DSM_SGT_E*) and
+ * dsm_sgt_last_error() describes the last failure.  A handle is used by
+ * one host thread at a time.
+ */
+#ifndef DSM_SGT_H
+#define DSM_SGT_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DSM_SGT_MAXZONE 64
+
+/* Parsed inputs.  Fields mirror what the reference reads:
+ *   parameter file (stdin)  src/SGTsh/others.f90:1-60  (pinput)
+ *   DSM.config              src/SGTsh/others.f90:62-92 (readDSMconf)
+ *   model file              src/SGTpsv/SGTpsv-MPI.f90:100-105 (28 values per zone)
+ * Polynomials are stored coefficient-fastest like the reference's
+ * rrho(4,nzone); omegai_seed is the DSM.config value, converted by the
+ * engine as -log(seed)/tlen (SGTpsv-MPI.f90:125). */
+typedef struct dsm_sgt_input {
+    int32_t nzone;
+    int32_t maxlmax;
+    int32_t imin, imax;
+    int32_t rsgtswitch, tsgtswitch, synnswitch, psgtswitch;
+    double vrmin[DSM_SGT_MAXZONE], vrmax[DSM_SGT_MAXZONE];
+    double rho[DSM_SGT_MAXZONE][4], vpv[DSM_SGT_MAXZONE][4], vph[DSM_SGT_MAXZONE][4];
+    double vsv[DSM_SGT_MAXZONE][4], vsh[DSM_SGT_MAXZONE][4], eta[DSM_SGT_MAXZONE][4];
+    double qmu[DSM_SGT_MAXZONE], qkappa[DSM_SGT_MAXZONE];
+    double re, ratc, ratl, omegai_seed;
+    double tlen;
+    double rmin_, rmax_, rdelta_;
+    double r0min;
+    double thetamin, thetamax, thetadelta;
+} dsm_sgt_input_t;
+
+enum { DSM_SGT_SH = 1, DSM_SGT_PSV = 2 };
+/* Arithmetic of the synthesis contraction.  FP64 is the parity form;
+ * FP32 and BF16X3 are the mixed forms (band solve and Legendre table stay
+ * FP64 in all forms). */
+enum { DSM_SGT_FP64 = 0, DSM_SGT_FP32 = 1, DSM_SGT_BF16X3 = 2 };
+/* Catalog kinds (file suffixes TSGT_*, RSGT_*, SYNN_*). */
+enum { DSM_SGT_TSGT = 0, DSM_SGT_RSGT = 1, DSM_SGT_SYNN = 2 };
+
+enum {
+    DSM_SGT_OK = 0,
+    DSM_SGT_EINVAL = 1,    /* bad argument or input (e.g. source outside the model) */
+    DSM_SGT_ESTATE = 2,    /* frequency not computed, or handle misuse */
+    DSM_SGT_EHIP = 3,      /* HIP runtime error (no device, launch failure) */
+    DSM_SGT_ENOMEM = 4,    /* device or host allocation failed */
+    DSM_SGT_EIO = 5,       /* catalog file could not be written */
+    DSM_SGT_EUNSUPPORTED = 6
+};
+
+typedef struct dsm_sgt dsm_sgt_t;
+
+/* Sizes of a run after the reference's own set-up (receiver trimming for
+ * SH at SGTsh-MPI.f90:152-162, source clamping in calspo). */
+typedef struct dsm_sgt_info {
+    int32_t mode;
+    int32_t r_n;        /* receiver radii */
+    int32_t theta_n;    /* epicentral distances */
+    int32_t nn;         /* band rows of the (omega,l) systems */
+    int32_t num_tsgt, num_rsgt, num_synn;  /* components per catalog record */
+    int32_t intir0;     /* int(r0*1e3) before calspo clamps r0 (file names) */
+    double r0;          /* source radius after calspo */
+} dsm_sgt_info_t;
+
+/* Per-stage device time and work since the last reset (HIP events on the
+ * engine's stream). */
+typedef struct dsm_sgt_stats {
+    double solve_ms, legendre_ms, synth_ms, other_ms;
+    int64_t solve_launches, synth_launches;
+    int64_t systems;          /* (omega,l) systems factored */
+    double solve_bytes;       /* algorithmic bytes of the band solves */
+    double synth_flops;       /* algorithmic flops of the synthesis contraction */
+    double synth_bytes;       /* operand bytes streamed by the synthesis */
+} dsm_sgt_stats_t;
+
+/* Create an engine on HIP device `device` for `mode` and `precision`.
+ * Runs the omega-independent set-up (grid, element integrals, Legendre
+ * table).  Replaces SGTsh-MPI.f90:146-348 / SGTpsv-MPI.f90:123-477. */
+int dsm_sgt_create(const dsm_sgt_input_t *in, int32_t mode, int32_t precision, int32_t device,
+                   dsm_sgt_t **handle);
+
+int dsm_sgt_get_info(dsm_sgt_t *h, dsm_sgt_info_t *info);
+/* Receiver radii (r_n values) and distances in degrees (theta_n values). */
+int dsm_sgt_get_grid(dsm_sgt_t *h, double *radii, double *theta_deg);
+
+/* Compute every frequency index i in [i_begin, i_end] except i = 0
+ * (the reference never computes i = 0, SGTpsv-MPI.f90:550).  Results stay
+ * in device memory until fetched.  Replaces the body of the frequency
+ * loop, SGTsh-MPI.f90:389-560 / SGTpsv-MPI.f90:535-1095. */
+int dsm_sgt_run(dsm_sgt_t *h, int32_t i_begin, int32_t i_end);
+
+/* Copy one catalog record: kind (DSM_SGT_TSGT/RSGT/SYNN), frequency i and
+ * receiver index ir (0-based; ignored for SYNN) into `out`, a Fortran
+ * array complex(4) out(num, theta_n), exactly the record the reference
+ * writes (SGTsh-MPI.f90:577-611, SGTpsv-MPI.f90:1113-1149). */
+int dsm_sgt_fetch(dsm_sgt_t *h, int32_t i, int32_t kind, int32_t ir, void *out);
+/* Same record before the complex64 cast: complex(8) out(num, theta_n). */
+int dsm_sgt_fetch_fp64(dsm_sgt_t *h, int32_t i, int32_t kind, int32_t ir, void *out);
+/* The l cut-off logged by the reference for frequency i (its llog-1,
+ * SGTsh-MPI.f90:559, SGTpsv-MPI.f90:1098). */
+int dsm_sgt_lcut(dsm_sgt_t *h, int32_t i, int32_t *llog_minus_1);
+/* Write the catalog files of frequency i under outputDir with the
+ * reference's names and byte layout (raw little-endian complex(4),
+ * component fastest, no record markers). */
+int dsm_sgt_write_catalog(dsm_sgt_t *h, int32_t i, const char *output_dir, const char *model_name);
+
+int dsm_sgt_get_stats(dsm_sgt_t *h, dsm_sgt_stats_t *stats);
+int dsm_sgt_reset_stats(dsm_sgt_t *h);
+/* Drop all computed frequencies (device memory is kept for reuse). */
+int dsm_sgt_clear(dsm_sgt_t *h);
+
+const char *dsm_sgt_last_error(dsm_sgt_t *h);
+void dsm_sgt_destroy(dsm_sgt_t *h);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
