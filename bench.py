@@ -1,0 +1,151 @@
+#!/usr/bin/env python3
+"""SGT catalog benchmark on one MI355X.
+
+Workload (one "step"): the full SH strain-Green's-tensor catalog of the 10 s
+configuration -- synthetic 4-zone sphere (fluid core removed for SH),
+tlen 3276.8 s, frequencies i = 1..328, re = 0.1 (531 band rows),
+lmax 4000, 40 receiver radii (4421..6371 km, 50 km), 1801 distances
+(0..180 deg, 0.1 deg), source at the surface -- computed into HBM (FP64
+accumulators), including the omega-independent set-up and the Legendre
+table.  The timed region starts with the model in host memory and ends with
+the catalog resident in HBM; writing the files is not timed.
+
+Prints one JSON line with the metric of the reference's headline benchmark,
+the roofline of the dominant kernel (the FP64-MFMA synthesis contraction)
+and a CPU baseline: the oracle (C restatement of the reference) timed on a
+bounded sample of the same workload on this host.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(HERE, "dsm-kernel_amd"))
+sys.path.insert(0, os.path.join(HERE, "tests"))
+
+METRIC = "SGT catalog wall time: batched/mixed vs plain FP64 on 1 GPU at ≤1e-6 rel err"
+FP64_MFMA_PEAK = 78.6e12
+HBM_PEAK = 8.0e12
+
+C2 = dict(tlen=3276.8, imax=328, rmin_=4421.0, rmax_=6371.0, rdelta_=50.0, thetamin=0.0,
+          thetamax=180.0, thetadelta=0.1, maxlmax=4000, re=1e-1, r0=6371.0)
+
+
+def make_input(p):
+    import dsm_sgt
+    from dsm_sgt.inputs import read_model, make_input as mk
+    with open(os.path.join(HERE, "dsm-kernel_amd", "configs", "model4.psv.model")) as f:
+        zones = read_model(f.read())
+    conf = dict(re=p["re"], ratc=1e-10, ratl=1e-5, omegai_seed=2e-2, maxlmax=p["maxlmax"])
+    params = dict(tlen=p["tlen"], rmin_=p["rmin_"], rmax_=p["rmax_"], rdelta_=p["rdelta_"],
+                  r0min=p["r0"], thetamin=p["thetamin"], thetamax=p["thetamax"],
+                  thetadelta=p["thetadelta"], imin=0, imax=p["imax"], rsgtswitch=1,
+                  tsgtswitch=1, synnswitch=1, psgtswitch=0)
+    return mk(zones, conf, params)
+
+
+def cpu_baseline(p, budget_s=20.0):
+    """Time the oracle on one frequency with two distance subsets, fit
+    t = a + b*theta_n and extrapolate to the full catalog (every frequency
+    runs the full l range: the surface source is never cut off before
+    lmax in this configuration, which the GPU run confirms)."""
+    import oracle_lib as ol
+    import dsm_inputs as di
+    q = dict(p)
+    i = p["imax"] // 2
+    pts = []
+    for nth in (2, 6):
+        q["thetamin"], q["thetamax"] = 10.0, 10.0 + (nth - 1) * 0.1
+        inp = di.synthetic_input(tlen=q["tlen"], imax=q["imax"], rmin_=q["rmin_"], rmax_=q["rmax_"],
+                                 rdelta_=q["rdelta_"], thetamin=q["thetamin"], thetamax=q["thetamax"],
+                                 thetadelta=0.1, maxlmax=q["maxlmax"], re=q["re"], r0=q["r0"])
+        t0 = time.perf_counter()
+        ol.sh_omega(inp, i, fast=True)
+        pts.append((nth, time.perf_counter() - t0))
+    (n1, t1), (n2, t2) = pts
+    b = (t2 - t1) / (n2 - n1)
+    a = max(t1 - b * n1, 0.0)
+    nfull = int(round((p["thetamax"] - p["thetamin"]) / p["thetadelta"])) + 1
+    per_omega = a + b * nfull
+    total = per_omega * p["imax"]
+    return dict(value=total, unit="s", cores=1, kind="port",
+                sample=f"oracle (gcc -O3 -march=native, 1 thread) on frequency {i} with "
+                       f"{n1} and {n2} distances ({t1:.1f} s, {t2:.1f} s); t = a + b*theta_n "
+                       f"extrapolated to {nfull} distances x {p['imax']} frequencies")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline sample")
+    ap.add_argument("--imax", type=int, default=None, help="(testing) fewer frequencies")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU only (the frequency axis is the "
+              "multi-GPU split; see DESIGN.md); run with --gpus 1", file=sys.stderr)
+        return 2
+    import dsm_sgt
+    p = dict(C2)
+    if args.imax:
+        p["imax"] = args.imax
+    inp = make_input(p)
+
+    def step():
+        eng = dsm_sgt.Engine(inp, mode=dsm_sgt.SH, precision=dsm_sgt.FP64)
+        eng.run(1, p["imax"])
+        return eng
+
+    for _ in range(args.warmup):
+        step().close()
+    times, stats = [], []
+    for _ in range(args.steps):
+        t0 = time.perf_counter()
+        eng = step()  # run() synchronises the device before returning
+        times.append(time.perf_counter() - t0)
+        stats.append(eng.stats())
+        cuts = [eng.lcut(i) for i in (1, p["imax"])]
+        eng.close()
+    ms = 1e3 * sum(times) / len(times)
+    st = stats[-1]
+    synth_ms_per_launch = st.synth_ms / max(st.synth_launches, 1)
+    flops_per_launch = st.synth_flops / max(st.synth_launches, 1)
+    achieved = flops_per_launch / (synth_ms_per_launch * 1e-3) if synth_ms_per_launch > 0 else 0.0
+    out = {
+        "metric": METRIC,
+        "value": ms / 1e3,
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "SGTsh 10 s catalog (SH plain FP64): 4-zone sphere, tlen 3276.8 s, "
+                               f"i=1..{p['imax']}, re 0.1 (531 rows), lmax {p['maxlmax']}, "
+                               "40 radii x 1801 distances, surface source",
+                   "form": "plain FP64", "frequencies": p["imax"], "lmax": p["maxlmax"],
+                   "radii": 40, "distances": 1801, "l_cut": cuts},
+        "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": FP64_MFMA_PEAK / 1e12,
+                     "unit": "TFLOP/s", "frac": achieved / FP64_MFMA_PEAK, "traffic": None,
+                     "kernel": "synth_f64_kernel",
+                     "avg_launch_ms": synth_ms_per_launch, "flops_per_launch": flops_per_launch},
+        "stages_ms": {"solve+coef": st.solve_ms, "legendre": st.legendre_ms, "synth": st.synth_ms,
+                      "other": st.other_ms},
+        "solve_hbm": {"achieved_GBps": st.solve_bytes / (st.solve_ms * 1e-3) / 1e9 if st.solve_ms else None,
+                      "peak_GBps": HBM_PEAK / 1e9, "systems": st.systems},
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(p)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -143,7 +143,6 @@ static void sh_synth_jobs(dsm_sgt_t *h)
         j.out = (double2 *)outs[d.kind];
         h->jobs.push_back(j);
         if (d.kind < 2) h->njobs_main++;
-        else break;  // SYNN: direct per-distance evaluation (sh_synn_lane)
     }
     h->njobs_syn = (int)h->jobs.size() - h->njobs_main;
 }
@@ -154,7 +153,7 @@ static int sh_create(dsm_sgt_t *h)
     if (rc) return rc;
     const ShSetup &s = h->sh;
     const int R = s.r_n;
-    h->theta_pad = ((s.theta_n + 63) / 64) * 64;
+    h->theta_pad = ((s.theta_n + 127) / 128) * 128;  // synthesis N tile
     h->lcap = ((s.maxlmax + 1 + 15) / 16) * 16;
     // stencil nodes, descending, always including the surface node nn
     std::vector<int> nodes(s.iista.begin(), s.iista.end());
@@ -345,9 +344,9 @@ static int sh_run_batch(dsm_sgt_t *h, const std::vector<int> &freqs)
         // algorithmic bytes: band read once, 14 nonzero right-hand sides'
         // (value, derivative) at every receiver written once (w = 2)
         h->stats.solve_bytes += (double)active.size() * (lend - l0 + 1) * 16.0 * (2.0 * nn + 4.0 * 14 * R);
-        for (int k : active)
-            HIPCHK(h, hipMemcpyAsync(&amp[(size_t)k * LC + l0], h->d_amp.as<double>() + (size_t)k * LC + l0,
-                                     sizeof(double) * (lend - l0 + 1), hipMemcpyDeviceToHost, h->stream));
+        // |g0(nn)| of the chunk for every slot: one strided copy
+        HIPCHK(h, hipMemcpy2DAsync(&amp[l0], sizeof(double) * LC, h->d_amp.as<double>() + l0, sizeof(double) * LC,
+                                   sizeof(double) * (lend - l0 + 1), S, hipMemcpyDeviceToHost, h->stream));
         HIPCHK(h, hipStreamSynchronize(h->stream));
         // replay the cut-off in the reference's order (SGTsh-MPI.f90:423-478)
         std::vector<int> still;
@@ -409,22 +408,18 @@ static int sh_run_batch(dsm_sgt_t *h, const std::vector<int> &freqs)
     launch_synth_f64(ga, h->stream);
     HIPCHK(h, hipGetLastError());
     ev_end(h, tg);
-    h->stats.synth_launches += 1;
-    ShSynnArgs ya{};
-    ya.xn = h->d_xn.as<double2>();
-    ya.nneed = h->nneed;
-    ya.lcap = LC;
-    ya.nl = h->d_nl.as<int>();
-    ya.slot_out = h->d_slotout.as<int>();
-    ya.theta_rad = h->d_theta.as<double>();
-    ya.theta_n = s.theta_n;
-    ya.nslot = S;
-    ya.out = h->d_cat_s.as<double2>();
-    TimedLaunch ty;
-    ev_begin(h, ty, 3);
-    launch_sh_synn(ya, h->stream);
+    // SYNN rows (surface node, no receiver dimension)
+    ga.E = h->d_Es.as<double2>();
+    ga.nq = SH_NQS;
+    ga.r_eff = 1;
+    ga.jobs = h->d_jobs.as<SynthJob>() + h->njobs_main;
+    ga.njobs = h->njobs_syn;
+    TimedLaunch tg2;
+    ev_begin(h, tg2, 3);
+    launch_synth_f64(ga, h->stream);
     HIPCHK(h, hipGetLastError());
-    ev_end(h, ty);
+    ev_end(h, tg2);
+    h->stats.synth_launches += 1;
     double sumL = 0.0;
     for (int k = 0; k < S; k++) sumL += nl[k];
     // 4 flops per (complex coefficient x real basis) per (r,theta); 36 rows

@@ -138,7 +138,6 @@ extern "C" int dsm_hostcheck_sh(const dsm_sgt_input_t *in, int32_t i, double *ts
     const size_t plane = (size_t)lcap * tpad;
     for (int jb = 0; jb < SH_NJOBS; jb++) {
         const ShJobDef &d = SH_JOBS[jb];
-        if (d.kind == 2) continue;  // SYNN below
         const int Re = d.kind == 2 ? 1 : R, nc = d.kind == 0 ? 10 : 5;
         double *out = d.kind == 0 ? tsgt : (d.kind == 1 ? rsgt : synn);
         for (int r = 0; r < Re; r++)
@@ -158,16 +157,5 @@ extern "C" int dsm_hostcheck_sh(const dsm_sgt_input_t *in, int32_t i, double *ts
                 out[2 * o2 + 1] = im;
             }
     }
-    ShSynnArgs ya{};
-    ya.xn = xn.data();
-    ya.nneed = nneed;
-    ya.lcap = lcap;
-    ya.nl = &nl;
-    ya.slot_out = &slot0;
-    ya.theta_rad = th.data();
-    ya.theta_n = T;
-    ya.nslot = 1;
-    ya.out = (double2 *)synn;
-    for (int k = 0; k < T; k++) sh_synn_lane(ya, 0, k);
     return 0;
 }

@@ -10,7 +10,6 @@ namespace dsm {
 // SH lane bodies and argument structs: sh_lane.h
 void launch_sh_solve(const ShSolveArgs &a, int nblocks, hipStream_t s);
 void launch_sh_coef(const ShCoefArgs &a, int nblocks, hipStream_t s);
-void launch_sh_synn(const ShSynnArgs &a, hipStream_t s);
 
 // ---- Legendre / vector-harmonic tables ------------------------------------
 // FP64 upward recurrence for P_l^m, m<=3 (trial.f90:85-117), one lane per

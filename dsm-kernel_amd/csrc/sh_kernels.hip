@@ -19,17 +19,6 @@ void launch_sh_coef(const ShCoefArgs &a, int nblocks, hipStream_t s)
     hipLaunchKernelGGL(sh_coef_kernel, dim3(nblocks), dim3(64), 0, s, a);
 }
 
-__global__ __launch_bounds__(64) void sh_synn_kernel(ShSynnArgs a)
-{
-    const int it = blockIdx.x * blockDim.x + threadIdx.x;
-    if (it < a.theta_n) sh_synn_lane(a, blockIdx.y, it);
-}
-
-void launch_sh_synn(const ShSynnArgs &a, hipStream_t s)
-{
-    hipLaunchKernelGGL(sh_synn_kernel, dim3((a.theta_n + 63) / 64, a.nslot), dim3(64), 0, s, a);
-}
-
 // Zero E[slot][row][l] for l >= nl[slot] (the solve chunks run past the cut-off).
 __global__ void zero_tail_kernel(double2 *E, int rows_per_slot, const int *slots, const int *nl, int lcap)
 {

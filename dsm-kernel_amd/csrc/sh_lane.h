@@ -412,90 +412,3 @@ __host__ __device__ inline void sh_coef_lane(const ShCoefArgs &a, int b, int tx)
 
 }  // namespace dsm
 
-// ---------------------------------------------------------------------------
-// SYNN (surface synthetics, utosynnSH others.f90:132-186).  The surface
-// receiver sees the source's whole l series (no cut-off at low frequency),
-// so the sum is evaluated per distance in the reference's order -- l
-// ascending, then m = -2..2, then imt -- with the reference's expressions
-// for the harmonics (calbvecphi0 trial.f90:1-81, calu others.f90:829-838)
-// and no contraction.  One lane per (frequency, distance).
-// ---------------------------------------------------------------------------
-#include "legendre_lane.h"
-namespace dsm {
-struct ShSynnArgs {
-    const double2 *xn;      // [slot][nneed][SH_NSOL][lcap]; node nn is need[0]
-    int nneed, lcap;
-    const int *nl;          // [slot] l terms
-    const int *slot_out;    // [slot] -> catalog position
-    const double *theta_rad;
-    int theta_n, nslot;
-    double2 *out;           // [pos][theta_n][5]
-};
-
-__host__ __device__ inline void sh_synn_lane(const ShSynnArgs &a, int sl, int it)
-{
-#pragma clang fp contract(off)
-    const double th = a.theta_rad[it];
-    double2 acc[5] = {cz(), cz(), cz(), cz(), cz()};
-    const bool pole = (th == 0.0 || th == DSM_PI);  // all harmonics vanish (trial.f90:151-164)
-    if (!pole) {
-        const double x = cos(th);
-        double plm[4][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
-        const double2 *X = a.xn + (size_t)sl * a.nneed * SH_NSOL * a.lcap;  // node nn
-        const int nl = a.nl[sl];
-        for (int l = 0; l < nl; l++) {
-            for (int m = 0; m <= (l < 3 ? l : 3); m++) calplm(l, m, x, plm[m]);
-            if (l == 0) continue;
-            const double lsq = sqrt((double)l * (double)(l + 1));
-            double2 b2[3], b3[3];  // index mu = 1, 2: +mu harmonics
-            for (int mu = 1; mu <= (l < 2 ? l : 2); mu++) {
-                double fact = 1.0;
-                for (int i = l - mu + 1; i <= l + mu; i++) fact = fact * (double)i;
-                const double coef = sqrt((double)(2 * l + 1) / (4.0 * DSM_PI) / fact);
-                const double plmdt = (double)mu * x / sin(th) * plm[mu][0] + plm[mu + 1][0];
-                // (i mu)/sin * coef * P, component-wise as complex/real
-                b2[mu] = make_double2(0.0 / sin(th) * coef * plm[mu][0], (double)mu / sin(th) * coef * plm[mu][0]);
-                b3[mu] = make_double2(-coef * plmdt, 0.0);
-            }
-            for (int m = -2; m <= 2; m++) {
-                if (m == 0 || (m < 0 ? -m : m) > l) continue;
-                const int mu = m < 0 ? -m : m;
-                double2 h2 = b2[mu], h3 = b3[mu];
-                if (m < 0) {  // b(-m) = (-1)^m conj(b(m))
-                    h2 = cconj(h2);
-                    h3 = cconj(h3);
-                    if (mu & 1) {
-                        h2 = cneg(h2);
-                        h3 = cneg(h3);
-                    }
-                }
-                // sources of this m in imt order; ex index of sh_excitation
-                const int nimt = (mu == 1) ? 2 : 3;
-                for (int t = 0; t < nimt; t++) {
-                    const int e = (mu == 1) ? ((m < 0 ? 0 : 2) + t) : ((m < 0 ? 4 : 7) + t);
-                    const double2 g = X[(size_t)e * a.lcap + l];
-                    const double2 gb2 = cmul(g, h2), gb3 = cmul(g, h3);
-                    const double2 u2 = make_double2(gb2.x / lsq, gb2.y / lsq);
-                    const double2 u3 = make_double2(gb3.x / lsq, gb3.y / lsq);
-                    const int imt = (mu == 1) ? 4 + t : (t == 0 ? 2 : (t == 1 ? 3 : 6));
-                    if (imt == 2) {
-                        acc[0] = csub(acc[0], cscale(u2, 0.5));
-                        acc[1] = csub(acc[1], cscale(u2, 0.5));
-                    } else if (imt == 3) {
-                        acc[0] = csub(acc[0], cscale(u2, 0.5));
-                        acc[1] = cadd(acc[1], cscale(u2, 0.5));
-                    } else if (imt == 4) {
-                        acc[3] = cadd(acc[3], u2);
-                    } else if (imt == 5) {
-                        acc[4] = cadd(acc[4], u3);
-                    } else {
-                        acc[2] = csub(acc[2], u3);
-                    }
-                }
-            }
-        }
-    }
-    double2 *o = a.out + ((size_t)a.slot_out[sl] * a.theta_n + it) * 5;
-    for (int c = 0; c < 5; c++) o[c] = acc[c];
-}
-}  // namespace dsm

@@ -9,100 +9,172 @@
 // with A the re/im parts of the coefficient rows and T the real theta
 // functions of the component's |m| (legendre.hip).
 //
-// Tile: 64 rows x 64 distances per 256-thread workgroup, 4 waves in a 2x2
-// arrangement, each wave 2x2 MFMA 16x16x4 FP64 blocks; K staged through LDS
-// in chunks of 16.
+// Tile: 128 rows x 128 distances per 512-thread workgroup; 8 waves in a 2x4
+// arrangement, each wave 4x2 blocks of v_mfma_f64_16x16x4f64 (64x32).  K
+// (= (theta function, l)) is staged through LDS in chunks of 16 with a
+// register-staged double buffer: the global loads of chunk t+1 are in
+// flight while the MFMAs of chunk t run.  128x128 tiles keep the operand
+// stream at ~5 TB/s chip-wide at the FP64 MFMA rate (64x64 tiles needed ~10).
 #include "dev_common.h"
 #include "kernels.h"
 
 namespace dsm {
 
-constexpr int BM = 64, BN = 64, BK = 16;
+constexpr int BM = 128, BN = 128, BK = 16;
 
-__global__ __launch_bounds__(256) void synth_f64_kernel(SynthArgs a)
+constexpr int NT = 512;
+struct TileLoad {
+    double2 a[2];  // 64 complex rows x 16 l / 512 threads
+    double2 b[2];  // 16 l x 128 theta / 512 threads (pairs)
+};
+
+__device__ __forceinline__ void load_tile(const SynthArgs &a, const SynthJob &job, const double *tab_mu,
+                                          size_t plane, int m0, int n0, int M, int R, int it, int nk, int tid,
+                                          TileLoad &t)
 {
-    __shared__ double As[BK][BM];
-    __shared__ double Bs[BK][BN];
-    __shared__ int kmax_s;
-    const SynthJob job = a.jobs[blockIdx.z];
+    const int jb = it / nk, l0 = (it - jb * nk) * BK;
+    const double2 *Eq = a.E + (size_t)(job.q0 + jb) * R * a.lcap;
+    const double *T = tab_mu + (size_t)job.b[jb] * plane;
+#pragma unroll
+    for (int u = 0; u < 2; u++) {
+        const int idx = tid + NT * u;
+        const int pr = idx >> 4, kk = idx & 15;
+        const int prow = (m0 >> 1) + pr;
+        double2 v = make_double2(0.0, 0.0);
+        if (2 * prow < M) {
+            const int sl = prow / R, r = prow - sl * R;
+            v = Eq[((size_t)sl * a.nq * R + r) * a.lcap + l0 + kk];
+        }
+        t.a[u] = v;
+    }
+#pragma unroll
+    for (int u = 0; u < 2; u++) {
+        const int idx = (tid + NT * u) * 2;
+        const int kk = idx >> 7, c = idx & 127;
+        t.b[u] = *(const double2 *)(T + (size_t)(l0 + kk) * a.theta_pad + n0 + c);
+    }
+}
+
+__device__ __forceinline__ void store_tile(double (*As)[BM], double (*Bs)[BN], int tid, const TileLoad &t)
+{
+#pragma unroll
+    for (int u = 0; u < 2; u++) {
+        const int idx = tid + NT * u;
+        const int pr = idx >> 4, kk = idx & 15;
+        As[kk][2 * pr] = t.a[u].x;
+        As[kk][2 * pr + 1] = t.a[u].y;
+    }
+#pragma unroll
+    for (int u = 0; u < 2; u++) {
+        const int idx = (tid + NT * u) * 2;
+        const int kk = idx >> 7, c = idx & 127;
+        Bs[kk][c] = t.b[u].x;
+        Bs[kk][c + 1] = t.b[u].y;
+    }
+}
+
+// Workgroups are dispatched round-robin over the 8 XCDs, each with its own
+// L2.  Every job's tiles (row panel major, distance tile minor) are cut into
+// 8 equal contiguous slices, one per XCD, and each XCD walks its slices job
+// by job: the tiles resident on one XCD share A panels (same row panel) and
+// B panels (neighbouring row panels), and all XCDs get the same work.
+// Returns false for the padding slots of the last slice.
+__device__ __forceinline__ bool xcd_tile(int nx, int ny, int &tx, int &ty, int &tz)
+{
+    constexpr int NXCD = 8;
+    const int T = nx * ny, cpx = (T + NXCD - 1) / NXCD;
+    const int pid = blockIdx.x, xcd = pid % NXCD, local = pid / NXCD;
+    tz = local / cpx;
+    const int t = xcd * cpx + (local - tz * cpx);
+    if (t >= T) return false;
+    ty = t / nx;
+    tx = t - ty * nx;
+    return true;
+}
+
+template <int KIND>  // 0: receiver rows, 1: surface rows (separate symbol for profiling)
+__global__ __launch_bounds__(512) void synth_f64_kernel(SynthArgs a, int nx, int ny)
+{
+    __shared__ double As[2][BK][BM];
+    __shared__ double Bs[2][BK][BN];
+    int tx, ty, tz;
+    if (!xcd_tile(nx, ny, tx, ty, tz)) return;
+    const SynthJob job = a.jobs[tz];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int R = a.r_eff;
     const int M = a.nslot * R * 2;
-    const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
-    if (tid == 0) kmax_s = 0;
-    __syncthreads();
-    if (tid < BM && m0 + tid < M) atomicMax(&kmax_s, a.nl[(m0 + tid) / (2 * R)]);
-    __syncthreads();
-    const int kmax = kmax_s;
+    const int m0 = ty * BM, n0 = tx * BN;
+    // K extent: the longest l series among the tile's frequencies
+    int kmax = 0;
+    {
+        const int s0 = m0 / (2 * R), s1 = min(M - 1, m0 + BM - 1) / (2 * R);
+        for (int sl = s0; sl <= s1; sl++) kmax = max(kmax, a.nl[sl]);
+    }
+    const int nk = (kmax + BK - 1) / BK;
+    const int niter = job.nb * nk;
     const double *tab_mu = a.tab + (size_t)(job.mu - 1) * 6 * a.lcap * a.theta_pad;
     const size_t plane = (size_t)a.lcap * a.theta_pad;
+    const int wr = wave >> 2, wc = wave & 3;
 
-    v4d acc[2][2];
-    for (int i = 0; i < 2; i++)
-        for (int j = 0; j < 2; j++) acc[i][j] = (v4d){0.0, 0.0, 0.0, 0.0};
-    const int wr = wave >> 1, wc = wave & 1;
-
-    for (int jb = 0; jb < job.nb; jb++) {
-        const double2 *Eq = a.E + (size_t)(job.q0 + jb) * R * a.lcap;
-        const double *T = tab_mu + (size_t)job.b[jb] * plane;
-        for (int l0 = 0; l0 < kmax; l0 += BK) {
-            // A tile: 32 complex rows x 16 l
-            for (int u = 0; u < 2; u++) {
-                const int idx = tid + 256 * u;
-                const int pr = idx >> 4, kk = idx & 15;
-                const int prow = (m0 >> 1) + pr;  // (slot, r) pair index
-                double2 v = make_double2(0.0, 0.0);
-                if (2 * prow < M) {
-                    const int sl = prow / R, r = prow - sl * R;
-                    v = Eq[((size_t)sl * a.nq * R + r) * a.lcap + l0 + kk];
-                }
-                As[kk][2 * pr] = v.x;
-                As[kk][2 * pr + 1] = v.y;
-            }
-            // B tile: 16 l x 64 theta
-            for (int u = 0; u < 2; u++) {
-                const int idx = (tid + 256 * u) * 2;
-                const int kk = idx >> 6, c = idx & 63;
-                const double2 v = *(const double2 *)(T + (size_t)(l0 + kk) * a.theta_pad + n0 + c);
-                Bs[kk][c] = v.x;
-                Bs[kk][c + 1] = v.y;
-            }
-            __syncthreads();
+    v4d acc[4][2];
 #pragma unroll
-            for (int k4 = 0; k4 < BK; k4 += 4) {
-                const int kk = k4 + (lane >> 4);
-                double av[2], bv[2];
-                for (int i = 0; i < 2; i++) av[i] = As[kk][wr * 32 + i * 16 + (lane & 15)];
-                for (int j = 0; j < 2; j++) bv[j] = Bs[kk][wc * 32 + j * 16 + (lane & 15)];
-                for (int i = 0; i < 2; i++)
-                    for (int j = 0; j < 2; j++)
-                        acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[i], bv[j], acc[i][j], 0, 0, 0);
-            }
-            __syncthreads();
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++) acc[i][j] = (v4d){0.0, 0.0, 0.0, 0.0};
+
+    TileLoad tl;
+    if (niter > 0) {
+        load_tile(a, job, tab_mu, plane, m0, n0, M, R, 0, nk, tid, tl);
+        store_tile(As[0], Bs[0], tid, tl);
+    }
+    __syncthreads();
+    for (int it = 0; it < niter; it++) {
+        const int st = it & 1;
+        if (it + 1 < niter) load_tile(a, job, tab_mu, plane, m0, n0, M, R, it + 1, nk, tid, tl);
+#pragma unroll
+        for (int k4 = 0; k4 < BK; k4 += 4) {
+            const int kk = k4 + (lane >> 4);
+            double av[4], bv[2];
+#pragma unroll
+            for (int i = 0; i < 4; i++) av[i] = As[st][kk][wr * 64 + i * 16 + (lane & 15)];
+#pragma unroll
+            for (int j = 0; j < 2; j++) bv[j] = Bs[st][kk][wc * 32 + j * 16 + (lane & 15)];
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+#pragma unroll
+                for (int j = 0; j < 2; j++)
+                    acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[i], bv[j], acc[i][j], 0, 0, 0);
         }
+        if (it + 1 < niter) store_tile(As[st ^ 1], Bs[st ^ 1], tid, tl);
+        __syncthreads();
     }
     // epilogue: row = (slot, r, re|im) -> out[pos][r][theta][comp].
     // v_mfma_f64_16x16x4f64 leaves D[i][j] in lane j + 16*(i%4), register i/4.
     double *out = (double *)job.out;
-    for (int i = 0; i < 2; i++)
-        for (int j = 0; j < 2; j++)
-            for (int q = 0; q < 4; q++) {
-                const int row = m0 + wr * 32 + i * 16 + (lane >> 4) + 4 * q;
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int row = m0 + wr * 64 + i * 16 + (lane >> 4) + 4 * q;
+            if (row >= M) continue;
+            const int pr = row >> 1, reim = row & 1;
+            const int sl = pr / R, r = pr - sl * R;
+            const size_t base = ((size_t)a.slot_out[sl] * R + r) * a.theta_n;
+#pragma unroll
+            for (int j = 0; j < 2; j++) {
                 const int col = n0 + wc * 32 + j * 16 + (lane & 15);
-                if (row < M && col < a.theta_n) {
-                    const int pr = row >> 1, reim = row & 1;
-                    const int sl = pr / R, r = pr - sl * R;
-                    const size_t o = (((size_t)a.slot_out[sl] * R + r) * a.theta_n + col) * job.ncomp + job.comp;
-                    out[2 * o + reim] = acc[i][j][q];
-                }
+                if (col < a.theta_n) out[2 * ((base + col) * job.ncomp + job.comp) + reim] = acc[i][j][q];
             }
+        }
 }
 
 void launch_synth_f64(const SynthArgs &a, hipStream_t s)
 {
     const int M = a.nslot * a.r_eff * 2;
-    dim3 grid(a.theta_pad / BN, (M + BM - 1) / BM, a.njobs);
-    hipLaunchKernelGGL(synth_f64_kernel, grid, dim3(256), 0, s, a);
+    const int nx = a.theta_pad / BN, ny = (M + BM - 1) / BM;
+    const dim3 grid(8 * ((nx * ny + 7) / 8) * a.njobs);
+    if (a.r_eff > 1) hipLaunchKernelGGL(synth_f64_kernel<0>, grid, dim3(NT), 0, s, a, nx, ny);
+    else hipLaunchKernelGGL(synth_f64_kernel<1>, grid, dim3(NT), 0, s, a, nx, ny);
 }
 
 }  // namespace dsm

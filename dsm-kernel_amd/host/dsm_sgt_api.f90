@@ -1,0 +1,78 @@
+! dsm_sgt_api.f90 -- iso_c_binding interface to include/dsm_sgt.h.
+module dsm_sgt_api
+  use iso_c_binding
+  implicit none
+  integer, parameter :: DSM_SGT_MAXZONE = 64
+  integer(c_int32_t), parameter :: DSM_SGT_SH = 1, DSM_SGT_PSV = 2
+  integer(c_int32_t), parameter :: DSM_SGT_FP64 = 0, DSM_SGT_FP32 = 1, DSM_SGT_BF16X3 = 2
+
+  ! dsm_sgt_input_t
+  type, bind(c) :: dsm_sgt_input
+     integer(c_int32_t) :: nzone, maxlmax, imin, imax
+     integer(c_int32_t) :: rsgtswitch, tsgtswitch, synnswitch, psgtswitch
+     real(c_double) :: vrmin(DSM_SGT_MAXZONE), vrmax(DSM_SGT_MAXZONE)
+     real(c_double) :: rho(4, DSM_SGT_MAXZONE), vpv(4, DSM_SGT_MAXZONE), vph(4, DSM_SGT_MAXZONE)
+     real(c_double) :: vsv(4, DSM_SGT_MAXZONE), vsh(4, DSM_SGT_MAXZONE), eta(4, DSM_SGT_MAXZONE)
+     real(c_double) :: qmu(DSM_SGT_MAXZONE), qkappa(DSM_SGT_MAXZONE)
+     real(c_double) :: re, ratc, ratl, omegai_seed, tlen
+     real(c_double) :: rmin_, rmax_, rdelta_, r0min
+     real(c_double) :: thetamin, thetamax, thetadelta
+  end type dsm_sgt_input
+
+  interface
+     integer(c_int32_t) function dsm_sgt_create(inp, mode, precision, device, handle) bind(c)
+       import :: c_int32_t, c_ptr, dsm_sgt_input
+       type(dsm_sgt_input), intent(in) :: inp
+       integer(c_int32_t), value :: mode, precision, device
+       type(c_ptr), intent(out) :: handle
+     end function dsm_sgt_create
+     integer(c_int32_t) function dsm_sgt_run(handle, i0, i1) bind(c)
+       import :: c_int32_t, c_ptr
+       type(c_ptr), value :: handle
+       integer(c_int32_t), value :: i0, i1
+     end function dsm_sgt_run
+     integer(c_int32_t) function dsm_sgt_lcut(handle, i, llog) bind(c)
+       import :: c_int32_t, c_ptr
+       type(c_ptr), value :: handle
+       integer(c_int32_t), value :: i
+       integer(c_int32_t), intent(out) :: llog
+     end function dsm_sgt_lcut
+     integer(c_int32_t) function dsm_sgt_write_catalog(handle, i, outdir, model) bind(c)
+       import :: c_int32_t, c_ptr, c_char
+       type(c_ptr), value :: handle
+       integer(c_int32_t), value :: i
+       character(kind=c_char), dimension(*), intent(in) :: outdir, model
+     end function dsm_sgt_write_catalog
+     type(c_ptr) function dsm_sgt_last_error(handle) bind(c)
+       import :: c_ptr
+       type(c_ptr), value :: handle
+     end function dsm_sgt_last_error
+     subroutine dsm_sgt_destroy(handle) bind(c)
+       import :: c_ptr
+       type(c_ptr), value :: handle
+     end subroutine dsm_sgt_destroy
+     integer(c_size_t) function c_strlen(s) bind(c, name='strlen')
+       import :: c_size_t, c_ptr
+       type(c_ptr), value :: s
+     end function c_strlen
+  end interface
+
+contains
+
+  ! last error message of a handle as a Fortran string
+  function dsm_sgt_message(handle) result(msg)
+    type(c_ptr), intent(in) :: handle
+    character(len=:), allocatable :: msg
+    character(kind=c_char), pointer :: p(:)
+    type(c_ptr) :: s
+    integer :: n, k
+    s = dsm_sgt_last_error(handle)
+    n = int(c_strlen(s))
+    call c_f_pointer(s, p, [n])
+    allocate(character(len=n) :: msg)
+    do k = 1, n
+       msg(k:k) = p(k)
+    enddo
+  end function dsm_sgt_message
+
+end module dsm_sgt_api
