@@ -62,4 +62,21 @@ int orc_sh_omega(const orc_input_t *in, int32_t i, int32_t use_ref,
                  double complex *tsgt, double complex *rsgt,
                  double complex *synn, int32_t *llog);
 
+/* Sizes of one P-SV run (SGTpsv-MPI.f90:120-240; no receiver trim). */
+typedef struct {
+    int32_t r_n, theta_n, nn, nzone;
+    double r0, spo;
+    int32_t spn, intir0, nslay, nllay;
+} orc_psv_info_t;
+
+int orc_psv_info(const orc_input_t *in, orc_psv_info_t *info);
+int orc_psv_load_ref(const char *path);
+int orc_psv_mix(uint64_t mask);
+/* One frequency of SGTpsv (SGTpsv-MPI.f90:542-1084): FP64 accumulators
+ * tsgt(20,r_n,theta_n), rsgt(10,r_n,theta_n), synn(10,theta_n); llog is
+ * the logged value llog-1 (SGTpsv-MPI.f90:1033). */
+int orc_psv_omega(const orc_input_t *in, int32_t i, int32_t use_ref,
+                  double complex *tsgt, double complex *rsgt,
+                  double complex *synn, int32_t *llog);
+
 #endif

@@ -20,6 +20,13 @@ class ShInfo(C.Structure):
                 ("spn", C.c_int32), ("intir0", C.c_int32)]
 
 
+class PsvInfo(C.Structure):
+    _fields_ = [("r_n", C.c_int32), ("theta_n", C.c_int32), ("nn", C.c_int32),
+                ("nzone", C.c_int32), ("r0", C.c_double), ("spo", C.c_double),
+                ("spn", C.c_int32), ("intir0", C.c_int32), ("nslay", C.c_int32),
+                ("nllay", C.c_int32)]
+
+
 _LIBS = {}
 
 
@@ -34,6 +41,11 @@ def lib(fast=False):
         L.orc_sh_load_ref.argtypes = [C.c_char_p]
         L.orc_sh_omega.argtypes = [C.POINTER(DsmInput), C.c_int32, C.c_int32,
                                    C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_int32)]
+        L.orc_psv_info.argtypes = [C.POINTER(DsmInput), C.POINTER(PsvInfo)]
+        L.orc_psv_load_ref.argtypes = [C.c_char_p]
+        L.orc_psv_mix.argtypes = [C.c_uint64]
+        L.orc_psv_omega.argtypes = [C.POINTER(DsmInput), C.c_int32, C.c_int32,
+                                    C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_int32)]
         _LIBS[name] = L
     return _LIBS[name]
 
@@ -72,4 +84,41 @@ def sh_omega(inp, i, use_ref=False, fast=False):
                         synn.ctypes.data, C.byref(llog))
     if rc:
         raise RuntimeError(f"orc_sh_omega rc={rc}")
+    return tsgt, rsgt, synn, llog.value
+
+
+def psv_info(inp):
+    info = PsvInfo()
+    rc = lib().orc_psv_info(C.byref(inp), C.byref(info))
+    if rc:
+        raise RuntimeError(f"orc_psv_info rc={rc}")
+    return info
+
+
+def psv_omega(inp, i, use_ref=False, fast=False, mix=None):
+    """One SGTpsv frequency: FP64 accumulators (tsgt[theta][r][20],
+    rsgt[theta][r][10], synn[theta][10]) and the logged l cut-off.
+    use_ref: the reference's routines for every call; mix: a bit mask
+    selecting the reference routine per entry of the oracle's table."""
+    L = lib(fast)
+    if use_ref or mix is not None:
+        rc = L.orc_psv_load_ref(ref_path("psv").encode())
+        if rc:
+            raise RuntimeError(f"cannot load reference routines rc={rc}")
+    mode = 1 if use_ref else 0
+    if mix is not None:
+        rc = L.orc_psv_mix(mix)
+        if rc:
+            raise RuntimeError(f"orc_psv_mix rc={rc}")
+        mode = 2
+    info = psv_info(inp)
+    rn, tn = info.r_n, info.theta_n
+    tsgt = np.zeros((tn, rn, 20), np.complex128)
+    rsgt = np.zeros((tn, rn, 10), np.complex128)
+    synn = np.zeros((tn, 10), np.complex128)
+    llog = C.c_int32()
+    rc = L.orc_psv_omega(C.byref(inp), i, mode, tsgt.ctypes.data, rsgt.ctypes.data,
+                         synn.ctypes.data, C.byref(llog))
+    if rc:
+        raise RuntimeError(f"orc_psv_omega rc={rc}")
     return tsgt, rsgt, synn, llog.value
