@@ -1,11 +1,12 @@
 // engine.cpp -- the SGT engine behind the C-ABI of include/dsm_sgt.h.
 //
 // Pipeline for a batch of frequencies (one HIP stream):
-//   host   : per-frequency band pieces a0/a2 (fe_setup_sh.cpp)
+//   host   : per-frequency band pieces (fe_setup_sh.cpp / fe_setup_psv.cpp)
+//   [P-SV]   psv_l0    : the l = 0 system and its coefficient rows
 //   loop over l chunks until every frequency of the batch is cut off:
-//     sh_solve  : (omega,l) band LDL^T + basis solves        -> stencil values
-//     sh_coef   : excitation, interpolation, strain maps     -> contraction rows, |g0(nn)|
-//     host      : replay of the reference's l cut-off (calamp) on |g0(nn)|
+//     *_solve  : (omega,l) band LDL^T + the distinct right-hand sides -> stencil values
+//     *_coef   : interpolation, strain maps -> contraction rows, cut-off amplitudes
+//     host     : replay of the reference's l cut-off (calamp) on the amplitudes
 //   zero_tail  : drop the rows past each frequency's cut-off
 //   synth      : FP64 MFMA contraction over l into the FP64 catalog in HBM
 #include <hip/hip_runtime.h>
@@ -14,6 +15,7 @@
 #include <cmath>
 #include <complex>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <sys/stat.h>
@@ -22,6 +24,8 @@
 #include "dsm_sgt.h"
 #include "fe_setup.h"
 #include "kernels.h"
+#include "legendre_lane.h"
+#include "psv_jobs.h"
 #include "sh_jobs.h"
 
 using namespace dsm;
@@ -63,6 +67,14 @@ struct dsm_sgt {
     std::string err;
     dsm_sgt_input_t in{};
     ShSetup sh;
+    PsvSetup psv;
+    PsvGeom pg;
+    // run geometry (either mode)
+    bool ready = false;
+    int r_n = 0, theta_n = 0, nn = 0, intir0 = 0;
+    int ncomp[3] = {0, 0, 0};
+    double r0 = 0;
+    std::vector<double> radii, theta_deg;
     // geometry of the device arrays
     int theta_pad = 0, lcap = 0, nneed = 0, nslot = 0, lchunk = 0;
     int ipos0 = 0, npos = 0;  // catalog covers frequencies [ipos0, ipos0+npos)
@@ -73,8 +85,10 @@ struct dsm_sgt {
     DevBuf d_theta, d_tab, d_need, d_recslot, d_w0, d_w1, d_rinv, d_src;
     DevBuf d_band, d_om, d_slots, d_nl, d_slotout, d_scratch, d_xn, d_E, d_Es, d_amp, d_jobs;
     DevBuf d_cat_t, d_cat_r, d_cat_s;
+    // P-SV only
+    DevBuf d_kind, d_fluid, d_irf, d_fluidK, d_noderow, d_nodec1, d_jsta0, d_l0work;
     std::vector<SynthJob> jobs;
-    int njobs_main = 0, njobs_syn = 0;
+    int njobs_main = 0, njobs_syn = 0, njobs_acc = 0;  // launch order: main, accumulate, SYNN
     // timing
     bool timing = true;
     std::vector<TimedLaunch> pending;
@@ -136,15 +150,51 @@ static void sh_synth_jobs(dsm_sgt_t *h)
         SynthJob j{};
         j.q0 = d.q0;
         j.nb = d.nb;
-        j.mu = d.mu;
-        for (int t = 0; t < 4; t++) j.b[t] = d.b[t];
-        j.comp = d.comp;
+        for (int t = 0; t < 4; t++) j.b[t] = (d.mu - 1) * 6 + d.b[t];
+        j.comp[0] = d.comp;
+        j.comp[1] = -1;
+        j.sign[0] = 1.0;
+        j.sign[1] = 0.0;
+        j.accumulate = 0;
         j.ncomp = ncomp[d.kind];
         j.out = (double2 *)outs[d.kind];
         h->jobs.push_back(j);
         if (d.kind < 2) h->njobs_main++;
     }
     h->njobs_syn = (int)h->jobs.size() - h->njobs_main;
+    h->njobs_acc = 0;
+}
+
+// Synthesis jobs of the P-SV contraction (table in psv_jobs.h), ordered
+// main (pass 0, TSGT/RSGT), accumulate (pass 1), SYNN.
+static void psv_synth_jobs(dsm_sgt_t *h)
+{
+    h->jobs.clear();
+    void *outs[3] = {h->d_cat_t.p, h->d_cat_r.p, h->d_cat_s.p};
+    const int ncomp[3] = {20, 10, 10};
+    int counts[3] = {0, 0, 0};
+    for (int grp = 0; grp < 3; grp++)
+        for (int k = 0; k < PSV_NJOBS; k++) {
+            const PsvJobDef &d = PSV_JOBS[k];
+            const int g = (d.kind == 2) ? 2 : d.pass;
+            if (g != grp) continue;
+            SynthJob j{};
+            j.q0 = d.q0;
+            j.nb = d.nb;
+            for (int t = 0; t < 4; t++) j.b[t] = psv_plane(d.mu, d.f[t]);
+            j.comp[0] = d.comp0;
+            j.comp[1] = d.comp1;
+            j.sign[0] = (double)d.sign0;
+            j.sign[1] = (double)d.sign1;
+            j.accumulate = d.pass;
+            j.ncomp = ncomp[d.kind];
+            j.out = (double2 *)outs[d.kind];
+            h->jobs.push_back(j);
+            counts[grp]++;
+        }
+    h->njobs_main = counts[0];
+    h->njobs_acc = counts[1];
+    h->njobs_syn = counts[2];
 }
 
 static int sh_create(dsm_sgt_t *h)
@@ -153,6 +203,16 @@ static int sh_create(dsm_sgt_t *h)
     if (rc) return rc;
     const ShSetup &s = h->sh;
     const int R = s.r_n;
+    h->r_n = R;
+    h->theta_n = s.theta_n;
+    h->nn = s.nn;
+    h->intir0 = s.intir0;
+    h->r0 = s.r0;
+    h->radii = s.r_;
+    h->theta_deg = s.theta_deg;
+    h->ncomp[0] = 10;
+    h->ncomp[1] = 5;
+    h->ncomp[2] = 5;
     h->theta_pad = ((s.theta_n + 127) / 128) * 128;  // synthesis N tile
     h->lcap = ((s.maxlmax + 1 + 15) / 16) * 16;
     // stencil nodes, descending, always including the surface node nn
@@ -245,6 +305,7 @@ static int sh_create(dsm_sgt_t *h)
     ev_end(h, t);
     HIPCHK(h, hipStreamSynchronize(h->stream));
     ev_collect(h);
+    h->ready = true;
     return DSM_SGT_OK;
 }
 
@@ -431,6 +492,327 @@ static int sh_run_batch(dsm_sgt_t *h, const std::vector<int> &freqs)
 }
 
 // ------------------------------------------------------------------------
+// P-SV
+// ------------------------------------------------------------------------
+static int psv_create(dsm_sgt_t *h)
+{
+    int rc = psv_setup(h->in, h->psv, h->err);
+    if (rc) return rc;
+    const PsvSetup &s = h->psv;
+    h->pg = psv_geometry(s);
+    const PsvGeom &g = h->pg;
+    const int R = s.r_n;
+    h->r_n = R;
+    h->theta_n = s.theta_n;
+    h->nn = s.nn;
+    h->intir0 = s.intir0;
+    h->r0 = s.r0;
+    h->radii = s.r_;
+    h->theta_deg = s.theta_deg;
+    h->ncomp[0] = 20;
+    h->ncomp[1] = 10;
+    h->ncomp[2] = 10;
+    h->theta_pad = ((s.theta_n + 127) / 128) * 128;
+    h->lcap = ((s.maxlmax + 1 + 15) / 16) * 16;
+    h->nneed = (int)g.need.size();
+    h->nslot = 32;
+    // one solve launch covers the whole l range unless told otherwise
+    h->lchunk = s.maxlmax + 1;
+    if (const char *e = std::getenv("DSM_SGT_LCHUNK")) h->lchunk = std::max(64, std::atoi(e));
+    h->ipos0 = std::max(0, h->in.imin);
+    h->npos = std::max(0, h->in.imax - h->ipos0 + 1);
+    h->llog.assign(h->npos, 0);
+    h->done.assign(h->npos, 0);
+
+    std::vector<double> th(s.theta_n);
+    for (int i = 0; i < s.theta_n; i++) th[i] = s.theta_deg[i] / 180.0 * PI;
+    std::vector<int> kind(s.nn);
+    psv_pack_kind(s, kind.data());
+    const int S = h->nslot, nn = s.nn, LC = h->lcap, N0 = s.nn0;
+    HIPCHK(h, h->d_theta.ensure(sizeof(double) * s.theta_n));
+    HIPCHK(h, h->d_tab.ensure(sizeof(double) * PSV_NPLANES * (size_t)LC * h->theta_pad));
+    HIPCHK(h, h->d_need.ensure(sizeof(int) * h->nneed));
+    HIPCHK(h, h->d_recslot.ensure(sizeof(int) * 6 * R));
+    HIPCHK(h, h->d_w0.ensure(sizeof(double) * 3 * R));
+    HIPCHK(h, h->d_w1.ensure(sizeof(double) * 3 * R));
+    HIPCHK(h, h->d_irf.ensure(sizeof(double) * R));
+    HIPCHK(h, h->d_fluid.ensure(sizeof(int) * R));
+    HIPCHK(h, h->d_kind.ensure(sizeof(int) * nn));
+    HIPCHK(h, h->d_noderow.ensure(sizeof(int) * N0));
+    HIPCHK(h, h->d_nodec1.ensure(sizeof(int) * N0));
+    HIPCHK(h, h->d_jsta0.ensure(sizeof(int) * R));
+    HIPCHK(h, hipMemcpy(h->d_theta.p, th.data(), sizeof(double) * s.theta_n, hipMemcpyHostToDevice));
+    HIPCHK(h, hipMemcpy(h->d_need.p, g.need.data(), sizeof(int) * h->nneed, hipMemcpyHostToDevice));
+    HIPCHK(h, hipMemcpy(h->d_recslot.p, g.rec_row.data(), sizeof(int) * 6 * R, hipMemcpyHostToDevice));
+    HIPCHK(h, hipMemcpy(h->d_w0.p, s.w0.data(), sizeof(double) * 3 * R, hipMemcpyHostToDevice));
+    HIPCHK(h, hipMemcpy(h->d_w1.p, s.w1.data(), sizeof(double) * 3 * R, hipMemcpyHostToDevice));
+    HIPCHK(h, hipMemcpy(h->d_irf.p, g.irf.data(), sizeof(double) * R, hipMemcpyHostToDevice));
+    HIPCHK(h, hipMemcpy(h->d_fluid.p, s.fluid_rec.data(), sizeof(int) * R, hipMemcpyHostToDevice));
+    HIPCHK(h, hipMemcpy(h->d_kind.p, kind.data(), sizeof(int) * nn, hipMemcpyHostToDevice));
+    HIPCHK(h, hipMemcpy(h->d_noderow.p, s.node_row.data(), sizeof(int) * N0, hipMemcpyHostToDevice));
+    HIPCHK(h, hipMemcpy(h->d_nodec1.p, s.node_c1.data(), sizeof(int) * N0, hipMemcpyHostToDevice));
+    HIPCHK(h, hipMemcpy(h->d_jsta0.p, s.jsta0.data(), sizeof(int) * R, hipMemcpyHostToDevice));
+
+    HIPCHK(h, h->d_band.ensure(sizeof(double2) * 16 * (size_t)nn * S));
+    HIPCHK(h, h->d_src.ensure(sizeof(PsvSrc) * S));
+    HIPCHK(h, h->d_fluidK.ensure(sizeof(double2) * (size_t)S * R));
+    HIPCHK(h, h->d_slots.ensure(sizeof(int) * S));
+    HIPCHK(h, h->d_nl.ensure(sizeof(int) * S));
+    HIPCHK(h, h->d_slotout.ensure(sizeof(int) * S));
+    HIPCHK(h, h->d_l0work.ensure(sizeof(double2) * 5 * (size_t)N0 * S));
+    const int n = nn - s.row0 + 1, klo = g.rowlo - s.row0 + 1, fs = s.jtmp - s.row0 + 1;
+    const int lch = std::min(h->lchunk, s.maxlmax + 1);
+    const size_t nthreads = (size_t)S * ((lch + 63) / 64 + 1) * 64;
+    HIPCHK(h, h->d_scratch.ensure(sizeof(double2) * (size_t)(3 * (n - klo + 1) + 4 * (n - fs + 1)) * nthreads));
+    HIPCHK(h, h->d_xn.ensure(sizeof(double2) * (size_t)S * h->nneed * PSV_NSOL * LC));
+    HIPCHK(h, h->d_E.ensure(sizeof(double2) * (size_t)S * PSV_NQ * R * LC));
+    HIPCHK(h, h->d_Es.ensure(sizeof(double2) * (size_t)S * PSV_NQS * LC));
+    HIPCHK(h, h->d_amp.ensure(sizeof(double2) * 2 * (size_t)S * LC));
+    const size_t np = (size_t)std::max(1, h->npos);
+    HIPCHK(h, h->d_cat_t.ensure(sizeof(double2) * np * R * s.theta_n * 20));
+    HIPCHK(h, h->d_cat_r.ensure(sizeof(double2) * np * R * s.theta_n * 10));
+    HIPCHK(h, h->d_cat_s.ensure(sizeof(double2) * np * s.theta_n * 10));
+    HIPCHK(h, hipMemsetAsync(h->d_cat_t.p, 0, h->d_cat_t.n, h->stream));
+    HIPCHK(h, hipMemsetAsync(h->d_cat_r.p, 0, h->d_cat_r.n, h->stream));
+    HIPCHK(h, hipMemsetAsync(h->d_cat_s.p, 0, h->d_cat_s.n, h->stream));
+    psv_synth_jobs(h);
+    HIPCHK(h, h->d_jobs.ensure(sizeof(SynthJob) * h->jobs.size()));
+    HIPCHK(h, hipMemcpy(h->d_jobs.p, h->jobs.data(), sizeof(SynthJob) * h->jobs.size(), hipMemcpyHostToDevice));
+
+    TimedLaunch t;
+    ev_begin(h, t, 1);
+    launch_legendre_psv(h->d_theta.as<double>(), s.theta_n, h->theta_pad, s.maxlmax, LC, h->d_tab.as<double>(),
+                        h->stream);
+    HIPCHK(h, hipGetLastError());
+    ev_end(h, t);
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    ev_collect(h);
+    h->ready = true;
+    return DSM_SGT_OK;
+}
+
+static int psv_run_batch(dsm_sgt_t *h, const std::vector<int> &freqs)
+{
+    const PsvSetup &s = h->psv;
+    const PsvGeom &g = h->pg;
+    const int S = (int)freqs.size(), nn = s.nn, R = s.r_n, LC = h->lcap;
+    std::vector<double2> band(16 * (size_t)nn * S), fk((size_t)S * R);
+    std::vector<PsvSrc> src(S);
+    std::vector<CutState> cut(S);
+    std::vector<int> slotout(S);
+    for (int k = 0; k < S; k++) {
+        PsvOmega o;
+        psv_omega_setup(s, freqs[k], o);
+        psv_pack_band(s, o, (double *)(band.data() + 16 * (size_t)nn * k));
+        for (int q = 0; q < 160; q++) {
+            src[k].anum[q] = make_double2(o.anum[q].real(), o.anum[q].imag());
+            src[k].bnum[q] = make_double2(o.bnum[q].real(), o.bnum[q].imag());
+        }
+        src[k].coef1 = make_double2(o.coef1_spn.real(), o.coef1_spn.imag());
+        src[k].coef2 = make_double2(o.coef2_spn.real(), o.coef2_spn.imag());
+        for (int r = 0; r < R; r++) fk[(size_t)k * R + r] = make_double2(o.fluidK[r].real(), o.fluidK[r].imag());
+        cut[k].lsuf = o.lsuf;
+        cut[k].llog = s.maxlmax;
+        slotout[k] = freqs[k] - h->ipos0;
+    }
+    HIPCHK(h, hipMemcpyAsync(h->d_band.p, band.data(), sizeof(double2) * band.size(), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipMemcpyAsync(h->d_src.p, src.data(), sizeof(PsvSrc) * S, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipMemcpyAsync(h->d_fluidK.p, fk.data(), sizeof(double2) * fk.size(), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipMemcpyAsync(h->d_slotout.p, slotout.data(), sizeof(int) * S, hipMemcpyHostToDevice, h->stream));
+    std::vector<int> active(S);
+    for (int k = 0; k < S; k++) active[k] = k;
+    HIPCHK(h, hipMemcpyAsync(h->d_slots.p, active.data(), sizeof(int) * S, hipMemcpyHostToDevice, h->stream));
+    const PsvSrcConst sc{s.ecC0, s.ecF0, s.ecL0, s.r0, s.ra[s.mtmp - 1], s.ra[s.mtmp]};
+    // l = 0 of every slot
+    {
+        PsvL0Args za{};
+        za.band = h->d_band.as<double2>();
+        za.kind = h->d_kind.as<int>();
+        za.slot_list = h->d_slots.as<int>();
+        za.nslots = S;
+        za.nn = nn;
+        za.nn0 = s.nn0;
+        za.row0_0 = s.row0_0;
+        za.jtmp = s.jtmp;
+        za.node_row = h->d_noderow.as<int>();
+        za.node_c1 = h->d_nodec1.as<int>();
+        za.work = h->d_l0work.as<double2>();
+        za.r_n = R;
+        za.jsta0 = h->d_jsta0.as<int>();
+        za.fluid = h->d_fluid.as<int>();
+        za.w0 = h->d_w0.as<double>();
+        za.w1 = h->d_w1.as<double>();
+        za.irf = h->d_irf.as<double>();
+        za.fluidK = h->d_fluidK.as<double2>();
+        za.src = h->d_src.as<PsvSrc>();
+        za.sc = sc;
+        za.E = h->d_E.as<double2>();
+        za.Es = h->d_Es.as<double2>();
+        za.lcap = LC;
+        TimedLaunch t0;
+        ev_begin(h, t0, 0);
+        launch_psv_l0(za, h->stream);
+        HIPCHK(h, hipGetLastError());
+        ev_end(h, t0);
+    }
+    const int n = nn - s.row0 + 1, klo = g.rowlo - s.row0 + 1, fs = s.jtmp - s.row0 + 1;
+    std::vector<double2> ampuv((size_t)S * LC * 2);
+    int l0 = 0;
+    while (!active.empty()) {
+        const int lend = std::min(l0 + h->lchunk - 1, s.maxlmax);
+        const int nlb = (lend - l0 + 64) / 64;
+        const int nblocks = (int)active.size() * nlb;
+        HIPCHK(h, hipMemcpyAsync(h->d_slots.p, active.data(), sizeof(int) * active.size(), hipMemcpyHostToDevice,
+                                 h->stream));
+        PsvSolveArgs sa{};
+        sa.band = h->d_band.as<double2>();
+        sa.kind = h->d_kind.as<int>();
+        sa.slot_list = h->d_slots.as<int>();
+        sa.nlb = nlb;
+        sa.l0 = l0;
+        sa.lend = lend;
+        sa.nn = nn;
+        sa.row0 = s.row0;
+        sa.jtmp = s.jtmp;
+        sa.rowlo = g.rowlo;
+        sa.nneed = h->nneed;
+        sa.need = h->d_need.as<int>();
+        sa.scratch = h->d_scratch.as<double2>();
+        sa.sstride = (long)nblocks * 64;
+        sa.xn = h->d_xn.as<double2>();
+        sa.lcap = LC;
+        sa.src = h->d_src.as<PsvSrc>();
+        sa.sc = sc;
+        TimedLaunch t0;
+        ev_begin(h, t0, 0);
+        launch_psv_solve(sa, nblocks, h->stream);
+        HIPCHK(h, hipGetLastError());
+        PsvCoefArgs ca{};
+        ca.xn = h->d_xn.as<double2>();
+        ca.slot_list = h->d_slots.as<int>();
+        ca.nlb = nlb;
+        ca.l0 = l0;
+        ca.lend = lend;
+        ca.lcap = LC;
+        ca.nneed = h->nneed;
+        ca.r_n = R;
+        ca.rec_row = h->d_recslot.as<int>();
+        ca.fluid = h->d_fluid.as<int>();
+        ca.w0 = h->d_w0.as<double>();
+        ca.w1 = h->d_w1.as<double>();
+        ca.irf = h->d_irf.as<double>();
+        ca.fluidK = h->d_fluidK.as<double2>();
+        ca.amp_i0 = g.amp_i0;
+        ca.amp_i1 = g.amp_i1;
+        ca.syn_i0 = g.syn_i0;
+        ca.syn_i1 = g.syn_i1;
+        ca.E = h->d_E.as<double2>();
+        ca.Es = h->d_Es.as<double2>();
+        ca.ampuv = h->d_amp.as<double2>();
+        launch_psv_coef(ca, nblocks, h->stream);
+        HIPCHK(h, hipGetLastError());
+        ev_end(h, t0);
+        h->stats.solve_launches++;
+        const long nsys = (long)active.size() * (lend - std::max(l0, 1) + 1);
+        h->stats.systems += nsys;
+        // bytes of the design: factor rows L (3 complex) written and read
+        // back from the deepest receiver row up, forward values of the 4
+        // source solutions from the source row up, 6 solutions at the
+        // recorded rows
+        h->stats.solve_bytes += (double)nsys * 16.0 * (2.0 * 3 * (n - klo + 1) + 2.0 * 4 * (n - fs + 1) +
+                                                       (double)PSV_NSOL * h->nneed);
+        HIPCHK(h, hipMemcpy2DAsync(&ampuv[2 * (size_t)l0], sizeof(double2) * 2 * LC,
+                                   h->d_amp.as<double2>() + 2 * (size_t)l0, sizeof(double2) * 2 * LC,
+                                   sizeof(double2) * 2 * (lend - l0 + 1), S, hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(h, hipStreamSynchronize(h->stream));
+        // replay the cut-off in the reference's order (SGTpsv-MPI.f90:616-634, 884-888)
+        std::vector<int> still;
+        for (int k : active) {
+            CutState &c = cut[k];
+            for (int l = l0; l <= lend && !c.done; l++) {
+                if (c.ismall > 20) {
+                    if (c.llog > l) c.llog = l;
+                    c.nl = l;
+                    c.done = true;
+                    break;
+                }
+                if (l == 0) continue;
+                const double2 u = ampuv[((size_t)k * LC + l) * 2], v = ampuv[((size_t)k * LC + l) * 2 + 1];
+                const double a1 = std::hypot(u.x, u.y), a2 = std::hypot(v.x, v.y);
+                const double a0 = std::sqrt(a1 * a1 + a2 * a2);
+                for (int m = -2; m <= 2; m++) {
+                    if (std::abs(m) > l) continue;
+                    calamp(c, m == 0 ? a0 : 0.0, l, s.ratl, false);
+                }
+            }
+            if (!c.done && lend >= s.maxlmax) {
+                c.nl = s.maxlmax + 1;
+                c.done = true;
+            }
+            if (!c.done) still.push_back(k);
+        }
+        active.swap(still);
+        l0 = lend + 1;
+    }
+    std::vector<int> nl(S), all(S);
+    for (int k = 0; k < S; k++) {
+        nl[k] = cut[k].nl;
+        all[k] = k;
+        h->llog[slotout[k]] = cut[k].llog - 1;
+        h->done[slotout[k]] = 1;
+    }
+    HIPCHK(h, hipMemcpyAsync(h->d_nl.p, nl.data(), sizeof(int) * S, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipMemcpyAsync(h->d_slots.p, all.data(), sizeof(int) * S, hipMemcpyHostToDevice, h->stream));
+    TimedLaunch tz;
+    ev_begin(h, tz, 3);
+    launch_zero_tail(h->d_E.as<double2>(), S, PSV_NQ * R, h->d_slots.as<int>(), h->d_nl.as<int>(), LC, h->stream);
+    launch_zero_tail(h->d_Es.as<double2>(), S, PSV_NQS, h->d_slots.as<int>(), h->d_nl.as<int>(), LC, h->stream);
+    HIPCHK(h, hipGetLastError());
+    ev_end(h, tz);
+
+    SynthArgs ga{};
+    ga.E = h->d_E.as<double2>();
+    ga.nq = PSV_NQ;
+    ga.r_eff = R;
+    ga.tab = h->d_tab.as<double>();
+    ga.theta_n = s.theta_n;
+    ga.theta_pad = h->theta_pad;
+    ga.lcap = LC;
+    ga.nslot = S;
+    ga.nl = h->d_nl.as<int>();
+    ga.slot_out = h->d_slotout.as<int>();
+    ga.jobs = h->d_jobs.as<SynthJob>();
+    ga.njobs = h->njobs_main;
+    TimedLaunch tg;
+    ev_begin(h, tg, 2);
+    launch_synth_f64(ga, h->stream);
+    HIPCHK(h, hipGetLastError());
+    ga.jobs = h->d_jobs.as<SynthJob>() + h->njobs_main;
+    ga.njobs = h->njobs_acc;
+    launch_synth_f64(ga, h->stream);
+    HIPCHK(h, hipGetLastError());
+    ev_end(h, tg);
+    ga.E = h->d_Es.as<double2>();
+    ga.nq = PSV_NQS;
+    ga.r_eff = 1;
+    ga.jobs = h->d_jobs.as<SynthJob>() + h->njobs_main + h->njobs_acc;
+    ga.njobs = h->njobs_syn;
+    TimedLaunch tg2;
+    ev_begin(h, tg2, 3);
+    launch_synth_f64(ga, h->stream);
+    HIPCHK(h, hipGetLastError());
+    ev_end(h, tg2);
+    h->stats.synth_launches += 2;
+    double sumL = 0.0;
+    for (int k = 0; k < S; k++) sumL += nl[k];
+    // 4 flops per (complex coefficient x real function) per (r,theta)
+    h->stats.synth_flops += 4.0 * sumL * s.theta_n * (double)PSV_NQ * R;
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    ev_collect(h);
+    return DSM_SGT_OK;
+}
+
+// ------------------------------------------------------------------------
 // C-ABI
 // ------------------------------------------------------------------------
 extern "C" {
@@ -448,8 +830,8 @@ int dsm_sgt_create(const dsm_sgt_input_t *in, int32_t mode, int32_t precision, i
     h->device = device;
     int rc = DSM_SGT_OK;
     do {
-        if (mode != DSM_SGT_SH) {
-            rc = fail(h, DSM_SGT_EUNSUPPORTED, "mode not available in this build");
+        if (mode != DSM_SGT_SH && mode != DSM_SGT_PSV) {
+            rc = fail(h, DSM_SGT_EINVAL, "unknown mode " + std::to_string(mode));
             break;
         }
         if (precision != DSM_SGT_FP64) {
@@ -469,7 +851,7 @@ int dsm_sgt_create(const dsm_sgt_input_t *in, int32_t mode, int32_t precision, i
             rc = fail(h, DSM_SGT_EHIP, "hipStreamCreate failed");
             break;
         }
-        rc = sh_create(h);
+        rc = (mode == DSM_SGT_SH) ? sh_create(h) : psv_create(h);
     } while (0);
     *handle = h;  // returned even on failure so dsm_sgt_last_error() can report
     return rc;
@@ -478,32 +860,32 @@ int dsm_sgt_create(const dsm_sgt_input_t *in, int32_t mode, int32_t precision, i
 int dsm_sgt_get_info(dsm_sgt_t *h, dsm_sgt_info_t *info)
 {
     if (!h || !info) return DSM_SGT_EINVAL;
-    if (h->sh.nn == 0) return fail(h, DSM_SGT_ESTATE, "engine not initialised");
+    if (!h->ready) return fail(h, DSM_SGT_ESTATE, "engine not initialised");
     info->mode = h->mode;
-    info->r_n = h->sh.r_n;
-    info->theta_n = h->sh.theta_n;
-    info->nn = h->sh.nn;
-    info->num_tsgt = 10;
-    info->num_rsgt = 5;
-    info->num_synn = 5;
-    info->intir0 = h->sh.intir0;
-    info->r0 = h->sh.r0;
+    info->r_n = h->r_n;
+    info->theta_n = h->theta_n;
+    info->nn = h->nn;
+    info->num_tsgt = h->ncomp[0];
+    info->num_rsgt = h->ncomp[1];
+    info->num_synn = h->ncomp[2];
+    info->intir0 = h->intir0;
+    info->r0 = h->r0;
     return DSM_SGT_OK;
 }
 
 int dsm_sgt_get_grid(dsm_sgt_t *h, double *radii, double *theta_deg)
 {
     if (!h) return DSM_SGT_EINVAL;
-    if (h->sh.nn == 0) return fail(h, DSM_SGT_ESTATE, "engine not initialised");
-    if (radii) std::memcpy(radii, h->sh.r_.data(), sizeof(double) * h->sh.r_n);
-    if (theta_deg) std::memcpy(theta_deg, h->sh.theta_deg.data(), sizeof(double) * h->sh.theta_n);
+    if (!h->ready) return fail(h, DSM_SGT_ESTATE, "engine not initialised");
+    if (radii) std::memcpy(radii, h->radii.data(), sizeof(double) * h->r_n);
+    if (theta_deg) std::memcpy(theta_deg, h->theta_deg.data(), sizeof(double) * h->theta_n);
     return DSM_SGT_OK;
 }
 
 int dsm_sgt_run(dsm_sgt_t *h, int32_t i_begin, int32_t i_end)
 {
     if (!h) return DSM_SGT_EINVAL;
-    if (h->sh.nn == 0) return fail(h, DSM_SGT_ESTATE, "engine not initialised");
+    if (!h->ready) return fail(h, DSM_SGT_ESTATE, "engine not initialised");
     if (i_begin < h->ipos0 || i_end >= h->ipos0 + h->npos || i_begin > i_end)
         return fail(h, DSM_SGT_EINVAL, "frequency range outside [imin, imax] of the input");
     if (hipSetDevice(h->device) != hipSuccess) return fail(h, DSM_SGT_EHIP, "hipSetDevice failed");
@@ -514,7 +896,7 @@ int dsm_sgt_run(dsm_sgt_t *h, int32_t i_begin, int32_t i_end)
         if (i != 0) all.push_back(i);
     for (size_t b = 0; b < all.size(); b += h->nslot) {
         std::vector<int> batch(all.begin() + b, all.begin() + std::min(all.size(), b + (size_t)h->nslot));
-        int rc = sh_run_batch(h, batch);
+        int rc = (h->mode == DSM_SGT_SH) ? sh_run_batch(h, batch) : psv_run_batch(h, batch);
         if (rc) return rc;
     }
     return DSM_SGT_OK;
@@ -526,18 +908,18 @@ static int fetch_common(dsm_sgt_t *h, int32_t i, int32_t kind, int32_t ir, std::
     const int pos = i - h->ipos0;
     if (pos < 0 || pos >= h->npos || !h->done[pos])
         return fail(h, DSM_SGT_ESTATE, "frequency " + std::to_string(i) + " not computed");
-    const int R = h->sh.r_n, T = h->sh.theta_n;
+    const int R = h->r_n, T = h->theta_n;
     const double2 *src;
     if (kind == DSM_SGT_TSGT) {
-        num = 10;
+        num = h->ncomp[0];
         if (ir < 0 || ir >= R) return fail(h, DSM_SGT_EINVAL, "receiver index out of range");
         src = h->d_cat_t.as<double2>() + ((size_t)pos * R + ir) * T * num;
     } else if (kind == DSM_SGT_RSGT) {
-        num = 5;
+        num = h->ncomp[1];
         if (ir < 0 || ir >= R) return fail(h, DSM_SGT_EINVAL, "receiver index out of range");
         src = h->d_cat_r.as<double2>() + ((size_t)pos * R + ir) * T * num;
     } else if (kind == DSM_SGT_SYNN) {
-        num = 5;
+        num = h->ncomp[2];
         src = h->d_cat_s.as<double2>() + (size_t)pos * T * num;
     } else {
         return fail(h, DSM_SGT_EINVAL, "unknown catalog kind");
@@ -597,8 +979,8 @@ static std::string i7(long v)
 
 static int write_record(dsm_sgt_t *h, const std::string &path, int32_t i, int32_t kind, int32_t ir)
 {
-    const int T = h->sh.theta_n;
-    const int num = kind == DSM_SGT_TSGT ? 10 : 5;
+    const int T = h->theta_n;
+    const int num = h->ncomp[kind];
     std::vector<float> rec((size_t)2 * num * T);
     int rc = dsm_sgt_fetch(h, i, kind, ir, rec.data());
     if (rc) return rc;
@@ -619,10 +1001,10 @@ int dsm_sgt_write_catalog(dsm_sgt_t *h, int32_t i, const char *output_dir, const
     ::mkdir((od + "/TSGT").c_str(), 0755);
     ::mkdir((od + "/RSGT").c_str(), 0755);
     ::mkdir((od + "/log").c_str(), 0755);
-    const int R = h->sh.r_n;
+    const int R = h->r_n;
     for (int ir = 0; ir < R; ir++) {
-        const long rr = (long)(h->sh.r_[ir] * 1.0e3);
-        std::string name = od + "/TSGT/" + mn + "." + i7(h->sh.intir0) + "." + i7(rr) + "." + i7(i) + ".TSGT_" + suf;
+        const long rr = (long)(h->radii[ir] * 1.0e3);
+        std::string name = od + "/TSGT/" + mn + "." + i7(h->intir0) + "." + i7(rr) + "." + i7(i) + ".TSGT_" + suf;
         int rc = write_record(h, name, i, DSM_SGT_TSGT, ir);
         if (rc) return rc;
         if (h->in.rsgtswitch) {
@@ -632,7 +1014,7 @@ int dsm_sgt_write_catalog(dsm_sgt_t *h, int32_t i, const char *output_dir, const
         }
     }
     if (h->in.synnswitch) {
-        std::string name = od + "/RSGT/" + mn + "." + i7(h->sh.intir0) + "." + i7(i) + ".SYNN_" + suf;
+        std::string name = od + "/RSGT/" + mn + "." + i7(h->intir0) + "." + i7(i) + ".SYNN_" + suf;
         int rc = write_record(h, name, i, DSM_SGT_SYNN, 0);
         if (rc) return rc;
     }

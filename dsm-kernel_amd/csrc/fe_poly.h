@@ -124,6 +124,50 @@ static inline void calmatc(int nlayer, int vnp, const double *vra, const double 
     }
 }
 
+// P-SV calmatc (src/SGTpsv/calmat.f90:1-62): the same integrals, no
+// symmetrisation.
+static inline void calmatc_psv(int nlayer, int vnp, const double *vra, const double *con, int rpow, int w1dn,
+                               int w2dn, const double *ra, double *m)
+{
+    int snp = 1;
+    for (int i = 1; i <= nlayer; i++) {
+        double rh = ra[i] - ra[i - 1];
+        double a[2][2], b[2][2];
+        if (w1dn == 0) {
+            a[0][1] = -1.0 / rh;
+            a[0][0] = ra[i] / rh;
+            a[1][1] = 1.0 / rh;
+            a[1][0] = -ra[i - 1] / rh;
+        } else {
+            a[0][1] = 0.0;
+            a[0][0] = -1.0 / rh;
+            a[1][1] = 0.0;
+            a[1][0] = 1.0 / rh;
+        }
+        if (w2dn == 0) {
+            b[0][1] = -1.0 / rh;
+            b[0][0] = ra[i] / rh;
+            b[1][1] = 1.0 / rh;
+            b[1][0] = -ra[i - 1] / rh;
+        } else {
+            b[0][1] = 0.0;
+            b[0][0] = -1.0 / rh;
+            b[1][1] = 0.0;
+            b[1][0] = 1.0 / rh;
+        }
+        for (int j = 0; j < 2; j++)
+            for (int k = 0; k < 2; k++) {
+                double c[5] = {0, 0, 0, 0, 0};
+                pmulti(2, a[j], 2, b[k], 3, c);
+                for (int l = 3; l >= 1; l--) {
+                    c[l + rpow - 1] = c[l - 1];
+                    if (rpow > 0) c[l - 1] = 0.0;
+                }
+                m[4 * (i - 1) + 2 * j + k] = pinteg(snp, 5, c, ra[i - 1], ra[i], vnp, vra, con);
+            }
+    }
+}
+
 // caltl: lumped mass (\int r^2 rho over each half layer).
 static inline void caltl(int nlayer, int vnp, const double *vra, const double *rho, const double *ra,
                          double *tl)

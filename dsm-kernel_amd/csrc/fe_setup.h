@@ -54,6 +54,90 @@ struct ShOmega {
 };
 void sh_omega_setup(const ShSetup &s, int i, ShOmega &o);
 
+// ---- P-SV set-up: SGTpsv-MPI.f90:120-480 ---------------------------------
+// Element integrals of one zone (calmat.f90 calmatc/caltl/calhl/calh5/
+// calhm1/calhm2 and their transposes), stored per zone.
+struct PsvZoneFE {
+    bool solid = true;
+    int nlayer = 0;
+    // solid zones, 4*nlayer each
+    std::vector<double> t, h1x, h1y, h1z, h2L, h2N, h3ax, h3ay, h3az, h4aL, h4aN, h5ax, h5ay, h5az, h6aL, h6aN, h3x,
+        h3y, h3z, h4L, h4N, h5x, h5y, h5z, h6L, h6N, h7x, h7y, h7z, h8L, h8N;
+    // solid zones, 4*(nlayer+1) each: h?m?(-2:1 or -1:2, nlayer+1)
+    std::vector<double> h3mx, h3my, h3mz, h5mx, h5my, h5mz, h4m1L, h4m1N, h4m2L, h4m2N, h6m1L, h6m1N, h6m2L, h6m2N;
+    // fluid zones, 4*nlayer each
+    std::vector<double> p1, p2, p3;
+};
+
+struct PsvSetup {
+    int nzone = 0, ndc = 0;
+    std::vector<int> iphase, nlayer;  // 1 solid, 2 fluid
+    std::vector<double> vrmin, vrmax, rrho, vpv, vph, vsv, vsh, eta, qmu, qkappa;
+    double rmin = 0, rmax = 0, omegai = 0, tlen = 0, ratl = 0;
+    int maxlmax = 0, imax = 0;
+    int r_n = 0, theta_n = 0;
+    std::vector<double> r_, theta_deg, rrsta, w0, w1, lambda, qkp;
+    std::vector<int> iista, istazone, ksta, jsta, fluid_rec;
+    double r0 = 0, spo = 0;
+    int intir0 = 0, spn = 0;
+    int nnlayer = 0, nslay = 0, nllay = 0;
+    std::vector<double> ra;
+    std::vector<int> isp, kkdr;
+    std::vector<PsvZoneFE> fe;
+    double ecC0 = 0, ecF0 = 0, ecL0 = 0;
+    int nn = 0;       // band rows (SGTpsv-MPI.f90:634)
+    int row0 = 1;     // first solved row: 3 when the model reaches r = 0
+    int jtmp = 0;     // first source row (1-based, SGTpsv-MPI.f90:643-648)
+    int mtmp = 0;     // source layer: ra(mtmp), ra(mtmp+1)
+    // band assembly (cala + calbc, calmat.f90:747-809): for every column j
+    // and band row ii (a(ii,j) = A(j-4+ii, j)) the kind of the entry --
+    // 0 zero, 1 a0 + l(l+1) a2, 2 sqrt(l(l+1)) a1, 3 calbc constant -- and
+    // whether a second zone adds to it (the node shared by two solid zones)
+    std::vector<int> term_kind;       // [nn][4]
+    std::vector<int> term_two;        // [nn][4]
+    // l = 0 branch (rea2, calmat.f90:1247-1315): node vector of nn0 entries
+    int nn0 = 0, row0_0 = 1;
+    std::vector<int> node_row;        // [nn0] band column of the node's U (or P) row, 1-based
+    std::vector<int> node_c1;         // [nn0] band row ii of c(1,i) (0: none)
+    std::vector<int> jsta0;           // [r_n] first d index of the receiver stencil (1-based)
+};
+
+// Returns 0 or a DSM_SGT_E* code with *err set.
+int psv_setup(const dsm_sgt_input_t &in, PsvSetup &s, std::string &err);
+
+// Per-frequency data (SGTpsv-MPI.f90:566-625): the band pieces of every
+// column, the excitation's RK tables and the source-zone moduli.
+struct PsvOmega {
+    double omega = 0;
+    int lsuf = 0;
+    cd coef1_spn, coef2_spn, womega;
+    std::vector<cd> anum, bnum;       // 160 each (anum(4,4,10))
+    // per column j (0-based) and band row ii: up to two contributions
+    // (P0,P1), (Q0,Q1); kinds in PsvSetup::term_kind
+    std::vector<cd> P0, P1, Q0, Q1;   // [nn][4]
+    std::vector<cd> fluidK;           // [r_n]: (omega - i omegai)/3 / lambda~ (calupfluid)
+};
+void psv_omega_setup(const PsvSetup &s, int i, PsvOmega &o);
+
+// Device layouts shared by the engine and the host test build:
+//  band  [nn][4 band rows][P0, P1, Q0, Q1] complex (interleaved re/im),
+//  kinds [nn]: per band row ii, 4 bits (kind | second term << 2) << 4*ii.
+void psv_pack_band(const PsvSetup &s, const PsvOmega &o, double *band);
+void psv_pack_kind(const PsvSetup &s, int *kind);
+
+// Rows the backward substitution must record: every receiver's six rows
+// ksta-1 .. ksta+4 (interpolate(2,..) reads g0(ksta-1 ..)), the surface
+// rows nn-1, nn (SYNN) -- descending -- and where each consumer finds them.
+struct PsvGeom {
+    std::vector<int> need;       // descending, 1-based
+    std::vector<int> rec_row;    // [r][6] indices into need
+    int amp_i0 = 0, amp_i1 = 0;  // rows ksta(r_n)-1, ksta(r_n)
+    int syn_i0 = 0, syn_i1 = 0;  // rows nn-1, nn
+    int rowlo = 0;               // lowest needed row
+    std::vector<double> irf;     // 1/float(r) per receiver
+};
+PsvGeom psv_geometry(const PsvSetup &s);
+
 // Interpolation weights of the reference's 3-node Taylor stencil
 // (others.f90 interpolate + glu): weights b with u = sum b_i g_i.
 void stencil_weights(double rsta, const double rr[3], int nderiv, double w[3]);

@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include "psv_lane.h"
 #include "sh_lane.h"
 
 namespace dsm {
@@ -11,29 +12,41 @@ namespace dsm {
 void launch_sh_solve(const ShSolveArgs &a, int nblocks, hipStream_t s);
 void launch_sh_coef(const ShCoefArgs &a, int nblocks, hipStream_t s);
 
+// P-SV lane bodies and argument structs: psv_lane.h
+void launch_psv_solve(const PsvSolveArgs &a, int nblocks, hipStream_t s);
+void launch_psv_coef(const PsvCoefArgs &a, int nblocks, hipStream_t s);
+void launch_psv_l0(const PsvL0Args &a, hipStream_t s);
+
 // ---- Legendre / vector-harmonic tables ------------------------------------
 // FP64 upward recurrence for P_l^m, m<=3 (trial.f90:85-117), one lane per
 // distance; writes the real basis functions of the synthesis for
 // l in [0, lmax] and |m| = 1, 2: tab[mu-1][b][l][theta_pad].
 void launch_legendre_sh(const double *theta_rad, int theta_n, int theta_pad, int lmax, int lcap,
                         double *tab, hipStream_t s);
+// P-SV functions F0..F9 of |m| = 0, 1, 2 (legendre_lane.h): tab[plane][l][theta_pad]
+void launch_legendre_psv(const double *theta_rad, int theta_n, int theta_pad, int lmax, int lcap,
+                         double *tab, hipStream_t s);
 
 // ---- zero the l tail of the coefficient rows ------------------------------
 void launch_zero_tail(double2 *E, int nslot_rows, int rows_per_slot, const int *slots,
                       const int *nl, int lcap, hipStream_t s);
 
 // ---- synthesis contraction -------------------------------------------------
-// out[(slot,r)][theta][comp] += sum_{j,l} E[slot][q0+j][r][l] * T[mu][b_j][l][theta]
-// as a real GEMM on FP64 MFMA (complex A rows split into re/im rows).
+// acc[(slot,r)][theta] = sum_{j,l} E[slot][q0+j][r][l] * T[b_j][l][theta]
+// as a real GEMM on FP64 MFMA (complex A rows split into re/im rows), then
+// out[..][comp_t] (=|+=) sign_t * acc for the job's one or two targets.
 struct SynthJob {
-    int q0, nb, mu, b[4];
-    int comp, ncomp;       // component index within the record, record width
+    int q0, nb, b[4];      // first row, number of theta functions, their table planes
+    int comp[2];           // target components (comp[1] < 0: one target)
+    double sign[2];        // +-1
+    int accumulate;        // add into the targets instead of writing them
+    int ncomp;             // record width
     double2 *out;          // catalog of this kind: [pos][r_eff][theta_n][ncomp]
 };
 struct SynthArgs {
     const double2 *E;      // [slot][nq][r_eff][lcap]
     int nq, r_eff;
-    const double *tab;     // [mu-1][6][lcap][theta_pad]
+    const double *tab;     // [plane][lcap][theta_pad]
     int theta_n, theta_pad, lcap;
     int nslot;
     const int *nl;         // [slot] number of l terms

@@ -26,4 +26,17 @@ void launch_legendre_sh(const double *theta_rad, int theta_n, int theta_pad, int
                        theta_pad, lmax, lcap, tab);
 }
 
+__global__ void legendre_psv_kernel(const double *theta_rad, int theta_n, int theta_pad, int lmax, int lcap,
+                                    double *tab)
+{
+    legendre_psv_column(theta_rad, theta_n, theta_pad, lmax, lcap, tab, blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+void launch_legendre_psv(const double *theta_rad, int theta_n, int theta_pad, int lmax, int lcap, double *tab,
+                         hipStream_t s)
+{
+    hipLaunchKernelGGL(legendre_psv_kernel, dim3((theta_pad + 63) / 64), dim3(64), 0, s, theta_rad, theta_n,
+                       theta_pad, lmax, lcap, tab);
+}
+
 }  // namespace dsm

@@ -84,4 +84,78 @@ __host__ __device__ inline void legendre_sh_column(const double *theta_rad, int 
         for (int b = 0; b < 12; b++) tab[(size_t)b * plane + (size_t)l * theta_pad + it] = 0.0;
 }
 
+// P-SV basis (caldvecphi0_withplm, src/SGTpsv/trial.f90:97-223, with the
+// single-precision sin/cot of locallyCartesianDerivatives, others.f90:
+// 310-344): for |m| = mu, c = sqrt((2l+1)/(4 pi) (l-mu)!/(l+mu)!),
+// Pt = mu x/s P_l^mu + P_l^{mu+1}, sf = float(s), cf = float(x/s):
+//   F0 = c P                F1 = c Pt
+//   F2 = (-x/s Pt + mu^2/(1-x^2) P - l(l+1) P) c
+//   F3 = (-x/(1-x^2) P + Pt/s) c          F4 = ((1/s) c) P
+//   F5 = F0/sf   F6 = F1/sf   F7 = F4 cf   F8 = (((1/s) P) c)/sf   F9 = F1 cf
+// Planes: mu=0 -> F0, F1, F2, F9 (planes 0..3); mu=1,2 -> F0..F9 (planes
+// 4+10(mu-1)+k).  At the poles only the mu=0 F0 survives, as the
+// default-kind cmplx(coeff)*(+-1) of the reference (trial.f90:112-131).
+constexpr int PSV_NPLANES = 24;
+__host__ __device__ inline int psv_plane(int mu, int f)
+{
+    if (mu == 0) return f == 9 ? 3 : f;
+    return 4 + 10 * (mu - 1) + f;
+}
+
+__host__ __device__ inline void legendre_psv_column(const double *theta_rad, int theta_n, int theta_pad, int lmax,
+                                                    int lcap, double *tab, int it)
+{
+    if (it >= theta_pad) return;
+    const size_t plane = (size_t)lcap * theta_pad;
+    if (it >= theta_n) {
+        for (int b = 0; b < PSV_NPLANES; b++)
+            for (int l = 0; l < lcap; l++) tab[(size_t)b * plane + (size_t)l * theta_pad + it] = 0.0;
+        return;
+    }
+    const double th = theta_rad[it];
+    const bool pole = (th == 0.0 || th == DSM_PI);
+    const double x = cos(th), s = sin(th);
+    const double sf = (double)(float)s, cf = (double)(float)(x / s);
+    double plm[4][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    for (int l = 0; l <= lmax; l++) {
+        double f[3][10];
+        for (int mu = 0; mu < 3; mu++)
+            for (int k = 0; k < 10; k++) f[mu][k] = 0.0;
+        if (pole) {
+            const double sign1 = (th == DSM_PI && (l & 1)) ? -1.0 : 1.0;
+            f[0][0] = (double)(float)sqrt((double)(2 * l + 1) / (4.0 * DSM_PI)) * sign1;
+        } else {
+            for (int m = 0; m <= (l < 3 ? l : 3); m++) calplm(l, m, x, plm[m]);
+            const double xl2 = (double)l * (double)(l + 1);
+            for (int mu = 0; mu <= (l < 2 ? l : 2); mu++) {
+                double fact = 1.0;
+                if (mu != 0)
+                    for (int i = l - mu + 1; i <= l + mu; i++) fact = fact * (double)i;
+                const double c = sqrt((double)(2 * l + 1) / (4.0 * DSM_PI) / fact);
+                const double P = plm[mu][0];
+                const double Pt = (double)mu * x / sin(th) * P + plm[mu + 1][0];
+                double *F = f[mu];
+                F[0] = c * P;
+                F[1] = c * Pt;
+                F[2] = (-x / sin(th) * Pt + (double)mu * (double)mu / (1 - x * x) * P - xl2 * P) * c;
+                F[3] = (-x / (1 - x * x) * P + 1.0 / sin(th) * Pt) * c;
+                F[4] = 1.0 / sin(th) * c * P;
+                F[5] = P * c / sf;
+                F[6] = Pt * c / sf;
+                F[7] = F[4] * cf;
+                F[8] = 1.0 / sin(th) * P * c / sf;
+                F[9] = c * Pt * cf;
+            }
+        }
+        for (int k = 0; k < 4; k++) {
+            const int fk = (k == 3) ? 9 : k;
+            tab[(size_t)psv_plane(0, fk) * plane + (size_t)l * theta_pad + it] = f[0][fk];
+        }
+        for (int mu = 1; mu <= 2; mu++)
+            for (int k = 0; k < 10; k++) tab[(size_t)psv_plane(mu, k) * plane + (size_t)l * theta_pad + it] = f[mu][k];
+    }
+    for (int l = lmax + 1; l < lcap; l++)
+        for (int b = 0; b < PSV_NPLANES; b++) tab[(size_t)b * plane + (size_t)l * theta_pad + it] = 0.0;
+}
+
 }  // namespace dsm

@@ -28,13 +28,12 @@ struct TileLoad {
     double2 b[2];  // 16 l x 128 theta / 512 threads (pairs)
 };
 
-__device__ __forceinline__ void load_tile(const SynthArgs &a, const SynthJob &job, const double *tab_mu,
-                                          size_t plane, int m0, int n0, int M, int R, int it, int nk, int tid,
-                                          TileLoad &t)
+__device__ __forceinline__ void load_tile(const SynthArgs &a, const SynthJob &job, size_t plane, int m0, int n0,
+                                          int M, int R, int it, int nk, int tid, TileLoad &t)
 {
     const int jb = it / nk, l0 = (it - jb * nk) * BK;
     const double2 *Eq = a.E + (size_t)(job.q0 + jb) * R * a.lcap;
-    const double *T = tab_mu + (size_t)job.b[jb] * plane;
+    const double *T = a.tab + (size_t)job.b[jb] * plane;
 #pragma unroll
     for (int u = 0; u < 2; u++) {
         const int idx = tid + NT * u;
@@ -112,7 +111,6 @@ __global__ __launch_bounds__(512) void synth_f64_kernel(SynthArgs a, int nx, int
     }
     const int nk = (kmax + BK - 1) / BK;
     const int niter = job.nb * nk;
-    const double *tab_mu = a.tab + (size_t)(job.mu - 1) * 6 * a.lcap * a.theta_pad;
     const size_t plane = (size_t)a.lcap * a.theta_pad;
     const int wr = wave >> 2, wc = wave & 3;
 
@@ -124,13 +122,13 @@ __global__ __launch_bounds__(512) void synth_f64_kernel(SynthArgs a, int nx, int
 
     TileLoad tl;
     if (niter > 0) {
-        load_tile(a, job, tab_mu, plane, m0, n0, M, R, 0, nk, tid, tl);
+        load_tile(a, job, plane, m0, n0, M, R, 0, nk, tid, tl);
         store_tile(As[0], Bs[0], tid, tl);
     }
     __syncthreads();
     for (int it = 0; it < niter; it++) {
         const int st = it & 1;
-        if (it + 1 < niter) load_tile(a, job, tab_mu, plane, m0, n0, M, R, it + 1, nk, tid, tl);
+        if (it + 1 < niter) load_tile(a, job, plane, m0, n0, M, R, it + 1, nk, tid, tl);
 #pragma unroll
         for (int k4 = 0; k4 < BK; k4 += 4) {
             const int kk = k4 + (lane >> 4);
@@ -151,6 +149,7 @@ __global__ __launch_bounds__(512) void synth_f64_kernel(SynthArgs a, int nx, int
     // epilogue: row = (slot, r, re|im) -> out[pos][r][theta][comp].
     // v_mfma_f64_16x16x4f64 leaves D[i][j] in lane j + 16*(i%4), register i/4.
     double *out = (double *)job.out;
+    const int ntgt = job.comp[1] >= 0 ? 2 : 1;
 #pragma unroll
     for (int i = 0; i < 4; i++)
 #pragma unroll
@@ -163,7 +162,12 @@ __global__ __launch_bounds__(512) void synth_f64_kernel(SynthArgs a, int nx, int
 #pragma unroll
             for (int j = 0; j < 2; j++) {
                 const int col = n0 + wc * 32 + j * 16 + (lane & 15);
-                if (col < a.theta_n) out[2 * ((base + col) * job.ncomp + job.comp) + reim] = acc[i][j][q];
+                if (col >= a.theta_n) continue;
+                for (int t = 0; t < ntgt; t++) {
+                    double *o = out + 2 * ((base + col) * job.ncomp + job.comp[t]) + reim;
+                    const double v = job.sign[t] * acc[i][j][q];
+                    *o = job.accumulate ? *o + v : v;
+                }
             }
         }
 }
