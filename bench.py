@@ -1,14 +1,16 @@
 #!/usr/bin/env python3
 """SGT catalog benchmark on one MI355X.
 
-Workload (one "step"): the full SH strain-Green's-tensor catalog of the 10 s
-configuration -- synthetic 4-zone sphere (fluid core removed for SH),
-tlen 3276.8 s, frequencies i = 1..328, re = 0.1 (531 band rows),
-lmax 4000, 40 receiver radii (4421..6371 km, 50 km), 1801 distances
-(0..180 deg, 0.1 deg), source at the surface -- computed into HBM (FP64
-accumulators), including the omega-independent set-up and the Legendre
-table.  The timed region starts with the model in host memory and ends with
-the catalog resident in HBM; writing the files is not timed.
+Workload (one "step"): the full P-SV strain-Green's-tensor catalog of the
+5 s configuration -- synthetic 4-zone sphere with a fluid outer core,
+tlen 3276.8 s, frequencies i = 1..656, re = 0.1 (3727 band rows), lmax 8000,
+60 receiver radii (3716..6371 km, 45 km), 1801 distances (0..180 deg,
+0.1 deg), source at the surface -- computed into HBM (FP64 accumulators of
+TSGT_PSV, RSGT_PSV and SYNN_PSV), including the omega-independent set-up
+and the Legendre table.  The timed region starts with the parsed model in
+host memory and ends with the catalog resident in HBM; writing the files is
+not timed.  `--mode sh` runs the 10 s SH catalog instead (i = 1..328,
+531 rows, lmax 4000, 40 radii 4421..6371 km).
 
 Prints one JSON line with the metric of the reference's headline benchmark,
 the roofline of the dominant kernel (the FP64-MFMA synthesis contraction)
@@ -29,12 +31,15 @@ METRIC = "SGT catalog wall time: batched/mixed vs plain FP64 on 1 GPU at ≤1e-6
 FP64_MFMA_PEAK = 78.6e12
 HBM_PEAK = 8.0e12
 
-C2 = dict(tlen=3276.8, imax=328, rmin_=4421.0, rmax_=6371.0, rdelta_=50.0, thetamin=0.0,
-          thetamax=180.0, thetadelta=0.1, maxlmax=4000, re=1e-1, r0=6371.0)
+WORKLOADS = {
+    "psv": dict(tlen=3276.8, imax=656, rmin_=3716.0, rmax_=6371.0, rdelta_=45.0, thetamin=0.0,
+                thetamax=180.0, thetadelta=0.1, maxlmax=8000, re=1e-1, r0=6371.0),
+    "sh": dict(tlen=3276.8, imax=328, rmin_=4421.0, rmax_=6371.0, rdelta_=50.0, thetamin=0.0,
+               thetamax=180.0, thetadelta=0.1, maxlmax=4000, re=1e-1, r0=6371.0),
+}
 
 
 def make_input(p):
-    import dsm_sgt
     from dsm_sgt.inputs import read_model, make_input as mk
     with open(os.path.join(HERE, "dsm-kernel_amd", "configs", "model4.psv.model")) as f:
         zones = read_model(f.read())
@@ -46,34 +51,36 @@ def make_input(p):
     return mk(zones, conf, params)
 
 
-def cpu_baseline(p, budget_s=20.0):
-    """Time the oracle on one frequency with two distance subsets, fit
-    t = a + b*theta_n and extrapolate to the full catalog (every frequency
-    runs the full l range: the surface source is never cut off before
-    lmax in this configuration, which the GPU run confirms)."""
+def cpu_baseline(mode, p, r_n, lsample=1000):
+    """Time the oracle (the C restatement of the reference's driver, one
+    thread) on one frequency of the same radial grid with all receivers,
+    the first `lsample` degrees and two small distance subsets; fit
+    t = L*(a + b*theta_n) (every per-l step costs the same) and extrapolate
+    to the full l range, distances and frequencies.  Every frequency of the
+    surface-source catalog runs the full l range (the GPU run reports the l
+    cut-off of the first and last frequency)."""
     import oracle_lib as ol
     import dsm_inputs as di
-    q = dict(p)
     i = p["imax"] // 2
     pts = []
-    for nth in (2, 6):
-        q["thetamin"], q["thetamax"] = 10.0, 10.0 + (nth - 1) * 0.1
-        inp = di.synthetic_input(tlen=q["tlen"], imax=q["imax"], rmin_=q["rmin_"], rmax_=q["rmax_"],
-                                 rdelta_=q["rdelta_"], thetamin=q["thetamin"], thetamax=q["thetamax"],
-                                 thetadelta=0.1, maxlmax=q["maxlmax"], re=q["re"], r0=q["r0"])
+    for nth in (2, 12):
+        inp = di.synthetic_input(tlen=p["tlen"], imax=p["imax"], rmin_=p["rmin_"], rmax_=p["rmax_"],
+                                 rdelta_=p["rdelta_"], thetamin=10.0, thetamax=10.0 + (nth - 1) * 0.1 + 0.01,
+                                 thetadelta=0.1, maxlmax=lsample, re=p["re"], r0=p["r0"])
         t0 = time.perf_counter()
-        ol.sh_omega(inp, i, fast=True)
+        (ol.psv_omega if mode == "psv" else ol.sh_omega)(inp, i, fast=True)
         pts.append((nth, time.perf_counter() - t0))
     (n1, t1), (n2, t2) = pts
     b = (t2 - t1) / (n2 - n1)
     a = max(t1 - b * n1, 0.0)
     nfull = int(round((p["thetamax"] - p["thetamin"]) / p["thetadelta"])) + 1
-    per_omega = a + b * nfull
+    per_omega = (a + b * nfull) * (p["maxlmax"] + 1) / (lsample + 1)
     total = per_omega * p["imax"]
     return dict(value=total, unit="s", cores=1, kind="port",
-                sample=f"oracle (gcc -O3 -march=native, 1 thread) on frequency {i} with "
-                       f"{n1} and {n2} distances ({t1:.1f} s, {t2:.1f} s); t = a + b*theta_n "
-                       f"extrapolated to {nfull} distances x {p['imax']} frequencies")
+                sample=f"oracle (gcc -O3, 1 thread) on frequency {i} of the same grid, {r_n} radii, "
+                       f"l=0..{lsample}, {n1} and {n2} distances ({t1:.1f} s, {t2:.1f} s); "
+                       f"t = L*(a + b*theta_n) extrapolated to l=0..{p['maxlmax']}, {nfull} distances "
+                       f"x {p['imax']} frequencies")
 
 
 def main():
@@ -81,22 +88,25 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--mode", choices=("psv", "sh"), default="psv")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline sample")
     ap.add_argument("--imax", type=int, default=None, help="(testing) fewer frequencies")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build measures one GPU only (the frequency axis is the "
-              "multi-GPU split; see DESIGN.md); run with --gpus 1", file=sys.stderr)
+        print("bench.py: this build runs on one GPU only; the catalog's frequencies are independent "
+              "and would be split across GPUs, but no multi-GPU run is part of it. Use --gpus 1.",
+              file=sys.stderr)
         return 2
     import dsm_sgt
-    p = dict(C2)
+    p = dict(WORKLOADS[args.mode])
     if args.imax:
         p["imax"] = args.imax
     inp = make_input(p)
+    mode = dsm_sgt.PSV if args.mode == "psv" else dsm_sgt.SH
 
     def step():
-        eng = dsm_sgt.Engine(inp, mode=dsm_sgt.SH, precision=dsm_sgt.FP64)
-        eng.run(1, p["imax"])
+        eng = dsm_sgt.Engine(inp, mode=mode, precision=dsm_sgt.FP64)
+        eng.run(1, p["imax"])  # returns with the device synchronised
         return eng
 
     for _ in range(args.warmup):
@@ -104,16 +114,25 @@ def main():
     times, stats = [], []
     for _ in range(args.steps):
         t0 = time.perf_counter()
-        eng = step()  # run() synchronises the device before returning
+        eng = step()
         times.append(time.perf_counter() - t0)
         stats.append(eng.stats())
         cuts = [eng.lcut(i) for i in (1, p["imax"])]
+        info = eng.info
         eng.close()
     ms = 1e3 * sum(times) / len(times)
     st = stats[-1]
     synth_ms_per_launch = st.synth_ms / max(st.synth_launches, 1)
     flops_per_launch = st.synth_flops / max(st.synth_launches, 1)
     achieved = flops_per_launch / (synth_ms_per_launch * 1e-3) if synth_ms_per_launch > 0 else 0.0
+    r_n, th_n, nn = info.r_n, info.theta_n, info.nn
+    if args.mode == "psv":
+        workload = (f"SGTpsv 5 s catalog (P-SV plain FP64): 4-zone sphere with fluid core, tlen 3276.8 s, "
+                    f"i=1..{p['imax']}, re 0.1 ({nn} rows), lmax {p['maxlmax']}, {r_n} radii "
+                    f"(3716..6371 km) x {th_n} distances, surface source")
+    else:
+        workload = (f"SGTsh 10 s catalog (SH plain FP64): 4-zone sphere, tlen 3276.8 s, i=1..{p['imax']}, "
+                    f"re 0.1 ({nn} rows), lmax {p['maxlmax']}, {r_n} radii x {th_n} distances, surface source")
     out = {
         "metric": METRIC,
         "value": ms / 1e3,
@@ -127,14 +146,11 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": "SGTsh 10 s catalog (SH plain FP64): 4-zone sphere, tlen 3276.8 s, "
-                               f"i=1..{p['imax']}, re 0.1 (531 rows), lmax {p['maxlmax']}, "
-                               "40 radii x 1801 distances, surface source",
-                   "form": "plain FP64", "frequencies": p["imax"], "lmax": p["maxlmax"],
-                   "radii": 40, "distances": 1801, "l_cut": cuts},
+        "config": {"workload": workload, "form": "plain FP64", "frequencies": p["imax"],
+                   "lmax": p["maxlmax"], "band_rows": nn, "radii": r_n, "distances": th_n, "l_cut": cuts},
         "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": FP64_MFMA_PEAK / 1e12,
                      "unit": "TFLOP/s", "frac": achieved / FP64_MFMA_PEAK, "traffic": None,
-                     "kernel": "synth_f64_kernel",
+                     "kernel": "synth_f64_kernel<0>",
                      "avg_launch_ms": synth_ms_per_launch, "flops_per_launch": flops_per_launch},
         "stages_ms": {"solve+coef": st.solve_ms, "legendre": st.legendre_ms, "synth": st.synth_ms,
                       "other": st.other_ms},
@@ -142,7 +158,7 @@ def main():
                       "peak_GBps": HBM_PEAK / 1e9, "systems": st.systems},
     }
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(p)
+        out["cpu_baseline"] = cpu_baseline(args.mode, p, r_n)
     print(json.dumps(out), flush=True)
     return 0
 
