@@ -1,5 +1,6 @@
-! sgt_host.f90 -- Fortran host of the MI355X SGT engine: `sgtsh_gpu < database.inf`
-! (and `sgtpsv_gpu`).  Same stdin grammar, DSM.config and model files as the
+! sgt_host.f90 -- Fortran host of the MI355X SGT engine (module; the programs
+! are sgtpsv_main.f90 and sgtsh_main.f90: `sgtpsv_gpu < database.inf`,
+! `sgtsh_gpu < database.inf`).  Same stdin grammar, DSM.config and model files as the
 ! reference's SGTsh / SGTpsv drivers (src/SGTsh/others.f90:1-110), same
 ! catalog files and log files; the per-frequency work is done on one GPU
 ! through the C-ABI (include/dsm_sgt.h).  No MPI: frequencies are batched on
@@ -39,7 +40,7 @@ contains
     character(len=120) :: recs(200), crecs(50), mrecs(2000)
     character(len=120) :: dsmconf, outdir, modelf, modelname
     character(len=200) :: list, list1, fname
-    character(len=22) :: tag
+    character(len=23) :: tag
     character(len=8) :: datex
     character(len=10) :: timex
     character(len=10) :: kind
@@ -108,6 +109,13 @@ contains
     write(21, '(/a,a4,a,a2,a,a2,a,a2,a,a2,a,a4)') '    Starting date and time:                     ', &
          datex(1:4), '-', datex(5:6), '-', datex(7:8), '.  ', timex(1:2), ':', timex(3:4), ':', timex(5:8)
     close(21)
+    if (mode == DSM_SGT_PSV) then   ! SGTpsv-MPI.f90:522-531
+       open(21, file=list, status='old', access='append', form='formatted')
+       call date_and_time(datex, timex)
+       write(21, '(/a,a4,a,a2,a,a2,a,a2,a,a2,a,a4)') '    here we go!:                     ', &
+            datex(1:4), '-', datex(5:6), '-', datex(7:8), '.  ', timex(1:2), ':', timex(3:4), ':', timex(5:8)
+       close(21)
+    endif
     write(tag, '(I7,".",I7,".",I7)') inp%imin, inp%imax, 0
     do j = 1, 22
        if (tag(j:j) == ' ') tag(j:j) = '0'
@@ -158,16 +166,3 @@ contains
   end function count_values
 
 end module sgt_host_io
-
-program sgtsh_gpu
-  use sgt_host_io
-  implicit none
-  character(len=16) :: arg
-  integer(c_int32_t) :: mode
-  mode = DSM_SGT_SH
-  if (command_argument_count() >= 1) then
-     call get_command_argument(1, arg)
-     if (trim(arg) == 'psv') mode = DSM_SGT_PSV
-  endif
-  call run_host(mode)
-end program sgtsh_gpu
