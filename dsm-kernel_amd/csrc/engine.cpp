@@ -82,7 +82,7 @@ struct dsm_sgt {
     std::vector<int> llog;    // per position: the reference's logged llog-1
     std::vector<char> done;   // per position: computed
     // device buffers
-    DevBuf d_theta, d_tab, d_need, d_recslot, d_w0, d_w1, d_rinv, d_src;
+    DevBuf d_theta, d_tab, d_tab32, d_need, d_recslot, d_w0, d_w1, d_rinv, d_src;
     DevBuf d_band, d_om, d_slots, d_nl, d_slotout, d_scratch, d_xn, d_E, d_Es, d_amp, d_jobs;
     DevBuf d_cat_t, d_cat_r, d_cat_s;
     // P-SV only
@@ -136,6 +136,24 @@ static void ev_collect(dsm_sgt_t *h)
         (void)hipEventDestroy(t.b);
     }
     h->pending.clear();
+}
+
+// Synthesis launch in the engine's precision.
+static void launch_synth(dsm_sgt_t *h, const SynthArgs &ga)
+{
+    if (h->precision == DSM_SGT_FP32) launch_synth_f32(ga, h->d_tab32.as<float>(), h->stream);
+    else launch_synth_f64(ga, h->stream);
+}
+
+// Rounded copy of the theta table for the mixed forms.
+static int make_mixed_tables(dsm_sgt_t *h, size_t nplanes)
+{
+    if (h->precision == DSM_SGT_FP64) return DSM_SGT_OK;
+    const size_t n = nplanes * (size_t)h->lcap * h->theta_pad;
+    HIPCHK(h, h->d_tab32.ensure(sizeof(float) * n));
+    launch_tab_to_f32(h->d_tab.as<double>(), h->d_tab32.as<float>(), n, h->stream);
+    HIPCHK(h, hipGetLastError());
+    return DSM_SGT_OK;
 }
 
 // Synthesis jobs of the SH contraction (table in sh_jobs.h).
@@ -214,7 +232,7 @@ static int sh_create(dsm_sgt_t *h)
     h->ncomp[1] = 5;
     h->ncomp[2] = 5;
     h->theta_pad = ((s.theta_n + 127) / 128) * 128;  // synthesis N tile
-    h->lcap = ((s.maxlmax + 1 + 15) / 16) * 16;
+    h->lcap = ((s.maxlmax + 1 + 63) / 64) * 64;
     // stencil nodes, descending, always including the surface node nn
     std::vector<int> nodes(s.iista.begin(), s.iista.end());
     nodes.push_back(s.nn);
@@ -302,6 +320,8 @@ static int sh_create(dsm_sgt_t *h)
     launch_legendre_sh(h->d_theta.as<double>(), s.theta_n, h->theta_pad, s.maxlmax, h->lcap, h->d_tab.as<double>(),
                        h->stream);
     HIPCHK(h, hipGetLastError());
+    rc = make_mixed_tables(h, 12);
+    if (rc) return rc;
     ev_end(h, t);
     HIPCHK(h, hipStreamSynchronize(h->stream));
     ev_collect(h);
@@ -466,7 +486,7 @@ static int sh_run_batch(dsm_sgt_t *h, const std::vector<int> &freqs)
     ga.njobs = h->njobs_main;
     TimedLaunch tg;
     ev_begin(h, tg, 2);
-    launch_synth_f64(ga, h->stream);
+    launch_synth(h, ga);
     HIPCHK(h, hipGetLastError());
     ev_end(h, tg);
     // SYNN rows (surface node, no receiver dimension)
@@ -477,7 +497,7 @@ static int sh_run_batch(dsm_sgt_t *h, const std::vector<int> &freqs)
     ga.njobs = h->njobs_syn;
     TimedLaunch tg2;
     ev_begin(h, tg2, 3);
-    launch_synth_f64(ga, h->stream);
+    launch_synth(h, ga);
     HIPCHK(h, hipGetLastError());
     ev_end(h, tg2);
     h->stats.synth_launches += 1;
@@ -513,7 +533,7 @@ static int psv_create(dsm_sgt_t *h)
     h->ncomp[1] = 10;
     h->ncomp[2] = 10;
     h->theta_pad = ((s.theta_n + 127) / 128) * 128;
-    h->lcap = ((s.maxlmax + 1 + 15) / 16) * 16;
+    h->lcap = ((s.maxlmax + 1 + 63) / 64) * 64;
     h->nneed = (int)g.need.size();
     h->nslot = 32;
     // one solve launch covers the whole l range unless told otherwise
@@ -584,6 +604,8 @@ static int psv_create(dsm_sgt_t *h)
     launch_legendre_psv(h->d_theta.as<double>(), s.theta_n, h->theta_pad, s.maxlmax, LC, h->d_tab.as<double>(),
                         h->stream);
     HIPCHK(h, hipGetLastError());
+    rc = make_mixed_tables(h, PSV_NPLANES);
+    if (rc) return rc;
     ev_end(h, t);
     HIPCHK(h, hipStreamSynchronize(h->stream));
     ev_collect(h);
@@ -785,11 +807,11 @@ static int psv_run_batch(dsm_sgt_t *h, const std::vector<int> &freqs)
     ga.njobs = h->njobs_main;
     TimedLaunch tg;
     ev_begin(h, tg, 2);
-    launch_synth_f64(ga, h->stream);
+    launch_synth(h, ga);
     HIPCHK(h, hipGetLastError());
     ga.jobs = h->d_jobs.as<SynthJob>() + h->njobs_main;
     ga.njobs = h->njobs_acc;
-    launch_synth_f64(ga, h->stream);
+    launch_synth(h, ga);
     HIPCHK(h, hipGetLastError());
     ev_end(h, tg);
     ga.E = h->d_Es.as<double2>();
@@ -799,7 +821,7 @@ static int psv_run_batch(dsm_sgt_t *h, const std::vector<int> &freqs)
     ga.njobs = h->njobs_syn;
     TimedLaunch tg2;
     ev_begin(h, tg2, 3);
-    launch_synth_f64(ga, h->stream);
+    launch_synth(h, ga);
     HIPCHK(h, hipGetLastError());
     ev_end(h, tg2);
     h->stats.synth_launches += 2;
@@ -834,7 +856,7 @@ int dsm_sgt_create(const dsm_sgt_input_t *in, int32_t mode, int32_t precision, i
             rc = fail(h, DSM_SGT_EINVAL, "unknown mode " + std::to_string(mode));
             break;
         }
-        if (precision != DSM_SGT_FP64) {
+        if (precision != DSM_SGT_FP64 && precision != DSM_SGT_FP32) {
             rc = fail(h, DSM_SGT_EUNSUPPORTED, "precision not available in this build");
             break;
         }

@@ -55,5 +55,9 @@ struct SynthArgs {
     int njobs;
 };
 void launch_synth_f64(const SynthArgs &a, hipStream_t s);
+// Mixed form (synth32.hip): FP32 operands on FP32 MFMA, FP64 accumulation
+// across 32-wide K chunks; tab32 is the theta table rounded to FP32.
+void launch_synth_f32(const SynthArgs &a, const float *tab32, hipStream_t s);
+void launch_tab_to_f32(const double *src, float *dst, size_t n, hipStream_t s);
 
 }  // namespace dsm
