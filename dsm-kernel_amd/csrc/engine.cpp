@@ -656,6 +656,7 @@ static int psv_run_batch(dsm_sgt_t *h, const std::vector<int> &freqs)
         za.nn0 = s.nn0;
         za.row0_0 = s.row0_0;
         za.jtmp = s.jtmp;
+        za.js = s.js0;
         za.node_row = h->d_noderow.as<int>();
         za.node_c1 = h->d_nodec1.as<int>();
         za.work = h->d_l0work.as<double2>();

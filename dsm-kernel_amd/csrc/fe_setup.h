@@ -99,6 +99,7 @@ struct PsvSetup {
     int nn0 = 0, row0_0 = 1;
     std::vector<int> node_row;        // [nn0] band column of the node's U (or P) row, 1-based
     std::vector<int> node_c1;         // [nn0] band row ii of c(1,i) (0: none)
+    int js0 = 0;                      // node index (0-based) of the source row jtmp
     std::vector<int> jsta0;           // [r_n] first d index of the receiver stencil (1-based)
 };
 

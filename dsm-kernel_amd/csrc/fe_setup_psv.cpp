@@ -600,6 +600,7 @@ int psv_setup(const dsm_sgt_input_t &in, PsvSetup &s, std::string &err)
             err = "source rows are not consecutive nodes of the l=0 system";
             return DSM_SGT_EUNSUPPORTED;
         }
+        s.js0 = (int)(it - s.node_row.begin());
     }
     for (int ir = 0; ir < r_n; ir++)
         if (s.jsta0[ir] < s.row0_0 || s.jsta0[ir] + 2 > s.nn0) {

@@ -250,6 +250,7 @@ extern "C" int dsm_hostcheck_psv(const dsm_sgt_input_t *in, int32_t i, double *t
     za.nn0 = s.nn0;
     za.row0_0 = s.row0_0;
     za.jtmp = s.jtmp;
+    za.js = s.js0;
     za.node_row = s.node_row.data();
     za.node_c1 = s.node_c1.data();
     za.work = work.data();
@@ -265,7 +266,7 @@ extern "C" int dsm_hostcheck_psv(const dsm_sgt_input_t *in, int32_t i, double *t
     za.E = E.data();
     za.Es = Es.data();
     za.lcap = lcap;
-    psv_l0_lane(za, 0);
+    psv_l0_serial(za, 0);
     for (int b = 0; b < nlb; b++)
         for (int t = 0; t < 64; t++) {
             psv_solve_lane(sa, b, t);

@@ -76,6 +76,7 @@ struct PsvL0Args {
     const int *slot_list;
     int nslots;
     int nn, nn0, row0_0, jtmp;
+    int js;                // node index of the source row jtmp (js, js+1: the source's U rows)
     const int *node_row;   // [nn0]
     const int *node_c1;    // [nn0]
     double2 *work;         // [slot][2*nn0 + 3*nn0]
@@ -777,59 +778,135 @@ __host__ __device__ inline void psv_coef_lane(const PsvCoefArgs &a, int b, int t
 // ---------------------------------------------------------------------------
 // l = 0 (SGTpsv-MPI.f90:698-855): the U-only system of rea2, tridiagonal
 // LDL^T (dcsymbdl0 / dcsbdlv0 with m = 1), three right-hand sides, and
-// its coefficient rows at l = 0.  One lane per frequency.
+// its coefficient rows at l = 0.  Four phases per frequency t, the lanes
+// of one workgroup sharing t (psv_kernels.hip):
+//   assemble (nodes in parallel), factor (one lane), substitute (one lane
+//   per right-hand side), rows (one lane per receiver, lane R: SYNN).
+// The serial sweeps carry the running pivot / solution in registers and
+// read the band in groups of PSV_L0_G rows issued ahead of their use.
+// work[t]: c(1,i) [N0], c(2,i) [N0], z [3][N0].
 // ---------------------------------------------------------------------------
-__host__ __device__ inline void psv_l0_lane(const PsvL0Args &a, int t)
+constexpr int PSV_L0_G = 16;
+
+__host__ __device__ inline void psv_l0_assemble(const PsvL0Args &a, int t, int i)
 {
-    if (t >= a.nslots) return;
     const int sl = a.slot_list[t];
     const int N0 = a.nn0;
     const double2 *band = a.band + (size_t)sl * a.nn * 16;
-    double2 *c1 = a.work + (size_t)t * 5 * N0;  // c(1,i)
-    double2 *c2 = c1 + N0;                      // c(2,i)
-    double2 *z = c2 + N0;                       // [3][N0]
-    for (int i = 0; i < N0; i++) {
-        const int row = a.node_row[i];
-        const double2 *col = band + (size_t)(row - 1) * 16;
-        const int kb = a.kind[row - 1];
-        c2[i] = psv_entry(col, kb, 3, 0.0, 0.0);
-        c1[i] = a.node_c1[i] ? psv_entry(col, kb, a.node_c1[i] - 1, 0.0, 0.0) : cz();
-    }
-    const int i0 = a.row0_0 - 1, n = N0 - i0;
-    double2 *C1 = c1 + i0, *C2 = c2 + i0;
-    for (int k = 0; k < n - 1; k++) {
-        const double2 piv = crecip(C2[k]);
-        const double2 zz = cneg(C1[k + 1]);
-        const double2 w = cmul(C1[k + 1], piv);
-        C1[k + 1] = w;
-        C2[k + 1] = cadd(C2[k + 1], cmul(w, zz));
-    }
-    double2 g[2][2];
-    psv_excitation_l0(a.src[sl], a.sc, g);
-    const double c0f = (double)(float)sqrt(1.0 / (4.0 * DSM_PI) / 1.0);
-    // node index of the source U rows jtmp, jtmp+2 and the surface node
-    int js = -1;
-    for (int i = 0; i < N0; i++)
-        if (a.node_row[i] == a.jtmp) js = i;
-    if (js < 0 || js + 1 >= N0) return;  // excluded by psv_setup
-    for (int s = 0; s < 3; s++) {
-        double2 *zs = z + (size_t)s * N0;
-        for (int i = 0; i < N0; i++) zs[i] = cz();
-        if (s < 2) {
-            zs[js] = g[s][0];
-            zs[js + 1] = g[s][1];
-        } else {
-            zs[N0 - 1] = make_double2(-c0f, 0.0);
+    double2 *c1 = a.work + (size_t)t * 5 * N0, *c2 = c1 + N0;
+    const int row = a.node_row[i];
+    const double2 *col = band + (size_t)(row - 1) * 16;
+    const int kb = a.kind[row - 1];
+    c2[i] = psv_entry(col, kb, 3, 0.0, 0.0);
+    c1[i] = a.node_c1[i] ? psv_entry(col, kb, a.node_c1[i] - 1, 0.0, 0.0) : cz();
+}
+
+// dcsymbdl0, m = 1: for k = 0..n-2 (local nodes from row0_0)
+//   w = c1(k+1) / c2(k);  c2(k+1) = c2(k+1) + w (-c1(k+1));  c1(k+1) = w
+__host__ __device__ inline void psv_l0_factor(const PsvL0Args &a, int t)
+{
+    const int N0 = a.nn0, i0 = a.row0_0 - 1, n = N0 - i0;
+    double2 *C1 = a.work + (size_t)t * 5 * N0 + i0, *C2 = C1 + N0;
+    double2 d = C2[0];
+    double2 p1[PSV_L0_G], p2[PSV_L0_G];
+    for (int u = 0; u < PSV_L0_G; u++)
+        if (1 + u < n) {
+            p1[u] = C1[1 + u];
+            p2[u] = C2[1 + u];
         }
-        double2 *Z = zs + i0;
-        for (int j = 1; j < n; j++) Z[j] = csub(Z[j], cmul(C1[j], Z[j - 1]));
-        for (int j = 0; j < n; j++) Z[j] = cdiv(Z[j], C2[j]);
-        for (int j = n - 2; j >= 0; j--) Z[j] = csub(Z[j], cmul(C1[j + 1], Z[j + 1]));
+    for (int k0 = 0; k0 < n - 1; k0 += PSV_L0_G) {
+        double2 c1v[PSV_L0_G], c2v[PSV_L0_G];
+        for (int u = 0; u < PSV_L0_G; u++) {
+            c1v[u] = p1[u];
+            c2v[u] = p2[u];
+        }
+        for (int u = 0; u < PSV_L0_G; u++)  // next group, in flight during this one
+            if (k0 + PSV_L0_G + 1 + u < n) {
+                p1[u] = C1[k0 + PSV_L0_G + 1 + u];
+                p2[u] = C2[k0 + PSV_L0_G + 1 + u];
+            }
+        for (int u = 0; u < PSV_L0_G && k0 + u < n - 1; u++) {
+            const double2 piv = crecip(d);
+            const double2 zz = cneg(c1v[u]);
+            const double2 w = cmul(c1v[u], piv);
+            d = cadd(c2v[u], cmul(w, zz));
+            C1[k0 + u + 1] = w;
+            C2[k0 + u + 1] = d;
+        }
     }
-    // coefficient rows at l = 0 (V = 0; F1 vanishes at l = 0)
+}
+
+// dcsbdlv0, m = 1, right-hand side s: 0, 1 the source vectors at the
+// nodes js, js+1; 2 the back-propagated force -c0 at the surface node.
+// Forward z(j) -= c1(j) z(j-1), then z(j) /= c2(j), then backward
+// z(j) -= c1(j+1) z(j+1); the division is done as each forward value is
+// final (same operations).
+__host__ __device__ inline void psv_l0_subst(const PsvL0Args &a, int t, int s)
+{
+    const int sl = a.slot_list[t];
+    const int N0 = a.nn0, i0 = a.row0_0 - 1, n = N0 - i0;
+    const double2 *C1 = a.work + (size_t)t * 5 * N0 + i0, *C2 = C1 + N0;
+    double2 *zs = a.work + (size_t)t * 5 * N0 + (size_t)(2 + s) * N0;
+    for (int i = 0; i < i0; i++) zs[i] = cz();
+    double2 *Z = zs + i0;
+    int ja = 0, jb = -1;
+    double2 va = cz(), vb = cz();
+    if (s < 2) {
+        double2 g[2][2];
+        psv_excitation_l0(a.src[sl], a.sc, g);
+        ja = a.js - i0;
+        jb = ja + 1;
+        va = g[s][0];
+        vb = g[s][1];
+    } else {
+        const double c0f = (double)(float)sqrt(1.0 / (4.0 * DSM_PI) / 1.0);
+        ja = n - 1;
+        va = make_double2(-c0f, 0.0);
+    }
+    // forward: zero below the first nonzero entry
+    for (int j = 0; j < ja; j++) Z[j] = cz();
+    double2 zprev = cz();
+    for (int j0 = ja; j0 < n; j0 += PSV_L0_G) {
+        double2 c1v[PSV_L0_G], c2v[PSV_L0_G];
+        for (int u = 0; u < PSV_L0_G; u++)
+            if (j0 + u < n) {
+                c1v[u] = C1[j0 + u];
+                c2v[u] = C2[j0 + u];
+            }
+        for (int u = 0; u < PSV_L0_G && j0 + u < n; u++) {
+            const int j = j0 + u;
+            const double2 b = (j == ja) ? va : (j == jb) ? vb : cz();
+            const double2 f = (j == 0) ? b : csub(b, cmul(c1v[u], zprev));
+            zprev = f;
+            Z[j] = cdiv(f, c2v[u]);
+        }
+    }
+    // backward
+    double2 x = Z[n - 1];
+    for (int j1 = n - 2; j1 >= 0; j1 -= PSV_L0_G) {
+        double2 c1v[PSV_L0_G], zv[PSV_L0_G];
+        for (int u = 0; u < PSV_L0_G; u++)
+            if (j1 - u >= 0) {
+                c1v[u] = C1[j1 - u + 1];
+                zv[u] = Z[j1 - u];
+            }
+        for (int u = 0; u < PSV_L0_G && j1 - u >= 0; u++) {
+            x = csub(zv[u], cmul(c1v[u], x));
+            Z[j1 - u] = x;
+        }
+    }
+}
+
+// coefficient rows at l = 0 (V = 0; F1 vanishes at l = 0) of receiver r;
+// r == r_n: the SYNN rows
+__host__ __device__ inline void psv_l0_rows(const PsvL0Args &a, int t, int r)
+{
+    const int sl = a.slot_list[t];
+    const int N0 = a.nn0;
+    const double2 *z = a.work + (size_t)t * 5 * N0 + (size_t)2 * N0;
     const int R = a.r_n, LC = a.lcap;
-    double2 *E = a.E + (size_t)sl * PSV_NQ * R * LC;
-    for (int r = 0; r < R; r++) {
+    if (r < R) {
+        double2 *E = a.E + (size_t)sl * PSV_NQ * R * LC;
         const int j0 = a.jsta0[r] - 1;
         const double *w0 = a.w0 + 3 * r, *w1 = a.w1 + 3 * r;
         PsvVals v[PSV_NSOL];
@@ -842,14 +919,24 @@ __host__ __device__ inline void psv_l0_lane(const PsvL0Args &a, int t)
         }
         psv_rows(v, a.fluid[r] != 0, a.fluidK[(size_t)sl * R + r], 0.0, a.irf[r], false, false,
                  [&](int q, double2 val) { E[(size_t)(q * R + r) * LC] = val; });
+    } else if (r == R) {
+        double2 us[PSV_NSOL], vs[PSV_NSOL], es[PSV_NQS];
+        for (int s = 0; s < PSV_NSOL; s++) us[s] = vs[s] = cz();
+        us[0] = z[N0 - 1];
+        us[1] = z[(size_t)N0 + N0 - 1];
+        psv_synn_rows(us, vs, 0.0, false, false, es);
+        double2 *Es = a.Es + (size_t)sl * PSV_NQS * LC;
+        for (int q = 0; q < PSV_NQS; q++) Es[(size_t)q * LC] = es[q];
     }
-    double2 us[PSV_NSOL], vs[PSV_NSOL], es[PSV_NQS];
-    for (int s = 0; s < PSV_NSOL; s++) us[s] = vs[s] = cz();
-    us[0] = z[N0 - 1];
-    us[1] = z[(size_t)N0 + N0 - 1];
-    psv_synn_rows(us, vs, 0.0, false, false, es);
-    double2 *Es = a.Es + (size_t)sl * PSV_NQS * LC;
-    for (int q = 0; q < PSV_NQS; q++) Es[(size_t)q * LC] = es[q];
+}
+
+// all phases for frequency t in one thread (host build)
+__host__ __device__ inline void psv_l0_serial(const PsvL0Args &a, int t)
+{
+    for (int i = 0; i < a.nn0; i++) psv_l0_assemble(a, t, i);
+    psv_l0_factor(a, t);
+    for (int s = 0; s < 3; s++) psv_l0_subst(a, t, s);
+    for (int r = 0; r <= a.r_n; r++) psv_l0_rows(a, t, r);
 }
 
 }  // namespace dsm
