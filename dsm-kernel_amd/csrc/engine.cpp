@@ -83,6 +83,10 @@ struct dsm_sgt {
     std::vector<char> done;   // per position: computed
     // device buffers
     DevBuf d_theta, d_tab, d_tab32, d_need, d_recslot, d_w0, d_w1, d_rinv, d_src;
+    // BF16 forms: split operands, distance symmetry
+    int ns = 0;                // bf16 terms per operand (0: not a BF16 form)
+    int mirror = 0, nh = 0, nhp = 0;
+    DevBuf d_tb, d_eb, d_esb, d_stash;
     DevBuf d_band, d_om, d_slots, d_nl, d_slotout, d_scratch, d_xn, d_E, d_Es, d_amp, d_jobs;
     DevBuf d_cat_t, d_cat_r, d_cat_s;
     // P-SV only
@@ -138,21 +142,96 @@ static void ev_collect(dsm_sgt_t *h)
     h->pending.clear();
 }
 
-// Synthesis launch in the engine's precision.
-static void launch_synth(dsm_sgt_t *h, const SynthArgs &ga)
+// Synthesis launch in the engine's precision; Eb is the bf16 split of
+// ga.E (BF16 forms).
+static void launch_synth(dsm_sgt_t *h, const SynthArgs &ga, const DevBuf &Eb)
 {
-    if (h->precision == DSM_SGT_FP32) launch_synth_f32(ga, h->d_tab32.as<float>(), h->stream);
-    else launch_synth_f64(ga, h->stream);
+    if (h->ns) {
+        SynthB16Args b{};
+        b.Eb = Eb.as<unsigned short>();
+        b.Tb = h->d_tb.as<unsigned short>();
+        b.nq = ga.nq;
+        b.r_eff = ga.r_eff;
+        b.nslot = ga.nslot;
+        b.lcap = ga.lcap;
+        b.nplanes = (h->mode == DSM_SGT_PSV) ? PSV_NPLANES : 12;
+        b.theta_n = ga.theta_n;
+        b.nh = h->nh;
+        b.nhp = h->nhp;
+        b.mirror = h->mirror;
+        b.nl = ga.nl;
+        b.slot_out = ga.slot_out;
+        b.jobs = ga.jobs;
+        b.njobs = ga.njobs;
+        b.stash = h->d_stash.as<double>();
+        launch_synth_b16(b, h->ns, h->stream);
+    } else if (h->precision == DSM_SGT_FP32) {
+        launch_synth_f32(ga, h->d_tab32.as<float>(), h->stream);
+    } else {
+        launch_synth_f64(ga, h->stream);
+    }
 }
 
-// Rounded copy of the theta table for the mixed forms.
+// bf16 split of the coefficient rows E [slot][nq][r_eff][lcap] into Eb
+static int split_rows(dsm_sgt_t *h, const DevBuf &E, int nslot, int nq, int r_eff, DevBuf &Eb)
+{
+    if (!h->ns) return DSM_SGT_OK;
+    launch_split_rows(E.as<double2>(), nslot, nq, r_eff, h->lcap, h->ns, Eb.as<unsigned short>(), h->stream);
+    HIPCHK(h, hipGetLastError());
+    return DSM_SGT_OK;
+}
+
+// Distance symmetry of the BF16 forms: the grid is symmetric about 90
+// degrees and every job's functions share one parity under
+// theta -> pi - theta.  par[b]: parity of table plane b without (-1)^l.
+static void set_mirror(dsm_sgt_t *h, const int *par)
+{
+    h->mirror = 0;
+    const int n = h->theta_n;
+    bool sym = h->ns > 0 && n >= 2;
+    for (int i = 0; sym && i < n; i++)
+        if (std::fabs(h->theta_deg[i] + h->theta_deg[n - 1 - i] - 180.0) > 1e-9) sym = false;
+    for (auto &j : h->jobs) {
+        j.msign = par[j.b[0]];
+        for (int t = 1; t < j.nb; t++)
+            if (par[j.b[t]] != par[j.b[0]]) sym = false;
+    }
+    if (const char *e = std::getenv("DSM_SGT_MIRROR"))
+        if (std::atoi(e) == 0) sym = false;
+    h->mirror = sym ? 1 : 0;
+    h->nh = sym ? (n + 1) / 2 : n;
+    h->nhp = ((h->nh + 127) / 128) * 128;
+}
+
+// Tables of the mixed forms: FP32 copy, or the bf16 split (after set_mirror).
 static int make_mixed_tables(dsm_sgt_t *h, size_t nplanes)
 {
     if (h->precision == DSM_SGT_FP64) return DSM_SGT_OK;
+    if (h->ns) {
+        HIPCHK(h, h->d_tb.ensure(sizeof(unsigned short) * h->ns * nplanes * (size_t)h->nhp * h->lcap));
+        launch_split_table(h->d_tab.as<double>(), (int)nplanes, h->lcap, h->theta_pad, h->nh, h->nhp, h->ns,
+                           h->d_tb.as<unsigned short>(), h->stream);
+        HIPCHK(h, hipGetLastError());
+        return DSM_SGT_OK;
+    }
     const size_t n = nplanes * (size_t)h->lcap * h->theta_pad;
     HIPCHK(h, h->d_tab32.ensure(sizeof(float) * n));
     launch_tab_to_f32(h->d_tab.as<double>(), h->d_tab32.as<float>(), n, h->stream);
     HIPCHK(h, hipGetLastError());
+    return DSM_SGT_OK;
+}
+
+// Device buffers of the BF16 forms for nslot frequencies: split rows and
+// the stash of the largest launch.
+static int alloc_b16(dsm_sgt_t *h, int nslot, int nq, int nqs, int R)
+{
+    if (!h->ns) return DSM_SGT_OK;
+    const size_t M = (size_t)nslot * R * 2, Ms = (size_t)nslot * 2;
+    HIPCHK(h, h->d_eb.ensure(sizeof(unsigned short) * nq * h->ns * M * h->lcap));
+    HIPCHK(h, h->d_esb.ensure(sizeof(unsigned short) * nqs * h->ns * Ms * h->lcap));
+    size_t g = synth_b16_grid(nslot, R, h->nhp, std::max(h->njobs_main, h->njobs_acc));
+    g = std::max(g, synth_b16_grid(nslot, 1, h->nhp, h->njobs_syn));
+    HIPCHK(h, h->d_stash.ensure(sizeof(double) * g * 8 * 32 * 64));
     return DSM_SGT_OK;
 }
 
@@ -312,6 +391,16 @@ static int sh_create(dsm_sgt_t *h)
     HIPCHK(h, hipMemsetAsync(h->d_cat_r.p, 0, h->d_cat_r.n, h->stream));
     HIPCHK(h, hipMemsetAsync(h->d_cat_s.p, 0, h->d_cat_s.n, h->stream));
     sh_synth_jobs(h);
+    {
+        // parity of Y0..Y5 (legendre_lane.h) times (-1)^mu
+        static const int fpar[6] = {1, -1, -1, 1, 1, -1};
+        int par[12];
+        for (int mu = 1; mu <= 2; mu++)
+            for (int b = 0; b < 6; b++) par[(mu - 1) * 6 + b] = fpar[b] * ((mu & 1) ? -1 : 1);
+        set_mirror(h, par);
+    }
+    rc = alloc_b16(h, S, 36, 5, R);
+    if (rc) return rc;
     HIPCHK(h, h->d_jobs.ensure(sizeof(SynthJob) * h->jobs.size()));
     HIPCHK(h, hipMemcpy(h->d_jobs.p, h->jobs.data(), sizeof(SynthJob) * h->jobs.size(), hipMemcpyHostToDevice));
 
@@ -469,6 +558,9 @@ static int sh_run_batch(dsm_sgt_t *h, const std::vector<int> &freqs)
     launch_zero_tail(h->d_E.as<double2>(), S, 36 * R, h->d_slots.as<int>(), h->d_nl.as<int>(), LC, h->stream);
     launch_zero_tail(h->d_Es.as<double2>(), S, 5, h->d_slots.as<int>(), h->d_nl.as<int>(), LC, h->stream);
     HIPCHK(h, hipGetLastError());
+    int rc = split_rows(h, h->d_E, S, 36, R, h->d_eb);
+    if (!rc) rc = split_rows(h, h->d_Es, S, 5, 1, h->d_esb);
+    if (rc) return rc;
     ev_end(h, tz);
 
     SynthArgs ga{};
@@ -486,7 +578,7 @@ static int sh_run_batch(dsm_sgt_t *h, const std::vector<int> &freqs)
     ga.njobs = h->njobs_main;
     TimedLaunch tg;
     ev_begin(h, tg, 2);
-    launch_synth(h, ga);
+    launch_synth(h, ga, h->d_eb);
     HIPCHK(h, hipGetLastError());
     ev_end(h, tg);
     // SYNN rows (surface node, no receiver dimension)
@@ -497,7 +589,7 @@ static int sh_run_batch(dsm_sgt_t *h, const std::vector<int> &freqs)
     ga.njobs = h->njobs_syn;
     TimedLaunch tg2;
     ev_begin(h, tg2, 3);
-    launch_synth(h, ga);
+    launch_synth(h, ga, h->d_esb);
     HIPCHK(h, hipGetLastError());
     ev_end(h, tg2);
     h->stats.synth_launches += 1;
@@ -596,6 +688,19 @@ static int psv_create(dsm_sgt_t *h)
     HIPCHK(h, hipMemsetAsync(h->d_cat_r.p, 0, h->d_cat_r.n, h->stream));
     HIPCHK(h, hipMemsetAsync(h->d_cat_s.p, 0, h->d_cat_s.n, h->stream));
     psv_synth_jobs(h);
+    {
+        // parity of F0..F9 (legendre_lane.h) times (-1)^mu
+        static const int fpar[10] = {1, -1, 1, -1, 1, 1, -1, -1, 1, 1};
+        int par[PSV_NPLANES];
+        for (int mu = 0; mu <= 2; mu++)
+            for (int f = 0; f < 10; f++) {
+                if (mu == 0 && !(f <= 2 || f == 9)) continue;
+                par[psv_plane(mu, f)] = fpar[f] * ((mu & 1) ? -1 : 1);
+            }
+        set_mirror(h, par);
+    }
+    rc = alloc_b16(h, S, PSV_NQ, PSV_NQS, R);
+    if (rc) return rc;
     HIPCHK(h, h->d_jobs.ensure(sizeof(SynthJob) * h->jobs.size()));
     HIPCHK(h, hipMemcpy(h->d_jobs.p, h->jobs.data(), sizeof(SynthJob) * h->jobs.size(), hipMemcpyHostToDevice));
 
@@ -791,6 +896,9 @@ static int psv_run_batch(dsm_sgt_t *h, const std::vector<int> &freqs)
     launch_zero_tail(h->d_E.as<double2>(), S, PSV_NQ * R, h->d_slots.as<int>(), h->d_nl.as<int>(), LC, h->stream);
     launch_zero_tail(h->d_Es.as<double2>(), S, PSV_NQS, h->d_slots.as<int>(), h->d_nl.as<int>(), LC, h->stream);
     HIPCHK(h, hipGetLastError());
+    int rc = split_rows(h, h->d_E, S, PSV_NQ, R, h->d_eb);
+    if (!rc) rc = split_rows(h, h->d_Es, S, PSV_NQS, 1, h->d_esb);
+    if (rc) return rc;
     ev_end(h, tz);
 
     SynthArgs ga{};
@@ -808,11 +916,11 @@ static int psv_run_batch(dsm_sgt_t *h, const std::vector<int> &freqs)
     ga.njobs = h->njobs_main;
     TimedLaunch tg;
     ev_begin(h, tg, 2);
-    launch_synth(h, ga);
+    launch_synth(h, ga, h->d_eb);
     HIPCHK(h, hipGetLastError());
     ga.jobs = h->d_jobs.as<SynthJob>() + h->njobs_main;
     ga.njobs = h->njobs_acc;
-    launch_synth(h, ga);
+    launch_synth(h, ga, h->d_eb);
     HIPCHK(h, hipGetLastError());
     ev_end(h, tg);
     ga.E = h->d_Es.as<double2>();
@@ -822,7 +930,7 @@ static int psv_run_batch(dsm_sgt_t *h, const std::vector<int> &freqs)
     ga.njobs = h->njobs_syn;
     TimedLaunch tg2;
     ev_begin(h, tg2, 3);
-    launch_synth(h, ga);
+    launch_synth(h, ga, h->d_esb);
     HIPCHK(h, hipGetLastError());
     ev_end(h, tg2);
     h->stats.synth_launches += 2;
@@ -857,10 +965,11 @@ int dsm_sgt_create(const dsm_sgt_input_t *in, int32_t mode, int32_t precision, i
             rc = fail(h, DSM_SGT_EINVAL, "unknown mode " + std::to_string(mode));
             break;
         }
-        if (precision != DSM_SGT_FP64 && precision != DSM_SGT_FP32) {
-            rc = fail(h, DSM_SGT_EUNSUPPORTED, "precision not available in this build");
+        if (precision < DSM_SGT_FP64 || precision > DSM_SGT_BF16X6) {
+            rc = fail(h, DSM_SGT_EINVAL, "unknown precision " + std::to_string(precision));
             break;
         }
+        h->ns = (precision == DSM_SGT_BF16X3) ? 2 : (precision == DSM_SGT_BF16X6) ? 3 : 0;
         int ndev = 0;
         if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev) {
             rc = fail(h, DSM_SGT_EHIP, "no HIP device " + std::to_string(device));

@@ -42,6 +42,7 @@ struct SynthJob {
     int accumulate;        // add into the targets instead of writing them
     int ncomp;             // record width
     double2 *out;          // catalog of this kind: [pos][r_eff][theta_n][ncomp]
+    double msign;          // F(l, pi - theta) = msign (-1)^l F(l, theta) for all the job's functions
 };
 struct SynthArgs {
     const double2 *E;      // [slot][nq][r_eff][lcap]
@@ -59,5 +60,26 @@ void launch_synth_f64(const SynthArgs &a, hipStream_t s);
 // across 32-wide K chunks; tab32 is the theta table rounded to FP32.
 void launch_synth_f32(const SynthArgs &a, const float *tab32, hipStream_t s);
 void launch_tab_to_f32(const double *src, float *dst, size_t n, hipStream_t s);
+
+// Mixed forms on BF16 MFMA (synth_b16.hip): operands split into ns = 2 or
+// 3 bf16 terms, degrees stored even-l first; with `mirror` the columns
+// c < nh = (theta_n+1)/2 give distances c and theta_n-1-c.
+struct SynthB16Args {
+    const unsigned short *Eb;  // [q][ns][m][lcap], m = (slot r_eff + r) 2 + re|im
+    const unsigned short *Tb;  // [ns][plane][nhp][lcap]
+    int nq, r_eff, nslot, lcap, nplanes;
+    int theta_n, nh, nhp, mirror;
+    const int *nl;
+    const int *slot_out;
+    const SynthJob *jobs;
+    int njobs;
+    double *stash;             // [workgroup][8 waves][32][64 lanes]
+};
+void launch_split_rows(const double2 *E, int nslot, int nq, int r_eff, int lcap, int ns, unsigned short *Eb,
+                       hipStream_t s);
+void launch_split_table(const double *tab, int nplanes, int lcap, int theta_pad, int nh, int nhp, int ns,
+                        unsigned short *Tb, hipStream_t s);
+size_t synth_b16_grid(int nslot, int r_eff, int nhp, int njobs);
+void launch_synth_b16(const SynthB16Args &a, int ns, hipStream_t s);
 
 }  // namespace dsm

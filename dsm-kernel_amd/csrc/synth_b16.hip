@@ -1,0 +1,346 @@
+// synth_b16.hip -- the synthesis contraction of the mixed forms on
+// v_mfma_f32_16x16x32_bf16.
+//
+// Operands.  Every FP64 operand x (coefficient rows E, theta functions T)
+// is split into NS bfloat16 terms x = x0 + x1 (+ x2) + residual, each term
+// the bf16 rounding of what the previous ones leave (residual ~2^-17 |x|
+// for NS = 2, ~2^-25 |x| for NS = 3).  The product is the sum of the
+// cross products of order < NS:
+//   NS = 2 (BF16X3): x0 y0 + x0 y1 + x1 y0
+//   NS = 3 (BF16X6): the above + x0 y2 + x1 y1 + x2 y0
+// issued small terms first into one FP32 MFMA accumulator per 32-wide K
+// chunk, which is then added into FP64; so the error does not grow with
+// the length of the l series.
+//
+// Degree order.  Both operands store the degrees permuted, even l first
+// (l -> l/2) and odd l after (l -> lcap/2 + l/2), so every K chunk holds
+// degrees of one parity.  On a distance grid symmetric about 90 degrees
+// the functions obey F(l, pi - theta) = sigma (-1)^l F(l, theta) with
+// sigma = +-1 fixed per catalog component (the parity of the function
+// times (-1)^|m|), so with e and o the even-l and odd-l partial sums at
+// theta,
+//   out(theta) = e + o,   out(pi - theta) = sigma (e - o),
+// and only the columns theta <= 90 degrees are computed.  The even half is
+// accumulated first and parked in a per-workgroup stash (coalesced, one
+// FP64 per lane and register) while the odd half accumulates in the same
+// registers.
+//
+// Tile: 128 x 128 per 512-thread workgroup, 8 waves in 2 x 4, each wave
+// 4 x 2 blocks of 16 x 16; operands staged through LDS with a register-
+// staged double buffer, rows padded to 80 bytes (conflict-free 16-byte
+// fragment reads).
+#include "dev_common.h"
+#include "kernels.h"
+
+namespace dsm {
+
+namespace {
+
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+typedef unsigned short u16;
+
+constexpr int BM = 128, BN = 128, BK = 32, NT = 512;
+constexpr int LDSW = 40;  // u16 per LDS row: 32 degrees + 8 pad
+
+__host__ __device__ inline int lperm(int l, int lcap) { return (l & 1) ? (lcap >> 1) + (l >> 1) : (l >> 1); }
+
+__device__ __forceinline__ u16 bf16_rn(float f)
+{
+    unsigned u = __float_as_uint(f);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (u16)(u >> 16);
+}
+__device__ __forceinline__ double bf16_value(u16 h) { return (double)__uint_as_float((unsigned)h << 16); }
+
+// x0, x1, ...: each the bf16 rounding of the remainder (the remainder is
+// exact in FP64)
+__device__ __forceinline__ void split_terms(double x, int ns, u16 *t)
+{
+    double r = x;
+    for (int s = 0; s < ns; s++) {
+        t[s] = bf16_rn((float)r);
+        r -= bf16_value(t[s]);
+    }
+}
+
+// XCD-sliced tile order, as in synth.hip
+__device__ __forceinline__ bool xcd_tile_b16(int nx, int ny, int &tx, int &ty, int &tz)
+{
+    constexpr int NXCD = 8;
+    const int T = nx * ny, cpx = (T + NXCD - 1) / NXCD;
+    const int pid = blockIdx.x, xcd = pid % NXCD, local = pid / NXCD;
+    tz = local / cpx;
+    const int t = xcd * cpx + (local - tz * cpx);
+    if (t >= T) return false;
+    ty = t / nx;
+    tx = t - ty * nx;
+    return true;
+}
+
+template <int NS>
+struct StageB16 {
+    uint4 a[NS], b[NS];
+};
+
+}  // namespace
+
+// E [slot][q][r][lcap] (FP64 complex) -> Eb [q][s][m][lcap] (bf16 terms,
+// degrees permuted), m = (slot r_eff + r) 2 + re|im
+__global__ void split_rows_kernel(const double2 *E, int nslot, int nq, int R, int lcap, int ns, u16 *Eb)
+{
+    const size_t total = (size_t)nslot * nq * R * lcap;
+    const size_t M = (size_t)nslot * R * 2;
+    for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+         idx += (size_t)gridDim.x * blockDim.x) {
+        const int l = (int)(idx % lcap);
+        size_t rest = idx / lcap;
+        const int r = (int)(rest % R);
+        rest /= R;
+        const int q = (int)(rest % nq);
+        const int sl = (int)(rest / nq);
+        const double2 v = E[idx];
+        const size_t m = ((size_t)sl * R + r) * 2;
+        const int lp = lperm(l, lcap);
+        u16 tr[3], ti[3];
+        split_terms(v.x, ns, tr);
+        split_terms(v.y, ns, ti);
+        for (int s = 0; s < ns; s++) {
+            u16 *row = Eb + (((size_t)q * ns + s) * M + m) * lcap + lp;
+            row[0] = tr[s];
+            row[lcap] = ti[s];
+        }
+    }
+}
+
+// tab [plane][lcap][theta_pad] (FP64) -> Tb [s][plane][nhp][lcap], columns
+// c < nh taken from distance c, the others zero
+__global__ void split_table_kernel(const double *tab, int nplanes, int lcap, int theta_pad, int nh, int nhp, int ns,
+                                   u16 *Tb)
+{
+    const size_t total = (size_t)nplanes * nhp * lcap;
+    for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+         idx += (size_t)gridDim.x * blockDim.x) {
+        const int l = (int)(idx % lcap);
+        size_t rest = idx / lcap;
+        const int c = (int)(rest % nhp);
+        const int p = (int)(rest / nhp);
+        const double v = (c < nh) ? tab[((size_t)p * lcap + l) * theta_pad + c] : 0.0;
+        u16 t[3];
+        split_terms(v, ns, t);
+        for (int s = 0; s < ns; s++) Tb[(((size_t)s * nplanes + p) * nhp + c) * lcap + lperm(l, lcap)] = t[s];
+    }
+}
+
+// job: the workgroup's job in global memory (read with uniform loads; a
+// private copy indexed by jb would live in scratch)
+template <int NS>
+__device__ __forceinline__ void load_stage(const SynthB16Args &a, const SynthJob *job, int M, int m0, int n0, int jb,
+                                           int lp0, int tid, StageB16<NS> &st)
+{
+    const int row = tid >> 2, seg = tid & 3;
+    const bool okr = (m0 + row) < M;
+    const int plane = job->b[jb];
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+        const u16 *pa = a.Eb + (((size_t)(job->q0 + jb) * NS + s) * M + m0 + row) * a.lcap + lp0 + 8 * seg;
+        st.a[s] = okr ? *(const uint4 *)pa : make_uint4(0, 0, 0, 0);
+        const u16 *pb = a.Tb + (((size_t)s * a.nplanes + plane) * a.nhp + n0 + row) * a.lcap + lp0 + 8 * seg;
+        st.b[s] = *(const uint4 *)pb;
+    }
+}
+
+template <int NS>
+__device__ __forceinline__ void store_stage(u16 *As, u16 *Bs, int tid, const StageB16<NS> &st)
+{
+    const int row = tid >> 2, seg = tid & 3;
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+        *(uint4 *)&As[(s * BM + row) * LDSW + 8 * seg] = st.a[s];
+        *(uint4 *)&Bs[(s * BN + row) * LDSW + 8 * seg] = st.b[s];
+    }
+}
+
+template <int NS, int KIND>  // KIND 0: receiver rows, 1: surface rows (separate symbol for profiling)
+__global__ __launch_bounds__(512) void synth_b16_kernel(SynthB16Args a, int nx, int ny)
+{
+    __shared__ __attribute__((aligned(16))) u16 As[2][NS * BM * LDSW];
+    __shared__ __attribute__((aligned(16))) u16 Bs[2][NS * BN * LDSW];
+    int tx, ty, tz;
+    if (!xcd_tile_b16(nx, ny, tx, ty, tz)) return;
+    const SynthJob *job = a.jobs + tz;
+    const int nb = job->nb;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int R = a.r_eff;
+    const int M = a.nslot * R * 2;
+    const int m0 = ty * BM, n0 = tx * BN;
+    int kmax = 0;
+    {
+        const int s0 = m0 / (2 * R), s1 = min(M - 1, m0 + BM - 1) / (2 * R);
+        for (int sl = s0; sl <= s1; sl++) kmax = max(kmax, a.nl[sl]);
+    }
+    const int ce = ((kmax + 1) / 2 + BK - 1) / BK;  // chunks of even degrees l < kmax
+    const int co = (kmax / 2 + BK - 1) / BK;        // odd
+    const int LH = a.lcap >> 1;
+    const int nit_e = nb * ce, nit = nit_e + nb * co;
+    const int wr = wave >> 2, wc = wave & 3;
+    const int fr = lane & 15, fk = 8 * (lane >> 4);
+
+    v4d acc64[4][2];
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++) acc64[i][j] = (v4d){0.0, 0.0, 0.0, 0.0};
+    double *stash = a.stash + (((size_t)blockIdx.x * 8 + wave) * 32) * 64 + lane;
+    bool stashed = false;
+
+    auto itmap = [&](int it, int &jb, int &lp0) {
+        if (it < nit_e) {
+            jb = it / ce;
+            lp0 = (it - jb * ce) * BK;
+        } else {
+            const int u = it - nit_e;
+            jb = u / co;
+            lp0 = LH + (u - jb * co) * BK;
+        }
+    };
+
+    StageB16<NS> stg;
+    if (nit > 0) {
+        int jb, lp0;
+        itmap(0, jb, lp0);
+        load_stage<NS>(a, job, M, m0, n0, jb, lp0, tid, stg);
+        store_stage<NS>(As[0], Bs[0], tid, stg);
+    }
+    __syncthreads();
+    for (int it = 0; it < nit; it++) {
+        const int st = it & 1;
+        if (it + 1 < nit) {
+            int jb, lp0;
+            itmap(it + 1, jb, lp0);
+            load_stage<NS>(a, job, M, m0, n0, jb, lp0, tid, stg);
+        }
+        v8bf af[NS][4], bfg[NS][2];
+#pragma unroll
+        for (int s = 0; s < NS; s++) {
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+                af[s][i] = *(const v8bf *)&As[st][(s * BM + wr * 64 + i * 16 + fr) * LDSW + fk];
+#pragma unroll
+            for (int j = 0; j < 2; j++)
+                bfg[s][j] = *(const v8bf *)&Bs[st][(s * BN + wc * 32 + j * 16 + fr) * LDSW + fk];
+        }
+        v4f acc[4][2];
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+            for (int j = 0; j < 2; j++) acc[i][j] = (v4f){0.f, 0.f, 0.f, 0.f};
+        // cross products of order < NS, smallest first
+#pragma unroll
+        for (int ord = NS - 1; ord >= 0; ord--)
+#pragma unroll
+            for (int sa = ord; sa >= 0; sa--) {
+                const int sb = ord - sa;
+#pragma unroll
+                for (int i = 0; i < 4; i++)
+#pragma unroll
+                    for (int j = 0; j < 2; j++)
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[sa][i], bfg[sb][j], acc[i][j], 0, 0, 0);
+            }
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+            for (int j = 0; j < 2; j++)
+#pragma unroll
+                for (int q = 0; q < 4; q++) acc64[i][j][q] += (double)acc[i][j][q];
+        if (a.mirror && it == nit_e - 1) {
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+#pragma unroll
+                for (int j = 0; j < 2; j++)
+#pragma unroll
+                    for (int q = 0; q < 4; q++) {
+                        stash[(size_t)((i * 2 + j) * 4 + q) * 64] = acc64[i][j][q];
+                        acc64[i][j][q] = 0.0;
+                    }
+            stashed = true;
+        }
+        if (it + 1 < nit) store_stage<NS>(As[st ^ 1], Bs[st ^ 1], tid, stg);
+        __syncthreads();
+    }
+    // epilogue: v_mfma_f32_16x16x32_bf16 leaves D[i][j] in lane j + 16*(i/4), register i%4
+    double *out = (double *)job->out;
+    const int ncomp = job->ncomp, accum = job->accumulate;
+    const int comp0 = job->comp[0], comp1 = job->comp[1];
+    const double sign0 = job->sign[0], sign1 = job->sign[1], msign = job->msign;
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int row = m0 + wr * 64 + i * 16 + 4 * (lane >> 4) + q;
+            if (row >= M) continue;
+            const int pr = row >> 1, reim = row & 1;
+            const int sl = pr / R, r = pr - sl * R;
+            const size_t base = ((size_t)a.slot_out[sl] * R + r) * a.theta_n;
+#pragma unroll
+            for (int j = 0; j < 2; j++) {
+                const int col = n0 + wc * 32 + j * 16 + fr;
+                if (col >= a.nh) continue;
+                const double o = acc64[i][j][q];
+                const double e = stashed ? stash[(size_t)((i * 2 + j) * 4 + q) * 64] : 0.0;
+                const double dv = e + o;
+                const int mc = a.theta_n - 1 - col;
+                const bool mir = a.mirror && mc != col;
+                const double mv = msign * (e - o);
+#pragma unroll
+                for (int t = 0; t < 2; t++) {
+                    const int comp = t ? comp1 : comp0;
+                    if (comp < 0) continue;
+                    const double sg = t ? sign1 : sign0;
+                    double *od = out + 2 * ((base + col) * ncomp + comp) + reim;
+                    const double v = sg * dv;
+                    *od = accum ? *od + v : v;
+                    if (mir) {
+                        double *om = out + 2 * ((base + mc) * ncomp + comp) + reim;
+                        const double w = sg * mv;
+                        *om = accum ? *om + w : w;
+                    }
+                }
+            }
+        }
+}
+
+void launch_split_rows(const double2 *E, int nslot, int nq, int r_eff, int lcap, int ns, unsigned short *Eb,
+                       hipStream_t s)
+{
+    hipLaunchKernelGGL(split_rows_kernel, dim3(8192), dim3(256), 0, s, E, nslot, nq, r_eff, lcap, ns, Eb);
+}
+
+void launch_split_table(const double *tab, int nplanes, int lcap, int theta_pad, int nh, int nhp, int ns,
+                        unsigned short *Tb, hipStream_t s)
+{
+    hipLaunchKernelGGL(split_table_kernel, dim3(8192), dim3(256), 0, s, tab, nplanes, lcap, theta_pad, nh, nhp, ns,
+                       Tb);
+}
+
+size_t synth_b16_grid(int nslot, int r_eff, int nhp, int njobs)
+{
+    const int M = nslot * r_eff * 2;
+    const int nx = nhp / BN, ny = (M + BM - 1) / BM;
+    return (size_t)8 * ((nx * ny + 7) / 8) * njobs;
+}
+
+void launch_synth_b16(const SynthB16Args &a, int ns, hipStream_t s)
+{
+    const int M = a.nslot * a.r_eff * 2;
+    const int nx = a.nhp / BN, ny = (M + BM - 1) / BM;
+    const dim3 grid((unsigned)synth_b16_grid(a.nslot, a.r_eff, a.nhp, a.njobs));
+    if (ns == 3) {
+        if (a.r_eff > 1) hipLaunchKernelGGL((synth_b16_kernel<3, 0>), grid, dim3(NT), 0, s, a, nx, ny);
+        else hipLaunchKernelGGL((synth_b16_kernel<3, 1>), grid, dim3(NT), 0, s, a, nx, ny);
+    } else {
+        if (a.r_eff > 1) hipLaunchKernelGGL((synth_b16_kernel<2, 0>), grid, dim3(NT), 0, s, a, nx, ny);
+        else hipLaunchKernelGGL((synth_b16_kernel<2, 1>), grid, dim3(NT), 0, s, a, nx, ny);
+    }
+}
+
+}  // namespace dsm

@@ -17,7 +17,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(os.path.dirname(_HERE), "libdsmsgt.so")
 
 SH, PSV = 1, 2
-FP64, FP32, BF16X3 = 0, 1, 2
+FP64, FP32, BF16X3, BF16X6 = 0, 1, 2, 3
 TSGT, RSGT, SYNN = 0, 1, 2
 
 ERRORS = {1: "EINVAL", 2: "ESTATE", 3: "EHIP", 4: "ENOMEM", 5: "EIO", 6: "EUNSUPPORTED"}

@@ -4,7 +4,7 @@ module dsm_sgt_api
   implicit none
   integer, parameter :: DSM_SGT_MAXZONE = 64
   integer(c_int32_t), parameter :: DSM_SGT_SH = 1, DSM_SGT_PSV = 2
-  integer(c_int32_t), parameter :: DSM_SGT_FP64 = 0, DSM_SGT_FP32 = 1, DSM_SGT_BF16X3 = 2
+  integer(c_int32_t), parameter :: DSM_SGT_FP64 = 0, DSM_SGT_FP32 = 1, DSM_SGT_BF16X3 = 2, DSM_SGT_BF16X6 = 3
 
   ! dsm_sgt_input_t
   type, bind(c) :: dsm_sgt_input

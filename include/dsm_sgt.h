@@ -53,10 +53,12 @@ typedef struct dsm_sgt_input {
 } dsm_sgt_input_t;
 
 enum { DSM_SGT_SH = 1, DSM_SGT_PSV = 2 };
-/* Arithmetic of the synthesis contraction.  FP64 is the parity form;
- * FP32 and BF16X3 are the mixed forms (band solve and Legendre table stay
- * FP64 in all forms). */
-enum { DSM_SGT_FP64 = 0, DSM_SGT_FP32 = 1, DSM_SGT_BF16X3 = 2 };
+/* Arithmetic of the synthesis contraction.  FP64 is the parity form; the
+ * mixed forms are FP32 (FP32 MFMA), BF16X3 (operands split into two bf16
+ * terms, three products) and BF16X6 (three terms, six products), all with
+ * FP32 MFMA accumulation over 32 degrees added into FP64.  The band solve
+ * and the Legendre table stay FP64 in all forms. */
+enum { DSM_SGT_FP64 = 0, DSM_SGT_FP32 = 1, DSM_SGT_BF16X3 = 2, DSM_SGT_BF16X6 = 3 };
 /* Catalog kinds (file suffixes TSGT_*, RSGT_*, SYNN_*). */
 enum { DSM_SGT_TSGT = 0, DSM_SGT_RSGT = 1, DSM_SGT_SYNN = 2 };
 

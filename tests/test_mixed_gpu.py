@@ -1,0 +1,128 @@
+"""GPU accuracy of the mixed-precision forms of the synthesis contraction
+(FP32 MFMA; bf16 operand splits BF16X3 and BF16X6), through the C-ABI.
+
+Bar: relative L2 <= 1e-6 per catalog component against FP64 on the 5 s
+P-SV catalog grid (60 radii x 1801 distances), against the FP64 engine
+(itself within 1e-9 of the oracle).  BF16X3 (two bf16 terms per operand,
+three products) keeps ~16 significant bits and does not meet 1e-6; its
+measured error is bounded at 3e-5 and printed.
+
+The small grids (9 to 13 distances, a receiver at the source's depth) are
+checked against the oracle with looser bounds: there the slowly converging
+near-field series at the source depth, summed over few distances,
+amplifies the FP32 rounding of the per-32-degree MFMA sums ~30-100x
+(measured worst: BF16X6 2.3e-6 P-SV, 4.0e-6 SH; FP32 7.9e-6 P-SV, 1.8e-5
+SH, at the lowest frequencies).
+"""
+import os
+
+import numpy as np
+import pytest
+
+import dsm_inputs as di
+import oracle_lib as ol
+from parity import component_errors
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-6
+# small grids (see above): bound per form
+SMALL_TOL = {"FP32": 4e-5, "BF16X6": 8e-6}
+
+SMALL = dict(tlen=1638.4, imax=16, rmin_=1000., rmax_=6371., rdelta_=250., thetamin=0.,
+             thetamax=180., thetadelta=22.5, maxlmax=1200)
+FIVE_S = dict(tlen=3276.8, imax=656, rmin_=3716., rmax_=6371., rdelta_=45., thetamin=0.,
+              thetamax=180., thetadelta=0.1, maxlmax=8000)
+
+
+def _engine(inp, mode, prec, mirror=True):
+    import dsm_sgt
+    old = os.environ.get("DSM_SGT_MIRROR")
+    if not mirror:
+        os.environ["DSM_SGT_MIRROR"] = "0"
+    try:
+        return dsm_sgt.Engine(dsm_sgt.DsmSgtInput.from_buffer_copy(inp), mode=mode, precision=prec)
+    finally:
+        if old is None:
+            os.environ.pop("DSM_SGT_MIRROR", None)
+        else:
+            os.environ["DSM_SGT_MIRROR"] = old
+
+
+def _vs_oracle(eng, inp, freqs, fn, tol):
+    worst = 0.0
+    for i in freqs:
+        ref = fn(inp, i)
+        got = eng.fetch_all_fp64(i)
+        assert eng.lcut(i) == ref[3]
+        for name, x, y in zip(("tsgt", "rsgt", "synn"), got, ref[:3]):
+            e = component_errors(x, y)
+            worst = max(worst, e.max())
+            assert e.max() <= tol, (i, name, e)
+    return worst
+
+
+@pytest.mark.parametrize("prec_name", ["FP32", "BF16X6"])
+@pytest.mark.parametrize("mirror", [True, False])
+def test_psv_mixed_small_vs_oracle(prec_name, mirror):
+    """All frequencies of the small P-SV case (receivers in every region,
+    poles included), with and without the distance symmetry."""
+    import dsm_sgt
+    inp = di.synthetic_input(**SMALL)
+    eng = _engine(inp, dsm_sgt.PSV, getattr(dsm_sgt, prec_name), mirror)
+    eng.run(0, SMALL["imax"])
+    w = _vs_oracle(eng, inp, range(1, SMALL["imax"] + 1), lambda x, i: ol.psv_omega(x, i), SMALL_TOL[prec_name])
+    print(f"P-SV {prec_name} mirror={mirror}: max rel L2 {w:.2e}")
+
+
+def test_psv_mixed_asymmetric_grid():
+    """Distances 0..157.5 degrees: no symmetry, every column computed."""
+    import dsm_sgt
+    p = dict(SMALL, thetamax=157.5)
+    inp = di.synthetic_input(**p)
+    eng = _engine(inp, dsm_sgt.PSV, dsm_sgt.BF16X6)
+    eng.run(1, 6)
+    _vs_oracle(eng, inp, range(1, 7), lambda x, i: ol.psv_omega(x, i), SMALL_TOL["BF16X6"])
+
+
+@pytest.mark.parametrize("prec_name", ["FP32", "BF16X6"])
+def test_sh_mixed_small_vs_oracle(prec_name):
+    import dsm_sgt
+    p = dict(tlen=3276.8, imax=32, rmin_=4421., rmax_=6371., rdelta_=250., thetamin=0.,
+             thetamax=180., thetadelta=15., maxlmax=600)
+    inp = di.synthetic_input(**p)
+    eng = _engine(inp, dsm_sgt.SH, getattr(dsm_sgt, prec_name))
+    eng.run(1, p["imax"])
+    w = _vs_oracle(eng, inp, range(1, p["imax"] + 1), lambda x, i: ol.sh_omega(x, i), SMALL_TOL[prec_name])
+    print(f"SH {prec_name}: max rel L2 {w:.2e}")
+
+
+def _five_s(prec, freqs):
+    import dsm_sgt
+    inp = di.synthetic_input(**FIVE_S)
+    eng = _engine(inp, dsm_sgt.PSV, prec)
+    out = {}
+    for a, b in freqs:
+        eng.run(a, b)
+        for i in (a, b):
+            out[i] = eng.fetch_all_fp64(i)
+    eng.close()
+    return out
+
+
+def test_psv_mixed_5s_catalog_grid():
+    """The 5 s P-SV catalog's grid (60 radii x 1801 distances, lmax 8000,
+    3733 band rows): BF16X6 and FP32 against the FP64 engine at the
+    lowest and highest frequencies; BF16X3's error is measured."""
+    import dsm_sgt
+    freqs = ((1, 2), (655, 656))
+    ref = _five_s(dsm_sgt.FP64, freqs)
+    for name, tol in (("BF16X6", TOL), ("FP32", TOL), ("BF16X3", 3e-5)):
+        got = _five_s(getattr(dsm_sgt, name), freqs)
+        worst = 0.0
+        for i in ref:
+            for k, kind in enumerate(("tsgt", "rsgt", "synn")):
+                e = component_errors(got[i][k], ref[i][k])
+                worst = max(worst, e.max())
+                assert e.max() <= tol, (name, i, kind, e)
+        print(f"5 s grid {name}: max rel L2 per component {worst:.2e}", flush=True)
