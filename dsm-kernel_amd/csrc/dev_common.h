@@ -46,4 +46,96 @@ __host__ __device__ inline double2 caxpy(double2 a, double s, double2 b)
     return make_double2(a.x + s * b.x, a.y + s * b.y);
 }
 
+// ---- bfloat16 operand terms of the mixed forms: x = t0 + t1 (+ t2) +
+// residual, each term the round-to-nearest-even bf16 of what the previous
+// ones leave (the remainder is exact in FP64).
+__host__ __device__ inline unsigned short bf16_rn(float f)
+{
+    unsigned u;
+    __builtin_memcpy(&u, &f, 4);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (unsigned short)(u >> 16);
+}
+__host__ __device__ inline double bf16_value(unsigned short h)
+{
+    const unsigned u = (unsigned)h << 16;
+    float f;
+    __builtin_memcpy(&f, &u, 4);
+    return (double)f;
+}
+__host__ __device__ inline void bf16_terms(double x, int ns, unsigned short *t)
+{
+    double r = x;
+    for (int s = 0; s < ns; s++) {
+        t[s] = bf16_rn((float)r);
+        r -= bf16_value(t[s]);
+    }
+}
+// degree order of the bf16 operands: even l first, then odd l
+__host__ __device__ inline int lperm(int l, int lcap) { return (l & 1) ? (lcap >> 1) + (l >> 1) : (l >> 1); }
+// Destination of the bf16 coefficient rows: Eb [q][ns][m][lcap] with
+// receiver-major rows m = (r nslot + slot) 2 + re|im.
+struct B16Rows {
+    unsigned short *Eb;
+    int ns, nslot, r_n, lcap;
+    __host__ __device__ void put(int q, int r, int slot, int l, double2 v) const
+    {
+        const size_t M = (size_t)nslot * r_n * 2;
+        const size_t m = ((size_t)r * nslot + slot) * 2;
+        const int lp = lperm(l, lcap);
+        unsigned short tr[3], ti[3];
+        bf16_terms(v.x, ns, tr);
+        bf16_terms(v.y, ns, ti);
+        for (int s = 0; s < ns; s++) {
+            unsigned short *row = Eb + (((size_t)q * ns + s) * M + m) * lcap + lp;
+            row[0] = tr[s];
+            row[lcap] = ti[s];
+        }
+    }
+};
+
+// ---- operation families: F = false keeps the reference's rounding
+// (separate products and sums, Smith quotients); F = true is the fast FP64
+// family of the mixed forms (fused multiply-adds, reciprocal by v_rcp_f64
+// and two Newton steps, ~1 ulp, no lane divergence).
+__host__ __device__ inline double rcp_newton(double x)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    double r = __builtin_amdgcn_rcp(x);
+#else
+    double r = 1.0 / x;
+#endif
+    double e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    return fma(r, e, r);
+}
+template <bool F>
+__host__ __device__ inline double2 cmul_t(double2 a, double2 b)
+{
+    if (!F) return cmul(a, b);
+    return make_double2(fma(a.x, b.x, -(a.y * b.y)), fma(a.x, b.y, a.y * b.x));
+}
+// c + a b
+template <bool F>
+__host__ __device__ inline double2 cmadd_t(double2 c, double2 a, double2 b)
+{
+    if (!F) return cadd(c, cmul(a, b));
+    return make_double2(fma(a.x, b.x, fma(-a.y, b.y, c.x)), fma(a.x, b.y, fma(a.y, b.x, c.y)));
+}
+template <bool F>
+__host__ __device__ inline double2 crecip_t(double2 q)
+{
+    if (!F) return crecip(q);
+    const double r = rcp_newton(fma(q.x, q.x, q.y * q.y));
+    return make_double2(q.x * r, -q.y * r);
+}
+// a + s b with real s
+template <bool F>
+__host__ __device__ inline double2 caxpy_t(double2 a, double s, double2 b)
+{
+    if (!F) return caxpy(a, s, b);
+    return make_double2(fma(s, b.x, a.x), fma(s, b.y, a.y));
+}
+
 }  // namespace dsm

@@ -86,11 +86,19 @@ struct dsm_sgt {
     // BF16 forms: split operands, distance symmetry
     int ns = 0;                // bf16 terms per operand (0: not a BF16 form)
     int mirror = 0, nh = 0, nhp = 0;
-    DevBuf d_tb, d_eb, d_esb, d_stash;
+    DevBuf d_tb, d_eb, d_esb, d_stash, d_nlr, d_tiles;
+    std::vector<int> tiles_main, tiles_acc;  // host copies of the tile orders (launch sizes)
+    int tiles_main_n = 0, tiles_acc_n = 0;
     DevBuf d_band, d_om, d_slots, d_nl, d_slotout, d_scratch, d_xn, d_E, d_Es, d_amp, d_jobs;
     DevBuf d_cat_t, d_cat_r, d_cat_s;
     // P-SV only
-    DevBuf d_kind, d_fluid, d_irf, d_fluidK, d_noderow, d_nodec1, d_jsta0, d_l0work;
+    DevBuf d_kind, d_fluid, d_irf, d_fluidK, d_noderow, d_nodec1, d_jsta0, d_l0work, d_gexc, d_ends;
+    // depth cut of the batched forms (0: off): per-row radius / wave speed,
+    // per-slot angular frequency, first solved row per (slot, l block)
+    double dcut = 0.0;
+    DevBuf d_rrow, d_vrow, d_omg, d_kstart;
+    // fused FP64 family (mixed forms): compact band and its selector bits
+    DevBuf d_cband, d_csel;
     std::vector<SynthJob> jobs;
     int njobs_main = 0, njobs_syn = 0, njobs_acc = 0;  // launch order: main, accumulate, SYNN
     // timing
@@ -144,7 +152,8 @@ static void ev_collect(dsm_sgt_t *h)
 
 // Synthesis launch in the engine's precision; Eb is the bf16 split of
 // ga.E (BF16 forms).
-static void launch_synth(dsm_sgt_t *h, const SynthArgs &ga, const DevBuf &Eb)
+static void launch_synth(dsm_sgt_t *h, const SynthArgs &ga, const DevBuf &Eb, const int *nlr = nullptr,
+                         const int *tiles = nullptr, int ntiles = 0)
 {
     if (h->ns) {
         SynthB16Args b{};
@@ -159,11 +168,13 @@ static void launch_synth(dsm_sgt_t *h, const SynthArgs &ga, const DevBuf &Eb)
         b.nh = h->nh;
         b.nhp = h->nhp;
         b.mirror = h->mirror;
-        b.nl = ga.nl;
+        b.nlr = nlr ? nlr : ga.nl;  // [r][slot]; [slot] when r_eff = 1
         b.slot_out = ga.slot_out;
         b.jobs = ga.jobs;
         b.njobs = ga.njobs;
         b.stash = h->d_stash.as<double>();
+        b.tiles = tiles;
+        b.ntiles = ntiles;
         launch_synth_b16(b, h->ns, h->stream);
     } else if (h->precision == DSM_SGT_FP32) {
         launch_synth_f32(ga, h->d_tab32.as<float>(), h->stream);
@@ -221,6 +232,40 @@ static int make_mixed_tables(dsm_sgt_t *h, size_t nplanes)
     return DSM_SGT_OK;
 }
 
+// Tile order of the main and accumulate launches of the BF16 forms: tiles
+// sorted by their l extent (the longest series among the tile's rows),
+// longest first, so the long tiles start first and spread over all XCDs.
+static int b16_tile_order(dsm_sgt_t *h, const std::vector<int> &nlr, int S, int R)
+{
+    int nx, ny;
+    synth_b16_tiles(S, R, h->nhp, nx, ny);
+    const int M = S * R * 2;
+    std::vector<int> kt(ny, 0);
+    for (int ty = 0; ty < ny; ty++)
+        for (int pr = (ty * 128) >> 1; pr <= (std::min(M - 1, ty * 128 + 127) >> 1); pr++)
+            kt[ty] = std::max(kt[ty], nlr[pr]);
+    auto order = [&](int j0, int nj, std::vector<int> &out) {
+        std::vector<int> rows;
+        for (int j = 0; j < nj; j++)
+            for (int ty = 0; ty < ny; ty++) rows.push_back(j * ny + ty);
+        std::stable_sort(rows.begin(), rows.end(), [&](int a, int b) { return kt[a % ny] > kt[b % ny]; });
+        out.clear();
+        for (int jt : rows)
+            for (int tx = 0; tx < nx; tx++) out.push_back(jt * nx + tx);
+        (void)j0;
+    };
+    order(0, h->njobs_main, h->tiles_main);
+    order(0, h->njobs_acc, h->tiles_acc);
+    h->tiles_main_n = (int)h->tiles_main.size();
+    h->tiles_acc_n = (int)h->tiles_acc.size();
+    HIPCHK(h, h->d_tiles.ensure(sizeof(int) * (h->tiles_main.size() + h->tiles_acc.size())));
+    HIPCHK(h, hipMemcpyAsync(h->d_tiles.p, h->tiles_main.data(), sizeof(int) * h->tiles_main.size(),
+                             hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipMemcpyAsync(h->d_tiles.as<int>() + h->tiles_main.size(), h->tiles_acc.data(),
+                             sizeof(int) * h->tiles_acc.size(), hipMemcpyHostToDevice, h->stream));
+    return DSM_SGT_OK;
+}
+
 // Device buffers of the BF16 forms for nslot frequencies: split rows and
 // the stash of the largest launch.
 static int alloc_b16(dsm_sgt_t *h, int nslot, int nq, int nqs, int R)
@@ -229,6 +274,10 @@ static int alloc_b16(dsm_sgt_t *h, int nslot, int nq, int nqs, int R)
     const size_t M = (size_t)nslot * R * 2, Ms = (size_t)nslot * 2;
     HIPCHK(h, h->d_eb.ensure(sizeof(unsigned short) * nq * h->ns * M * h->lcap));
     HIPCHK(h, h->d_esb.ensure(sizeof(unsigned short) * nqs * h->ns * Ms * h->lcap));
+    // degrees above maxlmax are never written: zero once
+    HIPCHK(h, hipMemset(h->d_eb.p, 0, h->d_eb.n));
+    HIPCHK(h, hipMemset(h->d_esb.p, 0, h->d_esb.n));
+    HIPCHK(h, h->d_nlr.ensure(sizeof(int) * (size_t)nslot * R));
     size_t g = synth_b16_grid(nslot, R, h->nhp, std::max(h->njobs_main, h->njobs_acc));
     g = std::max(g, synth_b16_grid(nslot, 1, h->nhp, h->njobs_syn));
     HIPCHK(h, h->d_stash.ensure(sizeof(double) * g * 8 * 32 * 64));
@@ -561,6 +610,12 @@ static int sh_run_batch(dsm_sgt_t *h, const std::vector<int> &freqs)
     int rc = split_rows(h, h->d_E, S, 36, R, h->d_eb);
     if (!rc) rc = split_rows(h, h->d_Es, S, 5, 1, h->d_esb);
     if (rc) return rc;
+    if (h->ns) {
+        std::vector<int> nlr((size_t)R * S);
+        for (int r = 0; r < R; r++)
+            for (int k = 0; k < S; k++) nlr[(size_t)r * S + k] = nl[k];
+        HIPCHK(h, hipMemcpyAsync(h->d_nlr.p, nlr.data(), sizeof(int) * nlr.size(), hipMemcpyHostToDevice, h->stream));
+    }
     ev_end(h, tz);
 
     SynthArgs ga{};
@@ -578,7 +633,7 @@ static int sh_run_batch(dsm_sgt_t *h, const std::vector<int> &freqs)
     ga.njobs = h->njobs_main;
     TimedLaunch tg;
     ev_begin(h, tg, 2);
-    launch_synth(h, ga, h->d_eb);
+    launch_synth(h, ga, h->d_eb, h->d_nlr.as<int>());
     HIPCHK(h, hipGetLastError());
     ev_end(h, tg);
     // SYNN rows (surface node, no receiver dimension)
@@ -674,8 +729,32 @@ static int psv_create(dsm_sgt_t *h)
     HIPCHK(h, h->d_l0work.ensure(sizeof(double2) * 5 * (size_t)N0 * S));
     const int n = nn - s.row0 + 1, klo = g.rowlo - s.row0 + 1, fs = s.jtmp - s.row0 + 1;
     const int lch = std::min(h->lchunk, s.maxlmax + 1);
-    const size_t nthreads = (size_t)S * ((lch + 63) / 64 + 1) * 64;
+    const size_t nthreads = (size_t)S * (psv_solve_blocks(0, lch - 1) + 1) * 256;
     HIPCHK(h, h->d_scratch.ensure(sizeof(double2) * (size_t)(3 * (n - klo + 1) + 4 * (n - fs + 1)) * nthreads));
+    HIPCHK(h, h->d_gexc.ensure(sizeof(double2) * (size_t)S * 16 * LC));
+    HIPCHK(h, h->d_ends.ensure(sizeof(double2) * 3 * nthreads));
+    // batched forms: depth cut (DSM_SGT_DEPTH_CUT overrides; 0 disables)
+    h->dcut = (h->precision == DSM_SGT_FP64) ? 0.0 : 40.0;
+    if (const char *e = std::getenv("DSM_SGT_DEPTH_CUT")) h->dcut = std::max(0.0, std::atof(e));
+    {
+        std::vector<double> rr, vv;
+        psv_row_profile(s, rr, vv);
+        HIPCHK(h, h->d_rrow.ensure(sizeof(double) * nn));
+        HIPCHK(h, h->d_vrow.ensure(sizeof(double) * nn));
+        HIPCHK(h, hipMemcpy(h->d_rrow.p, rr.data(), sizeof(double) * nn, hipMemcpyHostToDevice));
+        HIPCHK(h, hipMemcpy(h->d_vrow.p, vv.data(), sizeof(double) * nn, hipMemcpyHostToDevice));
+        HIPCHK(h, h->d_omg.ensure(sizeof(double) * S));
+        if (h->precision != DSM_SGT_FP64) {
+            std::vector<int> sel(nn, 0);
+            for (int j = 0; j < nn; j++)
+                for (int ii = 0; ii < 4; ii++)
+                    if (s.term_kind[4 * (size_t)j + ii] == 2) sel[j] |= 1 << ii;
+            HIPCHK(h, h->d_csel.ensure(sizeof(int) * nn));
+            HIPCHK(h, hipMemcpy(h->d_csel.p, sel.data(), sizeof(int) * nn, hipMemcpyHostToDevice));
+            HIPCHK(h, h->d_cband.ensure(sizeof(double2) * 8 * (size_t)nn * S));
+        }
+        HIPCHK(h, h->d_kstart.ensure(sizeof(int) * (size_t)S * (psv_solve_blocks(0, lch - 1) + 1)));
+    }
     HIPCHK(h, h->d_xn.ensure(sizeof(double2) * (size_t)S * h->nneed * PSV_NSOL * LC));
     HIPCHK(h, h->d_E.ensure(sizeof(double2) * (size_t)S * PSV_NQ * R * LC));
     HIPCHK(h, h->d_Es.ensure(sizeof(double2) * (size_t)S * PSV_NQS * LC));
@@ -727,9 +806,11 @@ static int psv_run_batch(dsm_sgt_t *h, const std::vector<int> &freqs)
     std::vector<PsvSrc> src(S);
     std::vector<CutState> cut(S);
     std::vector<int> slotout(S);
+    std::vector<double> omg(S);
     for (int k = 0; k < S; k++) {
         PsvOmega o;
         psv_omega_setup(s, freqs[k], o);
+        omg[k] = o.omega;
         psv_pack_band(s, o, (double *)(band.data() + 16 * (size_t)nn * k));
         for (int q = 0; q < 160; q++) {
             src[k].anum[q] = make_double2(o.anum[q].real(), o.anum[q].imag());
@@ -745,6 +826,7 @@ static int psv_run_batch(dsm_sgt_t *h, const std::vector<int> &freqs)
     HIPCHK(h, hipMemcpyAsync(h->d_band.p, band.data(), sizeof(double2) * band.size(), hipMemcpyHostToDevice, h->stream));
     HIPCHK(h, hipMemcpyAsync(h->d_src.p, src.data(), sizeof(PsvSrc) * S, hipMemcpyHostToDevice, h->stream));
     HIPCHK(h, hipMemcpyAsync(h->d_fluidK.p, fk.data(), sizeof(double2) * fk.size(), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipMemcpyAsync(h->d_omg.p, omg.data(), sizeof(double) * S, hipMemcpyHostToDevice, h->stream));
     HIPCHK(h, hipMemcpyAsync(h->d_slotout.p, slotout.data(), sizeof(int) * S, hipMemcpyHostToDevice, h->stream));
     std::vector<int> active(S);
     for (int k = 0; k < S; k++) active[k] = k;
@@ -777,6 +859,10 @@ static int psv_run_batch(dsm_sgt_t *h, const std::vector<int> &freqs)
         za.E = h->d_E.as<double2>();
         za.Es = h->d_Es.as<double2>();
         za.lcap = LC;
+        if (h->ns) {
+            za.eb = B16Rows{h->d_eb.as<unsigned short>(), h->ns, S, R, LC};
+            za.esb = B16Rows{h->d_esb.as<unsigned short>(), h->ns, S, 1, LC};
+        }
         TimedLaunch t0;
         ev_begin(h, t0, 0);
         launch_psv_l0(za, h->stream);
@@ -785,6 +871,8 @@ static int psv_run_batch(dsm_sgt_t *h, const std::vector<int> &freqs)
     }
     const int n = nn - s.row0 + 1, klo = g.rowlo - s.row0 + 1, fs = s.jtmp - s.row0 + 1;
     std::vector<double2> ampuv((size_t)S * LC * 2);
+    std::vector<int> kst_host;  // depth cut per (slot, block of 256 l), single-launch case
+    int kst_nblk = 0;
     int l0 = 0;
     while (!active.empty()) {
         const int lend = std::min(l0 + h->lchunk - 1, s.maxlmax);
@@ -806,14 +894,45 @@ static int psv_run_batch(dsm_sgt_t *h, const std::vector<int> &freqs)
         sa.nneed = h->nneed;
         sa.need = h->d_need.as<int>();
         sa.scratch = h->d_scratch.as<double2>();
-        sa.sstride = (long)nblocks * 64;
+        sa.sstride = (long)active.size() * psv_solve_blocks(l0, lend) * 256;
         sa.xn = h->d_xn.as<double2>();
         sa.lcap = LC;
         sa.src = h->d_src.as<PsvSrc>();
         sa.sc = sc;
+        sa.gexc = h->d_gexc.as<double2>();
+        sa.ends = h->d_ends.as<double2>();
+        sa.kstart = (h->dcut > 0.0) ? h->d_kstart.as<int>() : nullptr;
+        sa.cband = h->d_cband.as<double2>();
+        sa.csel = h->d_csel.as<int>();
         TimedLaunch t0;
         ev_begin(h, t0, 0);
-        launch_psv_solve(sa, nblocks, h->stream);
+        if (h->precision != DSM_SGT_FP64 && l0 == 0)
+            launch_psv_compact(h->d_band.as<double2>(), h->d_kind.as<int>(), nn, h->d_slots.as<int>(),
+                               (int)active.size(), h->d_cband.as<double2>(), h->stream);
+        if (h->dcut > 0.0) {
+            PsvDepthArgs da{};
+            da.rrow = h->d_rrow.as<double>();
+            da.vrow = h->d_vrow.as<double>();
+            da.omega = h->d_omg.as<double>();
+            da.slot_list = h->d_slots.as<int>();
+            da.nslots = (int)active.size();
+            da.nblk = psv_solve_blocks(l0, lend);
+            da.l0 = l0;
+            da.row0 = s.row0;
+            da.jtmp = s.jtmp;
+            da.dcut = h->dcut;
+            da.dcap = 0.35;
+            da.kstart = h->d_kstart.as<int>();
+            launch_psv_depth(da, h->stream);
+            if (h->ns && l0 == 0 && lend == s.maxlmax) {
+                kst_nblk = da.nblk;
+                kst_host.resize((size_t)S * kst_nblk);
+                HIPCHK(h, hipMemcpyAsync(kst_host.data(), h->d_kstart.p, sizeof(int) * kst_host.size(),
+                                         hipMemcpyDeviceToHost, h->stream));
+            }
+        }
+        launch_psv_excite(sa, (int)active.size(), h->stream);
+        launch_psv_solve(sa, (int)active.size(), h->precision != DSM_SGT_FP64, h->stream);
         HIPCHK(h, hipGetLastError());
         PsvCoefArgs ca{};
         ca.xn = h->d_xn.as<double2>();
@@ -837,6 +956,10 @@ static int psv_run_batch(dsm_sgt_t *h, const std::vector<int> &freqs)
         ca.E = h->d_E.as<double2>();
         ca.Es = h->d_Es.as<double2>();
         ca.ampuv = h->d_amp.as<double2>();
+        if (h->ns) {
+            ca.eb = B16Rows{h->d_eb.as<unsigned short>(), h->ns, S, R, LC};
+            ca.esb = B16Rows{h->d_esb.as<unsigned short>(), h->ns, S, 1, LC};
+        }
         launch_psv_coef(ca, nblocks, h->stream);
         HIPCHK(h, hipGetLastError());
         ev_end(h, t0);
@@ -893,12 +1016,37 @@ static int psv_run_batch(dsm_sgt_t *h, const std::vector<int> &freqs)
     HIPCHK(h, hipMemcpyAsync(h->d_slots.p, all.data(), sizeof(int) * S, hipMemcpyHostToDevice, h->stream));
     TimedLaunch tz;
     ev_begin(h, tz, 3);
-    launch_zero_tail(h->d_E.as<double2>(), S, PSV_NQ * R, h->d_slots.as<int>(), h->d_nl.as<int>(), LC, h->stream);
-    launch_zero_tail(h->d_Es.as<double2>(), S, PSV_NQS, h->d_slots.as<int>(), h->d_nl.as<int>(), LC, h->stream);
+    int rc = DSM_SGT_OK;
+    if (h->ns) {
+        // bf16 rows written by the coefficient kernels: zero the cut-off
+        // tails; l extent per (receiver, slot): the l cut-off, and the
+        // blocks of l whose solved rows reach the receiver's stencil
+        const int lmin = *std::min_element(nl.begin(), nl.end());
+        launch_zero_tail_b16(h->d_eb.as<unsigned short>(), PSV_NQ, h->ns, S, R, LC, h->d_nl.as<int>(), lmin,
+                             s.maxlmax, h->stream);
+        launch_zero_tail_b16(h->d_esb.as<unsigned short>(), PSV_NQS, h->ns, S, 1, LC, h->d_nl.as<int>(), lmin,
+                             s.maxlmax, h->stream);
+        std::vector<int> nlr((size_t)R * S);
+        for (int r = 0; r < R; r++) {
+            const int rlow = s.ksta[r] - s.row0;  // local row of ksta-1
+            for (int k = 0; k < S; k++) {
+                int lr = nl[k];
+                if (!kst_host.empty()) {
+                    int nb = 0;
+                    while (nb < kst_nblk && kst_host[(size_t)k * kst_nblk + nb] <= rlow) nb++;
+                    lr = std::min(lr, 256 * nb);
+                }
+                nlr[(size_t)r * S + k] = lr;
+            }
+        }
+        HIPCHK(h, hipMemcpyAsync(h->d_nlr.p, nlr.data(), sizeof(int) * nlr.size(), hipMemcpyHostToDevice, h->stream));
+        rc = b16_tile_order(h, nlr, S, R);
+        if (rc) return rc;
+    } else {
+        launch_zero_tail(h->d_E.as<double2>(), S, PSV_NQ * R, h->d_slots.as<int>(), h->d_nl.as<int>(), LC, h->stream);
+        launch_zero_tail(h->d_Es.as<double2>(), S, PSV_NQS, h->d_slots.as<int>(), h->d_nl.as<int>(), LC, h->stream);
+    }
     HIPCHK(h, hipGetLastError());
-    int rc = split_rows(h, h->d_E, S, PSV_NQ, R, h->d_eb);
-    if (!rc) rc = split_rows(h, h->d_Es, S, PSV_NQS, 1, h->d_esb);
-    if (rc) return rc;
     ev_end(h, tz);
 
     SynthArgs ga{};
@@ -916,11 +1064,12 @@ static int psv_run_batch(dsm_sgt_t *h, const std::vector<int> &freqs)
     ga.njobs = h->njobs_main;
     TimedLaunch tg;
     ev_begin(h, tg, 2);
-    launch_synth(h, ga, h->d_eb);
+    launch_synth(h, ga, h->d_eb, h->d_nlr.as<int>(), h->ns ? h->d_tiles.as<int>() : nullptr, h->tiles_main_n);
     HIPCHK(h, hipGetLastError());
     ga.jobs = h->d_jobs.as<SynthJob>() + h->njobs_main;
     ga.njobs = h->njobs_acc;
-    launch_synth(h, ga, h->d_eb);
+    launch_synth(h, ga, h->d_eb, h->d_nlr.as<int>(), h->ns ? h->d_tiles.as<int>() + h->tiles_main_n : nullptr,
+                 h->tiles_acc_n);
     HIPCHK(h, hipGetLastError());
     ev_end(h, tg);
     ga.E = h->d_Es.as<double2>();

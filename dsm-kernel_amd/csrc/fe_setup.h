@@ -139,6 +139,11 @@ struct PsvGeom {
 };
 PsvGeom psv_geometry(const PsvSetup &s);
 
+// Per band row j (0-based): radius of the row's node and the slowest wave
+// speed there (S in solids, P in fluids), for the depth cut of the batched
+// forms (psv_depth_kernel).
+void psv_row_profile(const PsvSetup &s, std::vector<double> &r, std::vector<double> &v);
+
 // Interpolation weights of the reference's 3-node Taylor stencil
 // (others.f90 interpolate + glu): weights b with u = sum b_i g_i.
 void stencil_weights(double rsta, const double rr[3], int nderiv, double w[3]);

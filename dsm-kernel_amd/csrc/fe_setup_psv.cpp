@@ -967,6 +967,26 @@ void psv_pack_kind(const PsvSetup &s, int *kind)
     }
 }
 
+void psv_row_profile(const PsvSetup &s, std::vector<double> &r, std::vector<double> &v)
+{
+    r.assign(s.nn, 0.0);
+    v.assign(s.nn, 0.0);
+    const int nz = s.nzone;
+    for (int iz = 0; iz < nz; iz++) {
+        const int j0 = s.kkdr[iz], j1 = (iz + 1 < nz) ? s.kkdr[iz + 1] - 1 : s.nn;
+        const bool solid = s.iphase[iz] == 1;
+        for (int j = j0; j <= j1; j++) {
+            int i = solid ? (j - j0) / 2 : (j - j0);
+            i = std::min(i, s.nlayer[iz]);
+            const double rr = s.ra[s.isp[iz] - 1 + i];
+            const Moduli m = eval_zone(s, iz, rr, s.rmax);
+            const double vs = solid ? std::min(m.vsv, m.vsh) : std::min(m.vpv, m.vph);
+            r[j - 1] = rr;
+            v[j - 1] = vs > 0.0 ? vs : std::min(m.vpv, m.vph);
+        }
+    }
+}
+
 PsvGeom psv_geometry(const PsvSetup &s)
 {
     PsvGeom g;

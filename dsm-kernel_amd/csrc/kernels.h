@@ -13,7 +13,16 @@ void launch_sh_solve(const ShSolveArgs &a, int nblocks, hipStream_t s);
 void launch_sh_coef(const ShCoefArgs &a, int nblocks, hipStream_t s);
 
 // P-SV lane bodies and argument structs: psv_lane.h
-void launch_psv_solve(const PsvSolveArgs &a, int nblocks, hipStream_t s);
+// excitation g of every (frequency, l) into a.gexc, then the band solve;
+// grids (l blocks, nslots active frequencies)
+void launch_psv_excite(const PsvSolveArgs &a, int nslots, hipStream_t s);
+// fast: the fused FP64 operation family of the mixed forms (dev_common.h)
+void launch_psv_solve(const PsvSolveArgs &a, int nslots, bool fast, hipStream_t s);
+int psv_solve_blocks(int l0, int lend);  // l blocks per frequency of launch_psv_solve
+void launch_psv_depth(const PsvDepthArgs &a, hipStream_t s);
+// compact band of the fused family (PsvSolveArgs::cband)
+void launch_psv_compact(const double2 *band, const int *kind, int nn, const int *slot_list, int nslots,
+                        double2 *cband, hipStream_t s);
 void launch_psv_coef(const PsvCoefArgs &a, int nblocks, hipStream_t s);
 void launch_psv_l0(const PsvL0Args &a, hipStream_t s);
 
@@ -65,21 +74,28 @@ void launch_tab_to_f32(const double *src, float *dst, size_t n, hipStream_t s);
 // 3 bf16 terms, degrees stored even-l first; with `mirror` the columns
 // c < nh = (theta_n+1)/2 give distances c and theta_n-1-c.
 struct SynthB16Args {
-    const unsigned short *Eb;  // [q][ns][m][lcap], m = (slot r_eff + r) 2 + re|im
+    const unsigned short *Eb;  // [q][ns][m][lcap], m = (r nslot + slot) 2 + re|im
     const unsigned short *Tb;  // [ns][plane][nhp][lcap]
     int nq, r_eff, nslot, lcap, nplanes;
     int theta_n, nh, nhp, mirror;
-    const int *nl;
+    const int *nlr;            // [r][slot]: l extent of every (receiver, slot) row
     const int *slot_out;
     const SynthJob *jobs;
     int njobs;
     double *stash;             // [workgroup][8 waves][32][64 lanes]
+    const int *tiles;          // optional [grid]: tile (job ny + ty) nx + tx per workgroup, longest first
+    int ntiles;
 };
+// zero degrees l in [max(lmin, nl(slot)), lend] of every bf16 row
+void launch_zero_tail_b16(unsigned short *Eb, int nq, int ns, int nslot, int r_eff, int lcap, const int *nl, int lmin,
+                          int lend, hipStream_t s);
 void launch_split_rows(const double2 *E, int nslot, int nq, int r_eff, int lcap, int ns, unsigned short *Eb,
                        hipStream_t s);
 void launch_split_table(const double *tab, int nplanes, int lcap, int theta_pad, int nh, int nhp, int ns,
                         unsigned short *Tb, hipStream_t s);
 size_t synth_b16_grid(int nslot, int r_eff, int nhp, int njobs);
+// tile geometry of launch_synth_b16: nx distance tiles, ny row tiles
+void synth_b16_tiles(int nslot, int r_eff, int nhp, int &nx, int &ny);
 void launch_synth_b16(const SynthB16Args &a, int ns, hipStream_t s);
 
 }  // namespace dsm

@@ -1,16 +1,426 @@
-// psv_kernels.hip -- P-SV (spheroidal) per-(omega,l) kernels: band solve,
-// coefficient rows of the synthesis contraction and the l = 0 system.
-// Built without FMA contraction (see psv_lane.h).
+// psv_kernels.hip -- P-SV (spheroidal) per-(omega,l) kernels: source
+// excitation, band solve, coefficient rows of the synthesis contraction and
+// the l = 0 system.  Built without FMA contraction (see psv_lane.h).
 #include "dev_common.h"
 #include "kernels.h"
 
 namespace dsm {
 
-__global__ __launch_bounds__(64) void psv_solve_kernel(PsvSolveArgs a) { psv_solve_lane(a, blockIdx.x, threadIdx.x); }
-
-void launch_psv_solve(const PsvSolveArgs &a, int nblocks, hipStream_t s)
+// ---------------------------------------------------------------------------
+// Source excitation g[s][i] (calya + calg, psv_excitation) of every
+// (frequency, l): grid (l blocks of 64, active frequencies).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void psv_excite_kernel(PsvSolveArgs a)
 {
-    hipLaunchKernelGGL(psv_solve_kernel, dim3(nblocks), dim3(64), 0, s, a);
+    const int sl = a.slot_list[blockIdx.y];
+    const int l = a.l0 + blockIdx.x * 64 + threadIdx.x;
+    if (l > a.lend || l == 0) return;
+    double2 g[4][4];
+    psv_excitation(l, a.src[sl], a.sc, g);
+    double2 *o = a.gexc + (size_t)sl * 16 * a.lcap + l;
+    for (int s = 0; s < 4; s++)
+        for (int i = 0; i < 4; i++) o[(size_t)(4 * s + i) * a.lcap] = g[s][i];
+}
+
+__global__ __launch_bounds__(64) void psv_depth_kernel(PsvDepthArgs a)
+{
+    const int t = blockIdx.x * 64 + threadIdx.x;
+    if (t >= a.nslots * a.nblk) return;
+    const int k = t / a.nblk, b = t - k * a.nblk;
+    const int sl = a.slot_list[k];
+    const double om = a.omega[sl];
+    const double al = (double)max(1, a.l0 + 256 * b) + 0.5;
+    // walk down from the source row; keep at least three rows below it
+    int kst = a.row0;
+    double D = 0.0;
+    for (int j = a.jtmp - 1; j >= a.row0; j--) {  // rows j (1-based), j+1
+        const double rm = 0.5 * (a.rrow[j - 1] + a.rrow[j]);
+        const double vm = 0.5 * (a.vrow[j - 1] + a.vrow[j]);
+        const double dr = a.rrow[j] - a.rrow[j - 1];
+        const double q = al / rm, p = om / vm;
+        const double k2 = q * q - p * p;
+        if (k2 > 0.0 && dr > 0.0) D += fmin(sqrt(k2) * dr, a.dcap);
+        if (D > a.dcut) {
+            kst = j;
+            break;
+        }
+    }
+    kst = min(kst, a.jtmp - 3);
+    a.kstart[t] = max(1, kst - a.row0 + 1);
+}
+
+void launch_psv_depth(const PsvDepthArgs &a, hipStream_t s)
+{
+    const int n = a.nslots * a.nblk;
+    hipLaunchKernelGGL(psv_depth_kernel, dim3((n + 63) / 64), dim3(64), 0, s, a);
+}
+
+// ---------------------------------------------------------------------------
+// Band solve for l >= 1 (the formulation of psv_solve_lane, same
+// operations in the same order), in two kernels: the upward sweep
+// (factorization fused with the forward substitution of the four source
+// solutions) and the backward substitution of the six solutions.  One lane
+// per (frequency, l); lane gid keeps its factors in scratch column gid
+// between the two.  F selects the operation family (dev_common.h): false,
+// the reference's rounding (the FP64 parity form); true, fused FP64
+// arithmetic for the mixed forms.
+//
+// Upward sweep: a workgroup holds 256 consecutive l of one frequency, so
+// every lane walks the same band columns at the same time: the columns are
+// staged through LDS in chunks of SV_CH (two buffers, the next chunk's
+// global loads in flight during the current one) and read as broadcasts.
+// Lanes past the l range shadow a valid l and store nothing, so every lane
+// reaches every barrier.
+// ---------------------------------------------------------------------------
+constexpr int SV_NT = 256, SV_CH = 32;
+
+// Compact band of the fused family: entry ii of column j is A + s B with
+// A = e0 (+ e2), B = e1 (+ e3) folded from the full column (two zones'
+// terms summed before the scale); one block per (slot, 64 columns).
+__global__ __launch_bounds__(256) void psv_compact_kernel(const double2 *band, const int *kind, int nn,
+                                                          const int *slot_list, double2 *cband)
+{
+    const int sl = slot_list[blockIdx.y];
+    const int t = blockIdx.x * 256 + threadIdx.x;  // (column, ii)
+    if (t >= 4 * nn) return;
+    const int j = t >> 2, ii = t & 3;
+    const int kb = kind[j];
+    const int kk = (kb >> (4 * ii)) & 3, two = (kb >> (4 * ii + 2)) & 1;
+    const double2 *e = band + ((size_t)sl * nn + j) * 16 + 4 * ii;
+    double2 A = cz(), B = cz();
+    if (kk == 3) {
+        A = e[0];
+    } else if (kk != 0) {
+        A = two ? cadd(e[0], e[2]) : e[0];
+        B = two ? cadd(e[1], e[3]) : e[1];
+    }
+    double2 *o = cband + (((size_t)sl * nn + j) * 4 + ii) * 2;
+    o[0] = A;
+    o[1] = B;
+}
+
+void launch_psv_compact(const double2 *band, const int *kind, int nn, const int *slot_list, int nslots,
+                        double2 *cband, hipStream_t s)
+{
+    hipLaunchKernelGGL(psv_compact_kernel, dim3((4 * nn + 255) / 256, nslots), dim3(256), 0, s, band, kind, nn,
+                       slot_list, cband);
+}
+
+template <bool F>
+__global__ __launch_bounds__(SV_NT) void psv_factor_kernel(PsvSolveArgs a)
+{
+    constexpr int CW = F ? 8 : 16;  // double2 per staged column
+    __shared__ double2 sband[2][SV_CH * CW];
+    __shared__ int skind[2][SV_CH];
+    const int tid = threadIdx.x;
+    const int sl = a.slot_list[blockIdx.y];
+    const int lraw = a.l0 + blockIdx.x * SV_NT + tid;
+    const bool act = lraw <= a.lend && lraw != 0;
+    const int l = act ? lraw : max(1, a.lend);
+    const long gid = ((long)blockIdx.y * gridDim.x + blockIdx.x) * SV_NT + tid;
+    const long ss = a.sstride;
+    const double l2 = (double)l * (double)(l + 1);
+    const double lsq = sqrt(l2);
+    const int n = a.nn - a.row0 + 1;
+    const int fs = a.jtmp - a.row0 + 1;    // local row of the first source row (>= 3)
+    const int klo = a.rowlo - a.row0 + 1;  // lowest local row needed
+    const int nsrc = (l >= 2) ? 4 : 3;
+    // local column j (1..n) is band row row0+j-1
+    const double2 *band0 = (F ? a.cband : a.band) + ((size_t)sl * a.nn + (a.row0 - 1)) * CW;
+    const int *kind0 = (F ? a.csel : a.kind) + (a.row0 - 1);
+    double2 *Ls = a.scratch;                                   // [k - klo][3]
+    double2 *Zs = a.scratch + (size_t)3 * (n - klo + 1) * ss;  // [k - fs][4]
+    const double2 *gx = a.gexc + (size_t)sl * 16 * a.lcap + l;
+    // first solved local row of this block (depth cut of the batched forms)
+    const int kst = a.kstart ? a.kstart[blockIdx.y * gridDim.x + blockIdx.x] : 1;
+
+    const int nch = (n + SV_CH - 1) / SV_CH;
+    constexpr int PER = SV_CH * CW / SV_NT;  // staged values per thread
+    double2 pf[PER];
+    int pk = 0;
+    auto fetch = [&](int c) {
+        const int base = c * SV_CH;
+#pragma unroll
+        for (int u = 0; u < PER; u++) {
+            const int e = tid + u * SV_NT;
+            pf[u] = (base + e / CW < n) ? band0[(size_t)base * CW + e] : cz();
+        }
+        if (tid < SV_CH) pk = (base + tid < n) ? kind0[base + tid] : 0;
+    };
+    auto put = [&](int buf) {
+#pragma unroll
+        for (int u = 0; u < PER; u++) sband[buf][tid + u * SV_NT] = pf[u];
+        if (tid < SV_CH) skind[buf][tid] = pk;
+    };
+    // entries of column j (1..n, visited in increasing order): advances the
+    // staging by one chunk on the first column of the next chunk
+    int cur = (kst - 1) / SV_CH;
+    auto column = [&](int j, double2 e[4]) {
+        const int c = (j - 1) / SV_CH;
+        if (c != cur) {
+            if (cur + 1 < nch) {
+                put((cur + 1) & 1);
+                if (cur + 2 < nch) fetch(cur + 2);
+            }
+            __syncthreads();
+            cur = c;
+        }
+        const int o = (j - 1) - c * SV_CH;
+        if (F) psv_col_fast(&sband[c & 1][o * CW], skind[c & 1][o], l2, lsq, e);
+        else psv_col_exact(&sband[c & 1][o * CW], skind[c & 1][o], l2, lsq, e);
+    };
+
+    fetch(cur);
+    put(cur & 1);
+    if (cur + 1 < nch) fetch(cur + 1);
+    __syncthreads();
+
+    PsvWin W;
+    {
+        double2 e[4];
+        column(kst, e);
+        W.d0 = e[3];
+        column(kst + 1, e);
+        W.e01 = e[2];
+        W.d1 = e[3];
+        column(kst + 2, e);
+        W.e02 = e[1];
+        W.e12 = e[2];
+        W.d2 = e[3];
+    }
+    W.w4m3 = W.w4m2 = W.w4m1 = W.w3m2 = W.w3m1 = W.w2m1 = cz();
+    auto storeL = [&](int k, double2 w2, double2 w3, double2 w4) {
+        if (act && k >= klo) {
+            Ls[(size_t)(3 * (k - klo) + 0) * ss + gid] = w2;
+            Ls[(size_t)(3 * (k - klo) + 1) * ss + gid] = w3;
+            Ls[(size_t)(3 * (k - klo) + 2) * ss + gid] = w4;
+        }
+    };
+    // phase 1: rows below the source, factorization only (fs <= n - 3);
+    // factors stored from klo up
+    for (int k = kst; k < min(fs, klo); k++) {
+        double2 e[4], w2, w3, w4;
+        column(k + 3, e);
+        psv_factor_step<F>(W, k, n, e, crecip_t<F>(W.d0), w2, w3, w4);
+    }
+    for (int k = max(kst, klo); k < fs; k++) {
+        double2 e[4], w2, w3, w4;
+        column(k + 3, e);
+        psv_factor_step<F>(W, k, n, e, crecip_t<F>(W.d0), w2, w3, w4);
+        storeL(k, w2, w3, w4);
+    }
+    // phase 2: forward values of the source solutions at row k (their
+    // right-hand sides sit at rows fs..fs+3), then the factorization step
+    double2 z1[4], z2[4], z3[4];
+    for (int s = 0; s < 4; s++) z1[s] = z2[s] = z3[s] = cz();
+    auto forward = [&](int k, double2 piv) {
+        const double2 a1 = W.w4m3, a2 = W.w3m2, a3 = W.w2m1;
+        const bool src_row = (k - fs) < 4;
+#pragma unroll
+        for (int s = 0; s < 4; s++) {
+            if (s >= nsrc) continue;
+            const double2 bk = src_row ? gx[(size_t)(4 * s + (k - fs)) * a.lcap] : cz();
+            double2 sum;
+            if (k == 3) sum = cmadd_t<F>(cmul_t<F>(a2, z2[s]), a3, z3[s]);
+            else sum = cmadd_t<F>(cmadd_t<F>(cmul_t<F>(a1, z1[s]), a2, z2[s]), a3, z3[s]);
+            const double2 zk = csub(bk, sum);
+            z1[s] = z2[s];
+            z2[s] = z3[s];
+            z3[s] = zk;
+            if (act) Zs[(size_t)(4 * (k - fs) + s) * ss + gid] = F ? cmul_t<F>(zk, piv) : cdiv(zk, W.d0);
+        }
+    };
+    for (int k = fs; k <= n - 3; k++) {
+        double2 e[4], w2, w3, w4;
+        column(k + 3, e);
+        const double2 piv = crecip_t<F>(W.d0);
+        forward(k, piv);
+        psv_factor_step<F>(W, k, n, e, piv, w2, w3, w4);
+        storeL(k, w2, w3, w4);
+    }
+    // rows n-2, n-1 (end-of-band updates) and n (forward value only)
+    double2 Dn1, Dn, w2n1;
+    {
+        double2 w2, w3, w4;
+        double2 piv = crecip_t<F>(W.d0);
+        forward(n - 2, piv);
+        psv_factor_step<F>(W, n - 2, n, nullptr, piv, w2, w3, w4);
+        storeL(n - 2, w2, w3, w4);
+        piv = crecip_t<F>(W.d0);
+        forward(n - 1, piv);
+        Dn1 = W.d0;
+        psv_factor_step<F>(W, n - 1, n, nullptr, piv, w2, w3, w4);
+        storeL(n - 1, w2, w3, w4);
+        w2n1 = w2;
+        piv = crecip_t<F>(W.d0);
+        forward(n, piv);
+        Dn = W.d0;
+    }
+    if (act) {
+        a.ends[gid] = Dn1;
+        a.ends[ss + gid] = Dn;
+        a.ends[2 * ss + gid] = w2n1;
+    }
+}
+
+// Backward substitution from row n down to klo for the six solutions,
+// recording them at the rows in need[]: the four source solutions from
+// their forward values, the back-propagated forces YB (row n-1, value
+// -float(c0)) and S (row n) from their forward values at rows n-1, n
+// only.  One lane per (frequency, l), 64-lane workgroups.  The rows from
+// the source up (forward values present) come first; below the source the
+// factors of four rows ahead are in flight in a ring of four registers
+// sets that is never copied (a copy would wait for its load).
+template <bool F>
+__global__ __launch_bounds__(64) void psv_back_kernel(PsvSolveArgs a)
+{
+    const int sub = blockIdx.x & 3, xb = blockIdx.x >> 2;
+    const int sl = a.slot_list[blockIdx.y];
+    const int lraw = a.l0 + xb * SV_NT + sub * 64 + threadIdx.x;
+    if (lraw > a.lend || lraw == 0) return;
+    const int l = lraw;
+    const long gid = ((long)blockIdx.y * (gridDim.x >> 2) + xb) * SV_NT + sub * 64 + threadIdx.x;
+    const long ss = a.sstride;
+    const double l2 = (double)l * (double)(l + 1);
+    const double lsq = sqrt(l2);
+    const int n = a.nn - a.row0 + 1;
+    const int fs = a.jtmp - a.row0 + 1;
+    const int klo = a.rowlo - a.row0 + 1;
+    const int nsrc = (l >= 2) ? 4 : 3;
+    const double2 *Ls = a.scratch;
+    const double2 *Zs = a.scratch + (size_t)3 * (n - klo + 1) * ss;
+    const double2 Dn1 = a.ends[gid], Dn = a.ends[ss + gid], w2n1 = a.ends[2 * ss + gid];
+    const int kst = a.kstart ? a.kstart[blockIdx.y * (gridDim.x >> 2) + xb] : 1;
+    const int kbot = max(klo, kst);  // rows below kst are outside the solved system
+
+    // back-propagated forces: forward values at rows n-1, n only
+    const double c0f = (double)(float)sqrt((double)(2 * l + 1) / (4.0 * DSM_PI) / 1.0);
+    const double cm1 = sqrt((double)(2 * l + 1) / (4.0 * DSM_PI) / (1.0 * (double)l * (double)(l + 1)));
+    const double hs = l2 * cm1 / 2.0;
+    const double2 yb_rhs = make_double2(-c0f, 0.0);
+    const double2 s_rhs = cdiv(make_double2(hs, 0.0), make_double2(lsq, 0.0));
+    const double2 zb_n1 = cdiv(yb_rhs, Dn1);
+    // z(n) = 0 - (a(1,n) 0 + a(2,n) 0 + a(3,n) z(n-1)); the zero products are exact
+    const double2 zb_n = cdiv(csub(cz(), cadd(cadd(cz(), cz()), cmul(w2n1, yb_rhs))), Dn);
+    const double2 zs_n = cdiv(s_rhs, Dn);
+
+    double2 x1[PSV_NSOL], x2[PSV_NSOL], x3[PSV_NSOL];  // b(k+1), b(k+2), b(k+3)
+    for (int s = 0; s < PSV_NSOL; s++) x1[s] = x2[s] = x3[s] = cz();
+    int p = 0;
+    double2 *xo = a.xn + (size_t)sl * a.nneed * PSV_NSOL * a.lcap + l;
+    struct Row {
+        double2 L2, L3, L4;
+    };
+    auto ld = [&](int k) {
+        Row r{cz(), cz(), cz()};
+        if (k >= kbot) {
+            r.L2 = (k < n) ? Ls[(size_t)(3 * (k - klo) + 0) * ss + gid] : cz();
+            r.L3 = (k < n - 1) ? Ls[(size_t)(3 * (k - klo) + 1) * ss + gid] : cz();
+            r.L4 = (k < n - 2) ? Ls[(size_t)(3 * (k - klo) + 2) * ss + gid] : cz();
+        }
+        return r;
+    };
+    auto record = [&](int k, const double2 *bk) {
+        const int row = a.row0 + k - 1;
+        if (p < a.nneed && a.need[p] == row) {
+            for (int s = 0; s < PSV_NSOL; s++) xo[(size_t)(PSV_NSOL * p + s) * a.lcap] = bk[s];
+            p++;
+        }
+    };
+    auto shift = [&](const double2 *bk) {
+#pragma unroll
+        for (int s = 0; s < PSV_NSOL; s++) {
+            x3[s] = x2[s];
+            x2[s] = x1[s];
+            x1[s] = bk[s];
+        }
+    };
+    // rows n .. fs: forward values present
+    for (int k = n; k >= max(fs, kbot); k--) {
+        const Row q = ld(k);
+        double2 zd[PSV_NSOL];
+#pragma unroll
+        for (int s = 0; s < 4; s++) zd[s] = (s < nsrc) ? Zs[(size_t)(4 * (k - fs) + s) * ss + gid] : cz();
+        zd[4] = (k == n) ? zb_n : (k == n - 1) ? zb_n1 : cz();
+        zd[5] = (k == n) ? zs_n : cz();
+        double2 bk[PSV_NSOL];
+        if (k == n) {
+#pragma unroll
+            for (int s = 0; s < PSV_NSOL; s++) bk[s] = zd[s];
+        } else if (F) {
+            const double2 m2 = cneg(q.L2), m3 = cneg(q.L3), m4 = cneg(q.L4);
+#pragma unroll
+            for (int s = 0; s < PSV_NSOL; s++)
+                bk[s] = cmadd_t<F>(cmadd_t<F>(cmadd_t<F>(zd[s], m4, x3[s]), m3, x2[s]), m2, x1[s]);
+        } else if (k == n - 1) {
+#pragma unroll
+            for (int s = 0; s < PSV_NSOL; s++) bk[s] = csub(zd[s], cmul(q.L2, x1[s]));
+        } else if (k == n - 2) {
+#pragma unroll
+            for (int s = 0; s < PSV_NSOL; s++) bk[s] = csub(zd[s], cadd(cmul(q.L3, x2[s]), cmul(q.L2, x1[s])));
+        } else {
+#pragma unroll
+            for (int s = 0; s < PSV_NSOL; s++)
+                bk[s] = csub(zd[s], cadd(cadd(cmul(q.L4, x3[s]), cmul(q.L3, x2[s])), cmul(q.L2, x1[s])));
+        }
+        shift(bk);
+        record(k, bk);
+    }
+    // rows fs-1 .. kbot: no right-hand side (fs - 1 <= n - 4)
+    auto step = [&](int k, const Row &q) {
+        double2 bk[PSV_NSOL];
+        if (F) {
+            const double2 m2 = cneg(q.L2), m3 = cneg(q.L3), m4 = cneg(q.L4);
+#pragma unroll
+            for (int s = 0; s < PSV_NSOL; s++)
+                bk[s] = cmadd_t<F>(cmadd_t<F>(cmul_t<F>(m4, x3[s]), m3, x2[s]), m2, x1[s]);
+        } else {
+#pragma unroll
+            for (int s = 0; s < PSV_NSOL; s++)
+                bk[s] = csub(cz(), cadd(cadd(cmul(q.L4, x3[s]), cmul(q.L3, x2[s])), cmul(q.L2, x1[s])));
+        }
+        shift(bk);
+        record(k, bk);
+    };
+    int k = fs - 1;
+    Row r0 = ld(k), r1 = ld(k - 1), r2 = ld(k - 2), r3 = ld(k - 3);
+    for (; k - 3 >= kbot; k -= 4) {
+        step(k, r0);
+        r0 = ld(k - 4);
+        step(k - 1, r1);
+        r1 = ld(k - 5);
+        step(k - 2, r2);
+        r2 = ld(k - 6);
+        step(k - 3, r3);
+        r3 = ld(k - 7);
+    }
+    if (k >= kbot) step(k, r0);
+    if (k - 1 >= kbot) step(k - 1, r1);
+    if (k - 2 >= kbot) step(k - 2, r2);
+    // recorded rows below the cut: zero
+    for (; p < a.nneed; p++)
+        for (int s = 0; s < PSV_NSOL; s++) xo[(size_t)(PSV_NSOL * p + s) * a.lcap] = cz();
+}
+
+void launch_psv_excite(const PsvSolveArgs &a, int nslots, hipStream_t s)
+{
+    const int nb = (a.lend - a.l0 + 64) / 64;
+    hipLaunchKernelGGL(psv_excite_kernel, dim3(nb, nslots), dim3(64), 0, s, a);
+}
+
+int psv_solve_blocks(int l0, int lend) { return (lend - l0 + SV_NT) / SV_NT; }
+
+void launch_psv_solve(const PsvSolveArgs &a, int nslots, bool fast, hipStream_t s)
+{
+    const int nb = psv_solve_blocks(a.l0, a.lend);
+    const dim3 g1(nb, nslots), g2(4 * nb, nslots);
+    if (fast) {
+        hipLaunchKernelGGL(psv_factor_kernel<true>, g1, dim3(SV_NT), 0, s, a);
+        hipLaunchKernelGGL(psv_back_kernel<true>, g2, dim3(64), 0, s, a);
+    } else {
+        hipLaunchKernelGGL(psv_factor_kernel<false>, g1, dim3(SV_NT), 0, s, a);
+        hipLaunchKernelGGL(psv_back_kernel<false>, g2, dim3(64), 0, s, a);
+    }
 }
 
 __global__ __launch_bounds__(64) void psv_coef_kernel(PsvCoefArgs a) { psv_coef_lane(a, blockIdx.x, threadIdx.x); }

@@ -52,6 +52,30 @@ struct PsvSolveArgs {
     int lcap;
     const PsvSrc *src;     // [slot]
     PsvSrcConst sc;
+    double2 *gexc;         // [slot][4 sources][4 rows][lcap]: excitation (psv_excite_kernel)
+    double2 *ends;         // [3][sstride]: d(n-1), d(n), w2(n-1) of the factorization
+    const int *kstart;     // [active slot][l block of 256]: first solved local row (null: 1)
+    const double2 *cband;  // fused family: [slot][nn][4][A, B] (psv_compact_kernel)
+    const int *csel;       // [nn]: bit ii set where entry ii scales with sqrt(l(l+1))
+};
+
+// Depth cut of the batched forms: for the smallest l of every block of 256
+// degrees, the deepest band row whose evanescent decay below the source,
+//   D = sum over rows of min(dcap, sqrt(max(0, ((l+1/2)/r)^2 - (omega/v)^2)) dr),
+// stays within dcut (calmdr's local quantity, others.f90:904-927, summed
+// row by row; v the slowest wave speed of the row).  The cap per row
+// follows the discrete system rather than the continuum: where the grid
+// does not resolve the decay length (kappa h >> 1) a 3-point element
+// operator decays by at most ~e^-1.3 per node whatever kappa is.  Rows
+// below are dropped from the system: their share of the solution is below
+// e^-dcut.
+struct PsvDepthArgs {
+    const double *rrow, *vrow;  // [nn] (psv_row_profile)
+    const double *omega;        // [slot] real angular frequency
+    const int *slot_list;
+    int nslots, nblk, l0, row0, jtmp;
+    double dcut, dcap;
+    int *kstart;                // [active slot][nblk] local rows
 };
 
 struct PsvCoefArgs {
@@ -68,6 +92,7 @@ struct PsvCoefArgs {
     double2 *E;            // [slot][PSV_NQ][r_n][lcap]
     double2 *Es;           // [slot][PSV_NQS][lcap]
     double2 *ampuv;        // [slot][lcap][2]
+    B16Rows eb, esb;       // mixed forms (eb.Eb non-null): bf16 rows instead of E, Es
 };
 
 struct PsvL0Args {
@@ -89,6 +114,7 @@ struct PsvL0Args {
     PsvSrcConst sc;
     double2 *E, *Es;
     int lcap;
+    B16Rows eb, esb;       // mixed forms: bf16 rows (see PsvCoefArgs)
 };
 
 // ---------------------------------------------------------------------------
@@ -96,6 +122,7 @@ struct PsvL0Args {
 // calmat.f90:747-809): contributions added in zone order, each formed as
 // the reference forms it.
 // ---------------------------------------------------------------------------
+template <bool F = false>
 __host__ __device__ inline double2 psv_entry(const double2 *col, int kbits, int ii, double l2, double lsq)
 {
     const int k = (kbits >> (4 * ii)) & 3;
@@ -104,8 +131,8 @@ __host__ __device__ inline double2 psv_entry(const double2 *col, int kbits, int 
     if (k == 0) return cz();
     if (k == 3) return e[0];
     const double s = (k == 1) ? l2 : lsq;
-    double2 v = caxpy(e[0], s, e[1]);
-    if (two) v = cadd(v, caxpy(e[2], s, e[3]));
+    double2 v = caxpy_t<F>(e[0], s, e[1]);
+    if (two) v = cadd(v, caxpy_t<F>(e[2], s, e[3]));
     return v;
 }
 
@@ -371,30 +398,46 @@ struct PsvWin {
     double2 w4m3, w4m2, w4m1, w3m2, w3m1, w2m1;
 };
 
-// One step k < n of dcsymbdl0 (m = 3): the main update for k <= n-3, the
-// end-of-band updates of dcsymbdl.f90:62-84 for k = n-2, n-1 (whose
-// product order differs).  Returns w2, w3, w4 of the step.
-__host__ __device__ inline void psv_factor_step(PsvWin &W, int k, int n, const double2 *c3col, int kb3, double l2,
-                                                double lsq, double2 &w2, double2 &w3, double2 &w4)
+// The four band entries a(1..4, j) of column j for this lane's l: from the
+// full column (16 values, entry ii = e0 + s e1 (+ e2 + s e3), psv_entry;
+// all values loaded first so the reads issue together), or, for the fused
+// family, from the compact column of psv_compact_kernel (entry ii =
+// A + s B with s = lsq where bit ii of sel is set, else l(l+1)).
+__host__ __device__ inline void psv_col_exact(const double2 *col, int kbits, double l2, double lsq, double2 a[4])
 {
-    const double2 piv = crecip(W.d0);
+    double2 e[16];
+    for (int t = 0; t < 16; t++) e[t] = col[t];
+    for (int ii = 0; ii < 4; ii++) a[ii] = psv_entry<false>(e, kbits, ii, l2, lsq);
+}
+__host__ __device__ inline void psv_col_fast(const double2 *col, int sel, double l2, double lsq, double2 a[4])
+{
+    double2 e[8];
+    for (int t = 0; t < 8; t++) e[t] = col[t];
+    for (int ii = 0; ii < 4; ii++) a[ii] = caxpy_t<true>(e[2 * ii], ((sel >> ii) & 1) ? lsq : l2, e[2 * ii + 1]);
+}
+
+// One step k < n of dcsymbdl0 (m = 3): the main update for k <= n-3 with
+// the entries a3[ii] = a(ii+1, k+3), the end-of-band updates of
+// dcsymbdl.f90:62-84 for k = n-2, n-1 (whose product order differs).
+// piv = 1 / W.d0 (crecip_t<F>).  Returns w2, w3, w4 of the step.
+template <bool F = false>
+__host__ __device__ inline void psv_factor_step(PsvWin &W, int k, int n, const double2 *a3, double2 piv, double2 &w2,
+                                                double2 &w3, double2 &w4)
+{
     w3 = cz();
     w4 = cz();
     if (k <= n - 3) {
-        const double2 a13 = psv_entry(c3col, kb3, 0, l2, lsq);
-        const double2 a23 = psv_entry(c3col, kb3, 1, l2, lsq);
-        const double2 a33 = psv_entry(c3col, kb3, 2, l2, lsq);
-        const double2 a43 = psv_entry(c3col, kb3, 3, l2, lsq);
+        const double2 a13 = a3[0], a23 = a3[1], a33 = a3[2], a43 = a3[3];
         const double2 zz2 = cneg(W.e01), zz3 = cneg(W.e02), zz4 = cneg(a13);
-        w2 = cmul(W.e01, piv);
-        w3 = cmul(W.e02, piv);
-        w4 = cmul(a13, piv);
-        const double2 nd1 = cadd(W.d1, cmul(w2, zz2));
-        const double2 nd2 = cadd(W.d2, cmul(w3, zz3));
-        const double2 nd3 = cadd(a43, cmul(w4, zz4));
-        const double2 ne12 = cadd(W.e12, cmul(w2, zz3));
-        const double2 ne23 = cadd(a33, cmul(w3, zz4));
-        const double2 ne13 = cadd(a23, cmul(w2, zz4));
+        w2 = cmul_t<F>(W.e01, piv);
+        w3 = cmul_t<F>(W.e02, piv);
+        w4 = cmul_t<F>(a13, piv);
+        const double2 nd1 = cmadd_t<F>(W.d1, w2, zz2);
+        const double2 nd2 = cmadd_t<F>(W.d2, w3, zz3);
+        const double2 nd3 = cmadd_t<F>(a43, w4, zz4);
+        const double2 ne12 = cmadd_t<F>(W.e12, w2, zz3);
+        const double2 ne23 = cmadd_t<F>(a33, w3, zz4);
+        const double2 ne13 = cmadd_t<F>(a23, w2, zz4);
         W.d0 = nd1;
         W.e01 = ne12;
         W.e02 = ne13;
@@ -403,18 +446,18 @@ __host__ __device__ inline void psv_factor_step(PsvWin &W, int k, int n, const d
         W.d2 = nd3;
     } else if (k == n - 2) {
         const double2 zz2 = cneg(W.e01), zz3 = cneg(W.e02);
-        w2 = cmul(W.e01, piv);
-        const double2 nd1 = cadd(W.d1, cmul(w2, zz2));
-        w3 = cmul(W.e02, piv);
-        const double2 ne12 = cadd(W.e12, cmul(w3, zz2));
-        const double2 nd2 = cadd(W.d2, cmul(w3, zz3));
+        w2 = cmul_t<F>(W.e01, piv);
+        const double2 nd1 = cmadd_t<F>(W.d1, w2, zz2);
+        w3 = cmul_t<F>(W.e02, piv);
+        const double2 ne12 = cmadd_t<F>(W.e12, w3, zz2);
+        const double2 nd2 = cmadd_t<F>(W.d2, w3, zz3);
         W.d0 = nd1;
         W.e01 = ne12;
         W.d1 = nd2;
     } else {  // k == n - 1
         const double2 zz2 = cneg(W.e01);
-        w2 = cmul(W.e01, piv);
-        W.d0 = cadd(W.d1, cmul(w2, zz2));
+        w2 = cmul_t<F>(W.e01, piv);
+        W.d0 = cmadd_t<F>(W.d1, w2, zz2);
     }
     W.w4m3 = W.w4m2;
     W.w4m2 = W.w4m1;
@@ -469,7 +512,9 @@ __host__ __device__ inline void psv_solve_lane(const PsvSolveArgs &a, int b, int
     // phase 1: rows below the source, factorization only
     for (int k = 1; k < fs; k++) {
         double2 w2, w3, w4;
-        psv_factor_step(W, k, n, col(k + 3), kb(k + 3), l2, lsq, w2, w3, w4);
+        double2 a3[4];
+        psv_col_exact(col(k + 3), kb(k + 3), l2, lsq, a3);
+        psv_factor_step(W, k, n, a3, crecip(W.d0), w2, w3, w4);
         storeL(k, w2, w3, w4);
     }
     // phase 2: from the source rows up, factorization + forward values of
@@ -501,7 +546,9 @@ __host__ __device__ inline void psv_solve_lane(const PsvSolveArgs &a, int b, int
                 break;
             }
             double2 w2, w3, w4;
-            psv_factor_step(W, k, n, col(k + 3 <= n ? k + 3 : n), kb(k + 3 <= n ? k + 3 : n), l2, lsq, w2, w3, w4);
+            double2 c3v[4];
+            psv_col_exact(col(k + 3 <= n ? k + 3 : n), kb(k + 3 <= n ? k + 3 : n), l2, lsq, c3v);
+            psv_factor_step(W, k, n, c3v, crecip(W.d0), w2, w3, w4);
             storeL(k, w2, w3, w4);
             if (k == n - 1) w2n1 = w2;
         }
@@ -754,8 +801,12 @@ __host__ __device__ inline void psv_coef_lane(const PsvCoefArgs &a, int b, int t
             vs[s] = xv(a.syn_i1, s);
         }
         psv_synn_rows(us, vs, il, mu1, mu2, es);
-        double2 *Es = a.Es + (size_t)sl * PSV_NQS * LC + l;
-        for (int q = 0; q < PSV_NQS; q++) Es[(size_t)q * LC] = es[q];
+        if (a.esb.Eb) {
+            for (int q = 0; q < PSV_NQS; q++) a.esb.put(q, 0, sl, l, es[q]);
+        } else {
+            double2 *Es = a.Es + (size_t)sl * PSV_NQS * LC + l;
+            for (int q = 0; q < PSV_NQS; q++) Es[(size_t)q * LC] = es[q];
+        }
     }
     double2 *E = a.E + (size_t)sl * PSV_NQ * R * LC + l;
     for (int r = 0; r < R; r++) {
@@ -770,8 +821,12 @@ __host__ __device__ inline void psv_coef_lane(const PsvCoefArgs &a, int b, int t
             z[s].Ud = cadd(cadd(cscale(g0, w1[0]), cscale(g2, w1[1])), cscale(g4, w1[2]));
             z[s].Vd = cadd(cadd(cscale(g1, w1[0]), cscale(g3, w1[1])), cscale(g5, w1[2]));
         }
-        psv_rows(z, a.fluid[r] != 0, a.fluidK[(size_t)sl * R + r], il, a.irf[r], mu1, mu2,
-                 [&](int q, double2 v) { E[(size_t)(q * R + r) * LC] = v; });
+        if (a.eb.Eb)
+            psv_rows(z, a.fluid[r] != 0, a.fluidK[(size_t)sl * R + r], il, a.irf[r], mu1, mu2,
+                     [&](int q, double2 v) { a.eb.put(q, r, sl, l, v); });
+        else
+            psv_rows(z, a.fluid[r] != 0, a.fluidK[(size_t)sl * R + r], il, a.irf[r], mu1, mu2,
+                     [&](int q, double2 v) { E[(size_t)(q * R + r) * LC] = v; });
     }
 }
 
@@ -917,16 +972,24 @@ __host__ __device__ inline void psv_l0_rows(const PsvL0Args &a, int t, int r)
             v[map[s]].U = cadd(cadd(cscale(zs[0], w0[0]), cscale(zs[1], w0[1])), cscale(zs[2], w0[2]));
             v[map[s]].Ud = cadd(cadd(cscale(zs[0], w1[0]), cscale(zs[1], w1[1])), cscale(zs[2], w1[2]));
         }
-        psv_rows(v, a.fluid[r] != 0, a.fluidK[(size_t)sl * R + r], 0.0, a.irf[r], false, false,
-                 [&](int q, double2 val) { E[(size_t)(q * R + r) * LC] = val; });
+        if (a.eb.Eb)
+            psv_rows(v, a.fluid[r] != 0, a.fluidK[(size_t)sl * R + r], 0.0, a.irf[r], false, false,
+                     [&](int q, double2 val) { a.eb.put(q, r, sl, 0, val); });
+        else
+            psv_rows(v, a.fluid[r] != 0, a.fluidK[(size_t)sl * R + r], 0.0, a.irf[r], false, false,
+                     [&](int q, double2 val) { E[(size_t)(q * R + r) * LC] = val; });
     } else if (r == R) {
         double2 us[PSV_NSOL], vs[PSV_NSOL], es[PSV_NQS];
         for (int s = 0; s < PSV_NSOL; s++) us[s] = vs[s] = cz();
         us[0] = z[N0 - 1];
         us[1] = z[(size_t)N0 + N0 - 1];
         psv_synn_rows(us, vs, 0.0, false, false, es);
-        double2 *Es = a.Es + (size_t)sl * PSV_NQS * LC;
-        for (int q = 0; q < PSV_NQS; q++) Es[(size_t)q * LC] = es[q];
+        if (a.esb.Eb) {
+            for (int q = 0; q < PSV_NQS; q++) a.esb.put(q, 0, sl, 0, es[q]);
+        } else {
+            double2 *Es = a.Es + (size_t)sl * PSV_NQS * LC;
+            for (int q = 0; q < PSV_NQS; q++) Es[(size_t)q * LC] = es[q];
+        }
     }
 }
 

@@ -25,6 +25,12 @@
 // FP64 per lane and register) while the odd half accumulates in the same
 // registers.
 //
+// Rows are receiver-major, m = (r nslot + slot) 2 + re|im, and every
+// (receiver, slot) row carries its own l extent nlr (the l cut-off, and in
+// the batched forms the depth cut: a receiver below the solved rows of a
+// block of l has no coefficients there), so a tile's K extent is the
+// longest series among its rows.
+//
 // Tile: 128 x 128 per 512-thread workgroup, 8 waves in 2 x 4, each wave
 // 4 x 2 blocks of 16 x 16; operands staged through LDS with a register-
 // staged double buffer, rows padded to 80 bytes (conflict-free 16-byte
@@ -41,27 +47,6 @@ typedef unsigned short u16;
 
 constexpr int BM = 128, BN = 128, BK = 32, NT = 512;
 constexpr int LDSW = 40;  // u16 per LDS row: 32 degrees + 8 pad
-
-__host__ __device__ inline int lperm(int l, int lcap) { return (l & 1) ? (lcap >> 1) + (l >> 1) : (l >> 1); }
-
-__device__ __forceinline__ u16 bf16_rn(float f)
-{
-    unsigned u = __float_as_uint(f);
-    u += 0x7fffu + ((u >> 16) & 1u);
-    return (u16)(u >> 16);
-}
-__device__ __forceinline__ double bf16_value(u16 h) { return (double)__uint_as_float((unsigned)h << 16); }
-
-// x0, x1, ...: each the bf16 rounding of the remainder (the remainder is
-// exact in FP64)
-__device__ __forceinline__ void split_terms(double x, int ns, u16 *t)
-{
-    double r = x;
-    for (int s = 0; s < ns; s++) {
-        t[s] = bf16_rn((float)r);
-        r -= bf16_value(t[s]);
-    }
-}
 
 // XCD-sliced tile order, as in synth.hip
 __device__ __forceinline__ bool xcd_tile_b16(int nx, int ny, int &tx, int &ty, int &tz)
@@ -85,11 +70,11 @@ struct StageB16 {
 }  // namespace
 
 // E [slot][q][r][lcap] (FP64 complex) -> Eb [q][s][m][lcap] (bf16 terms,
-// degrees permuted), m = (slot r_eff + r) 2 + re|im
+// degrees permuted), receiver-major rows m = (r nslot + slot) 2 + re|im
 __global__ void split_rows_kernel(const double2 *E, int nslot, int nq, int R, int lcap, int ns, u16 *Eb)
 {
     const size_t total = (size_t)nslot * nq * R * lcap;
-    const size_t M = (size_t)nslot * R * 2;
+    const B16Rows dst{Eb, ns, nslot, R, lcap};
     for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
          idx += (size_t)gridDim.x * blockDim.x) {
         const int l = (int)(idx % lcap);
@@ -98,17 +83,23 @@ __global__ void split_rows_kernel(const double2 *E, int nslot, int nq, int R, in
         rest /= R;
         const int q = (int)(rest % nq);
         const int sl = (int)(rest / nq);
-        const double2 v = E[idx];
-        const size_t m = ((size_t)sl * R + r) * 2;
-        const int lp = lperm(l, lcap);
-        u16 tr[3], ti[3];
-        split_terms(v.x, ns, tr);
-        split_terms(v.y, ns, ti);
-        for (int s = 0; s < ns; s++) {
-            u16 *row = Eb + (((size_t)q * ns + s) * M + m) * lcap + lp;
-            row[0] = tr[s];
-            row[lcap] = ti[s];
-        }
+        dst.put(q, r, sl, l, E[idx]);
+    }
+}
+
+// zero the degrees l in [nl(slot), lend] of every row (the l cut-off)
+__global__ void zero_tail_b16_kernel(u16 *Eb, int nq, int ns, int nslot, int R, int lcap, const int *nl, int lmin,
+                                     int lend)
+{
+    const size_t rows = (size_t)nq * ns * nslot * R * 2;
+    const int span = lend + 1 - lmin;
+    for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < rows * span;
+         idx += (size_t)gridDim.x * blockDim.x) {
+        const int l = lmin + (int)(idx % span);
+        const size_t row = idx / span;
+        const size_t m = row % ((size_t)nslot * R * 2);
+        const int sl = (int)((m >> 1) % nslot);
+        if (l >= nl[sl]) Eb[row * lcap + lperm(l, lcap)] = 0;
     }
 }
 
@@ -126,7 +117,7 @@ __global__ void split_table_kernel(const double *tab, int nplanes, int lcap, int
         const int p = (int)(rest / nhp);
         const double v = (c < nh) ? tab[((size_t)p * lcap + l) * theta_pad + c] : 0.0;
         u16 t[3];
-        split_terms(v, ns, t);
+        bf16_terms(v, ns, t);
         for (int s = 0; s < ns; s++) Tb[(((size_t)s * nplanes + p) * nhp + c) * lcap + lperm(l, lcap)] = t[s];
     }
 }
@@ -166,18 +157,24 @@ __global__ __launch_bounds__(512) void synth_b16_kernel(SynthB16Args a, int nx, 
     __shared__ __attribute__((aligned(16))) u16 As[2][NS * BM * LDSW];
     __shared__ __attribute__((aligned(16))) u16 Bs[2][NS * BN * LDSW];
     int tx, ty, tz;
-    if (!xcd_tile_b16(nx, ny, tx, ty, tz)) return;
+    if (a.tiles) {
+        // host order: longest l series first, dealt round-robin over the XCDs
+        if ((int)blockIdx.x >= a.ntiles) return;
+        const int t = a.tiles[blockIdx.x];
+        tx = t % nx;
+        ty = (t / nx) % ny;
+        tz = t / (nx * ny);
+    } else if (!xcd_tile_b16(nx, ny, tx, ty, tz)) {
+        return;
+    }
     const SynthJob *job = a.jobs + tz;
     const int nb = job->nb;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int R = a.r_eff;
     const int M = a.nslot * R * 2;
     const int m0 = ty * BM, n0 = tx * BN;
-    int kmax = 0;
-    {
-        const int s0 = m0 / (2 * R), s1 = min(M - 1, m0 + BM - 1) / (2 * R);
-        for (int sl = s0; sl <= s1; sl++) kmax = max(kmax, a.nl[sl]);
-    }
+    int kmax = 0;  // longest l series among the tile's (receiver, slot) rows
+    for (int pr = m0 >> 1; pr <= (min(M - 1, m0 + BM - 1) >> 1); pr++) kmax = max(kmax, a.nlr[pr]);
     const int ce = ((kmax + 1) / 2 + BK - 1) / BK;  // chunks of even degrees l < kmax
     const int co = (kmax / 2 + BK - 1) / BK;        // odd
     const int LH = a.lcap >> 1;
@@ -279,7 +276,7 @@ __global__ __launch_bounds__(512) void synth_b16_kernel(SynthB16Args a, int nx, 
             const int row = m0 + wr * 64 + i * 16 + 4 * (lane >> 4) + q;
             if (row >= M) continue;
             const int pr = row >> 1, reim = row & 1;
-            const int sl = pr / R, r = pr - sl * R;
+            const int r = pr / a.nslot, sl = pr - r * a.nslot;
             const size_t base = ((size_t)a.slot_out[sl] * R + r) * a.theta_n;
 #pragma unroll
             for (int j = 0; j < 2; j++) {
@@ -315,6 +312,14 @@ void launch_split_rows(const double2 *E, int nslot, int nq, int r_eff, int lcap,
     hipLaunchKernelGGL(split_rows_kernel, dim3(8192), dim3(256), 0, s, E, nslot, nq, r_eff, lcap, ns, Eb);
 }
 
+void launch_zero_tail_b16(unsigned short *Eb, int nq, int ns, int nslot, int r_eff, int lcap, const int *nl, int lmin,
+                          int lend, hipStream_t s)
+{
+    if (lmin > lend) return;
+    hipLaunchKernelGGL(zero_tail_b16_kernel, dim3(8192), dim3(256), 0, s, Eb, nq, ns, nslot, r_eff, lcap, nl, lmin,
+                       lend);
+}
+
 void launch_split_table(const double *tab, int nplanes, int lcap, int theta_pad, int nh, int nhp, int ns,
                         unsigned short *Tb, hipStream_t s)
 {
@@ -329,11 +334,17 @@ size_t synth_b16_grid(int nslot, int r_eff, int nhp, int njobs)
     return (size_t)8 * ((nx * ny + 7) / 8) * njobs;
 }
 
+void synth_b16_tiles(int nslot, int r_eff, int nhp, int &nx, int &ny)
+{
+    nx = nhp / BN;
+    ny = (nslot * r_eff * 2 + BM - 1) / BM;
+}
+
 void launch_synth_b16(const SynthB16Args &a, int ns, hipStream_t s)
 {
     const int M = a.nslot * a.r_eff * 2;
     const int nx = a.nhp / BN, ny = (M + BM - 1) / BM;
-    const dim3 grid((unsigned)synth_b16_grid(a.nslot, a.r_eff, a.nhp, a.njobs));
+    const dim3 grid(a.tiles ? (unsigned)a.ntiles : (unsigned)synth_b16_grid(a.nslot, a.r_eff, a.nhp, a.njobs));
     if (ns == 3) {
         if (a.r_eff > 1) hipLaunchKernelGGL((synth_b16_kernel<3, 0>), grid, dim3(NT), 0, s, a, nx, ny);
         else hipLaunchKernelGGL((synth_b16_kernel<3, 1>), grid, dim3(NT), 0, s, a, nx, ny);

@@ -35,18 +35,24 @@ FIVE_S = dict(tlen=3276.8, imax=656, rmin_=3716., rmax_=6371., rdelta_=45., thet
               thetamax=180., thetadelta=0.1, maxlmax=8000)
 
 
-def _engine(inp, mode, prec, mirror=True):
+def _engine(inp, mode, prec, mirror=True, depth_cut=None):
+    """depth_cut: None keeps the form's default (on for the mixed forms),
+    else the decay exponent (0 disables)."""
     import dsm_sgt
-    old = os.environ.get("DSM_SGT_MIRROR")
-    if not mirror:
-        os.environ["DSM_SGT_MIRROR"] = "0"
+    env = {"DSM_SGT_MIRROR": None if mirror else "0",
+           "DSM_SGT_DEPTH_CUT": None if depth_cut is None else str(depth_cut)}
+    old = {k: os.environ.get(k) for k in env}
+    for k, v in env.items():
+        if v is not None:
+            os.environ[k] = v
     try:
         return dsm_sgt.Engine(dsm_sgt.DsmSgtInput.from_buffer_copy(inp), mode=mode, precision=prec)
     finally:
-        if old is None:
-            os.environ.pop("DSM_SGT_MIRROR", None)
-        else:
-            os.environ["DSM_SGT_MIRROR"] = old
+        for k, v in old.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v
 
 
 def _vs_oracle(eng, inp, freqs, fn, tol):
@@ -97,10 +103,10 @@ def test_sh_mixed_small_vs_oracle(prec_name):
     print(f"SH {prec_name}: max rel L2 {w:.2e}")
 
 
-def _five_s(prec, freqs):
+def _five_s(prec, freqs, depth_cut=None):
     import dsm_sgt
     inp = di.synthetic_input(**FIVE_S)
-    eng = _engine(inp, dsm_sgt.PSV, prec)
+    eng = _engine(inp, dsm_sgt.PSV, prec, depth_cut=depth_cut)
     out = {}
     for a, b in freqs:
         eng.run(a, b)
@@ -126,3 +132,32 @@ def test_psv_mixed_5s_catalog_grid():
                 worst = max(worst, e.max())
                 assert e.max() <= tol, (name, i, kind, e)
         print(f"5 s grid {name}: max rel L2 per component {worst:.2e}", flush=True)
+
+
+def test_psv_depth_cut_5s_grid():
+    """The depth cut of the batched forms alone, in FP64 arithmetic: rows
+    whose evanescent decay below the source exceeds e^-40 dropped, against
+    the full system (the FP64 parity form)."""
+    import dsm_sgt
+    freqs = ((1, 2), (300, 300), (655, 656))
+    ref = _five_s(dsm_sgt.FP64, freqs)
+    got = _five_s(dsm_sgt.FP64, freqs, depth_cut=40)
+    worst = 0.0
+    for i in ref:
+        for k, kind in enumerate(("tsgt", "rsgt", "synn")):
+            e = component_errors(got[i][k], ref[i][k])
+            worst = max(worst, e.max())
+            assert e.max() <= 1e-10, (i, kind, e)
+    print(f"5 s grid depth cut (FP64): max rel L2 per component {worst:.2e}", flush=True)
+
+
+def test_psv_depth_cut_small_vs_oracle():
+    """The depth cut on a coarse grid (node spacing ~46 km in the mantle,
+    where the discrete decay is far slower than the continuum's): FP64
+    with the cut against the oracle on every frequency."""
+    import dsm_sgt
+    inp = di.synthetic_input(**SMALL)
+    eng = _engine(inp, dsm_sgt.PSV, dsm_sgt.FP64, depth_cut=40)
+    eng.run(1, SMALL["imax"])
+    w = _vs_oracle(eng, inp, range(1, SMALL["imax"] + 1), lambda x, i: ol.psv_omega(x, i), 1e-10)
+    print(f"small grid depth cut (FP64): max rel L2 {w:.2e}")
