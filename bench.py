@@ -12,10 +12,16 @@ host memory and ends with the catalog resident in HBM; writing the files is
 not timed.  `--mode sh` runs the 10 s SH catalog instead (i = 1..328,
 531 rows, lmax 4000, 40 radii 4421..6371 km).
 
+Times the batched mixed form by default (--form bf16x6: split-bf16
+synthesis on MFMA, fused FP64 band solve with the depth cut); the plain
+FP64 form (the parity form) is timed once beside it on the same input and
+the mixed catalog is compared with it, per component, at 17 frequencies
+spread over the catalog (bar: relative L2 <= 1e-6).
+
 Prints one JSON line with the metric of the reference's headline benchmark,
-the roofline of the dominant kernel (the FP64-MFMA synthesis contraction)
-and a CPU baseline: the oracle (C restatement of the reference) timed on a
-bounded sample of the same workload on this host.
+the roofline of the dominant kernel of the timed form and a CPU baseline:
+the oracle (C restatement of the reference) timed on a bounded sample of
+the same workload on this host.
 """
 import argparse
 import json
@@ -83,12 +89,33 @@ def cpu_baseline(mode, p, r_n, lsample=1000):
                        f"x {p['imax']} frequencies")
 
 
+FORMS = {"bf16x6": ("BF16X6", "bf16x6+f64"), "bf16x3": ("BF16X3", "bf16x3+f64"), "fp32": ("FP32", "f32+f64"),
+         "fp64": ("FP64", "f64")}
+BF16_PEAK = 2.5e15
+FP32_MFMA_PEAK = 157.3e12
+
+
+def catalog_errors(eng, ref, freqs):
+    """Worst relative L2 per catalog component (parity.py) of `eng` against
+    `ref` over the frequencies `freqs`, per kind."""
+    from parity import component_errors
+    worst = {"tsgt": 0.0, "rsgt": 0.0, "synn": 0.0}
+    for i in freqs:
+        got, want = eng.fetch_all_fp64(i), ref.fetch_all_fp64(i)
+        for k, kind in enumerate(("tsgt", "rsgt", "synn")):
+            worst[kind] = max(worst[kind], float(component_errors(got[k], want[k]).max()))
+    return worst
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--mode", choices=("psv", "sh"), default="psv")
+    ap.add_argument("--form", choices=tuple(FORMS), default="bf16x6",
+                    help="arithmetic of the timed catalog (the plain FP64 form is timed once beside it)")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the plain FP64 reference run")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline sample")
     ap.add_argument("--imax", type=int, default=None, help="(testing) fewer frequencies")
     args = ap.parse_args()
@@ -103,36 +130,66 @@ def main():
         p["imax"] = args.imax
     inp = make_input(p)
     mode = dsm_sgt.PSV if args.mode == "psv" else dsm_sgt.SH
+    prec = getattr(dsm_sgt, FORMS[args.form][0])
 
-    def step():
-        eng = dsm_sgt.Engine(inp, mode=mode, precision=dsm_sgt.FP64)
+    def step(pr):
+        eng = dsm_sgt.Engine(inp, mode=mode, precision=pr)
         eng.run(1, p["imax"])  # returns with the device synchronised
         return eng
 
     for _ in range(args.warmup):
-        step().close()
-    times, stats = [], []
+        step(prec).close()
+    times, eng = [], None
     for _ in range(args.steps):
+        if eng is not None:
+            eng.close()
         t0 = time.perf_counter()
-        eng = step()
+        eng = step(prec)
         times.append(time.perf_counter() - t0)
-        stats.append(eng.stats())
-        cuts = [eng.lcut(i) for i in (1, p["imax"])]
-        info = eng.info
-        eng.close()
+    st = eng.stats()
+    cuts = [eng.lcut(i) for i in (1, p["imax"])]
+    info = eng.info
     ms = 1e3 * sum(times) / len(times)
-    st = stats[-1]
-    synth_ms_per_launch = st.synth_ms / max(st.synth_launches, 1)
-    flops_per_launch = st.synth_flops / max(st.synth_launches, 1)
-    achieved = flops_per_launch / (synth_ms_per_launch * 1e-3) if synth_ms_per_launch > 0 else 0.0
+
+    # the plain FP64 form once, and the form's error against it
+    fp64 = None
+    if args.form != "fp64" and not args.no_fp64:
+        t0 = time.perf_counter()
+        ref = step(dsm_sgt.FP64)
+        fp64_s = time.perf_counter() - t0
+        fst = ref.stats()
+        freqs = sorted(set(int(round(x)) for x in [1 + k * (p["imax"] - 1) / 16 for k in range(17)]))
+        errs = catalog_errors(eng, ref, freqs)
+        fp64 = {"wall_s": fp64_s, "speedup": fp64_s / (ms / 1e3),
+                "stages_ms": {"solve+coef": fst.solve_ms, "synth": fst.synth_ms, "other": fst.other_ms},
+                "max_rel_l2_vs_fp64": errs, "error_frequencies": freqs, "bar": 1e-6}
+        ref.close()
+    eng.close()
+
     r_n, th_n, nn = info.r_n, info.theta_n, info.nn
-    if args.mode == "psv":
-        workload = (f"SGTpsv 5 s catalog (P-SV plain FP64): 4-zone sphere with fluid core, tlen 3276.8 s, "
-                    f"i=1..{p['imax']}, re 0.1 ({nn} rows), lmax {p['maxlmax']}, {r_n} radii "
-                    f"(3716..6371 km) x {th_n} distances, surface source")
+    launches = max(st.synth_launches, 1)
+    launch_ms = st.synth_ms / launches
+    if args.form in ("bf16x6", "bf16x3"):
+        kernel = "synth_b16_kernel<%d, 0>" % (3 if args.form == "bf16x6" else 2)
+        flops = st.synth_mfma_flops / launches
+        peak, bound_note = BF16_PEAK, "MFMA flops issued (all bf16 cross products), bf16 dense peak"
+    elif args.form == "fp32":
+        kernel, flops, peak = "synth_f32_kernel<0>", st.synth_flops / launches, FP32_MFMA_PEAK
+        bound_note = "algorithmic flops, FP32 MFMA peak"
     else:
-        workload = (f"SGTsh 10 s catalog (SH plain FP64): 4-zone sphere, tlen 3276.8 s, i=1..{p['imax']}, "
-                    f"re 0.1 ({nn} rows), lmax {p['maxlmax']}, {r_n} radii x {th_n} distances, surface source")
+        kernel, flops, peak = "synth_f64_kernel<0>", st.synth_flops / launches, FP64_MFMA_PEAK
+        bound_note = "algorithmic flops, FP64 MFMA peak"
+    achieved = flops / (launch_ms * 1e-3) if launch_ms > 0 else 0.0
+    form_txt = {"bf16x6": "batched mixed: bf16x6 synthesis, FP64 solve with the depth cut",
+                "bf16x3": "batched mixed: bf16x3 synthesis, FP64 solve with the depth cut",
+                "fp32": "mixed: FP32 synthesis, FP64 solve with the depth cut", "fp64": "plain FP64"}[args.form]
+    if args.mode == "psv":
+        workload = (f"SGTpsv 5 s catalog: 4-zone sphere with fluid core, tlen 3276.8 s, i=1..{p['imax']}, "
+                    f"re 0.1 ({nn} rows), lmax {p['maxlmax']}, {r_n} radii (3716..6371 km) x {th_n} distances, "
+                    f"surface source")
+    else:
+        workload = (f"SGTsh 10 s catalog: 4-zone sphere, tlen 3276.8 s, i=1..{p['imax']}, re 0.1 ({nn} rows), "
+                    f"lmax {p['maxlmax']}, {r_n} radii x {th_n} distances, surface source")
     out = {
         "metric": METRIC,
         "value": ms / 1e3,
@@ -144,21 +201,22 @@ def main():
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": FORMS[args.form][1],
         "data": "synthetic",
-        "config": {"workload": workload, "form": "plain FP64", "frequencies": p["imax"],
+        "config": {"workload": workload, "form": form_txt, "frequencies": p["imax"],
                    "lmax": p["maxlmax"], "band_rows": nn, "radii": r_n, "distances": th_n, "l_cut": cuts},
-        "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": FP64_MFMA_PEAK / 1e12,
-                     "unit": "TFLOP/s", "frac": achieved / FP64_MFMA_PEAK, "traffic": None,
-                     "kernel": "synth_f64_kernel<0>",
-                     "avg_launch_ms": synth_ms_per_launch, "flops_per_launch": flops_per_launch},
+        "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
+                     "frac": achieved / peak, "traffic": None, "kernel": kernel, "avg_launch_ms": launch_ms,
+                     "flops_per_launch": flops, "note": bound_note},
         "stages_ms": {"solve+coef": st.solve_ms, "legendre": st.legendre_ms, "synth": st.synth_ms,
                       "other": st.other_ms},
-        "solve_hbm": {"achieved_GBps": st.solve_bytes / (st.solve_ms * 1e-3) / 1e9 if st.solve_ms else None,
-                      "peak_GBps": HBM_PEAK / 1e9, "systems": st.systems},
+        "solve": {"systems": st.systems, "bytes_design": st.solve_bytes},
     }
+    if fp64 is not None:
+        out["plain_fp64"] = fp64
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.mode, p, r_n)
+        out["speedup_vs_cpu"] = out["cpu_baseline"]["value"] / out["value"]
     print(json.dumps(out), flush=True)
     return 0
 

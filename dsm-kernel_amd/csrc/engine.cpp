@@ -256,6 +256,16 @@ static int b16_tile_order(dsm_sgt_t *h, const std::vector<int> &nlr, int S, int 
     };
     order(0, h->njobs_main, h->tiles_main);
     order(0, h->njobs_acc, h->tiles_acc);
+    // MFMA flops issued: per tile and K step of 32 degrees, 2*128*128*32
+    // per cross product
+    const double per = 2.0 * 128 * 128 * 32 * (h->ns == 3 ? 6 : 3);
+    double fl = 0.0;
+    for (int j = 0; j < h->njobs_main + h->njobs_acc; j++)
+        for (int ty = 0; ty < ny; ty++) {
+            const int ce = ((kt[ty] + 1) / 2 + 31) / 32, co = (kt[ty] / 2 + 31) / 32;
+            fl += per * nx * h->jobs[j].nb * (ce + co);
+        }
+    h->stats.synth_mfma_flops += fl;
     h->tiles_main_n = (int)h->tiles_main.size();
     h->tiles_acc_n = (int)h->tiles_acc.size();
     HIPCHK(h, h->d_tiles.ensure(sizeof(int) * (h->tiles_main.size() + h->tiles_acc.size())));
@@ -436,9 +446,6 @@ static int sh_create(dsm_sgt_t *h)
     HIPCHK(h, h->d_cat_t.ensure(sizeof(double2) * np * R * s.theta_n * 10));
     HIPCHK(h, h->d_cat_r.ensure(sizeof(double2) * np * R * s.theta_n * 5));
     HIPCHK(h, h->d_cat_s.ensure(sizeof(double2) * np * s.theta_n * 5));
-    HIPCHK(h, hipMemsetAsync(h->d_cat_t.p, 0, h->d_cat_t.n, h->stream));
-    HIPCHK(h, hipMemsetAsync(h->d_cat_r.p, 0, h->d_cat_r.n, h->stream));
-    HIPCHK(h, hipMemsetAsync(h->d_cat_s.p, 0, h->d_cat_s.n, h->stream));
     sh_synth_jobs(h);
     {
         // parity of Y0..Y5 (legendre_lane.h) times (-1)^mu
@@ -763,9 +770,6 @@ static int psv_create(dsm_sgt_t *h)
     HIPCHK(h, h->d_cat_t.ensure(sizeof(double2) * np * R * s.theta_n * 20));
     HIPCHK(h, h->d_cat_r.ensure(sizeof(double2) * np * R * s.theta_n * 10));
     HIPCHK(h, h->d_cat_s.ensure(sizeof(double2) * np * s.theta_n * 10));
-    HIPCHK(h, hipMemsetAsync(h->d_cat_t.p, 0, h->d_cat_t.n, h->stream));
-    HIPCHK(h, hipMemsetAsync(h->d_cat_r.p, 0, h->d_cat_r.n, h->stream));
-    HIPCHK(h, hipMemsetAsync(h->d_cat_s.p, 0, h->d_cat_s.n, h->stream));
     psv_synth_jobs(h);
     {
         // parity of F0..F9 (legendre_lane.h) times (-1)^mu

@@ -33,7 +33,7 @@ class Stats(C.Structure):
     _fields_ = [("solve_ms", C.c_double), ("legendre_ms", C.c_double), ("synth_ms", C.c_double),
                 ("other_ms", C.c_double), ("solve_launches", C.c_int64), ("synth_launches", C.c_int64),
                 ("systems", C.c_int64), ("solve_bytes", C.c_double), ("synth_flops", C.c_double),
-                ("synth_bytes", C.c_double)]
+                ("synth_bytes", C.c_double), ("synth_mfma_flops", C.c_double)]
 
 
 class DsmSgtError(RuntimeError):

@@ -95,6 +95,7 @@ typedef struct dsm_sgt_stats {
     double solve_bytes;       /* algorithmic bytes of the band solves */
     double synth_flops;       /* algorithmic flops of the synthesis contraction */
     double synth_bytes;       /* operand bytes streamed by the synthesis */
+    double synth_mfma_flops;  /* MFMA flops the synthesis kernels issued (bf16 forms: all cross products) */
 } dsm_sgt_stats_t;
 
 /* Create an engine on HIP device `device` for `mode` and `precision`.
