@@ -17,8 +17,10 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <future>
 #include <string>
 #include <sys/stat.h>
+#include <thread>
 #include <vector>
 
 #include "dsm_sgt.h"
@@ -61,6 +63,19 @@ struct TimedLaunch {
 
 }  // namespace
 
+// Host part of one P-SV batch (per-frequency set-up of SGTpsv-MPI.f90:
+// 566-625 for every slot), prepared on host threads while the previous
+// batch runs on the device.
+struct PsvPrep {
+    std::vector<int> freqs;
+    double2 *band = nullptr;  // pinned [slot][nn][16]
+    size_t band_cap = 0;      // in double2
+    std::vector<double2> fk;
+    std::vector<PsvSrc> src;
+    std::vector<double> omg;
+    std::vector<int> lsuf;
+};
+
 struct dsm_sgt {
     int mode = 0, precision = 0, device = 0;
     hipStream_t stream = nullptr;
@@ -101,6 +116,7 @@ struct dsm_sgt {
     DevBuf d_cband, d_csel;
     std::vector<SynthJob> jobs;
     int njobs_main = 0, njobs_syn = 0, njobs_acc = 0;  // launch order: main, accumulate, SYNN
+    PsvPrep prep[2];
     // timing
     bool timing = true;
     std::vector<TimedLaunch> pending;
@@ -801,33 +817,72 @@ static int psv_create(dsm_sgt_t *h)
     return DSM_SGT_OK;
 }
 
-static int psv_run_batch(dsm_sgt_t *h, const std::vector<int> &freqs)
+// Host set-up of the frequencies `freqs` into `p`, split over host threads
+// (psv_omega_setup only reads the set-up).  Returns false if the pinned
+// buffer could not be allocated.
+static bool psv_prepare(const PsvSetup &s, const std::vector<int> &freqs, PsvPrep &p)
+{
+    const int S = (int)freqs.size(), nn = s.nn, R = s.r_n;
+    const size_t need = 16 * (size_t)nn * S;
+    if (p.band_cap < need) {
+        if (p.band) (void)hipHostFree(p.band);
+        p.band = nullptr;
+        p.band_cap = 0;
+        if (hipHostMalloc((void **)&p.band, sizeof(double2) * need, hipHostMallocPortable) != hipSuccess) return false;
+        p.band_cap = need;
+    }
+    p.freqs = freqs;
+    p.fk.assign((size_t)S * R, cz());
+    p.src.assign(S, PsvSrc{});
+    p.omg.assign(S, 0.0);
+    p.lsuf.assign(S, 0);
+    auto work = [&](int k0, int k1) {
+        for (int k = k0; k < k1; k++) {
+            PsvOmega o;
+            psv_omega_setup(s, freqs[k], o);
+            p.omg[k] = o.omega;
+            psv_pack_band(s, o, (double *)(p.band + 16 * (size_t)nn * k));
+            for (int q = 0; q < 160; q++) {
+                p.src[k].anum[q] = make_double2(o.anum[q].real(), o.anum[q].imag());
+                p.src[k].bnum[q] = make_double2(o.bnum[q].real(), o.bnum[q].imag());
+            }
+            p.src[k].coef1 = make_double2(o.coef1_spn.real(), o.coef1_spn.imag());
+            p.src[k].coef2 = make_double2(o.coef2_spn.real(), o.coef2_spn.imag());
+            for (int r = 0; r < R; r++) p.fk[(size_t)k * R + r] = make_double2(o.fluidK[r].real(), o.fluidK[r].imag());
+            p.lsuf[k] = o.lsuf;
+        }
+    };
+    // one thread by default: the set-up already runs beside the device
+    // work of the previous batch, and more threads have slowed the whole
+    // run on hosts with a CPU quota (DSM_SGT_HOST_THREADS overrides)
+    int nt = 1;
+    if (const char *e = std::getenv("DSM_SGT_HOST_THREADS")) nt = std::atoi(e);
+    nt = std::max(1, std::min(nt, S));
+    std::vector<std::thread> pool;
+    for (int t = 1; t < nt; t++) pool.emplace_back(work, S * t / nt, S * (t + 1) / nt);
+    work(0, S / nt);
+    for (auto &th : pool) th.join();
+    return true;
+}
+
+static int psv_run_batch(dsm_sgt_t *h, const PsvPrep &p)
 {
     const PsvSetup &s = h->psv;
     const PsvGeom &g = h->pg;
+    const std::vector<int> &freqs = p.freqs;
     const int S = (int)freqs.size(), nn = s.nn, R = s.r_n, LC = h->lcap;
-    std::vector<double2> band(16 * (size_t)nn * S), fk((size_t)S * R);
-    std::vector<PsvSrc> src(S);
+    const std::vector<PsvSrc> &src = p.src;
+    const std::vector<double2> &fk = p.fk;
+    const std::vector<double> &omg = p.omg;
     std::vector<CutState> cut(S);
     std::vector<int> slotout(S);
-    std::vector<double> omg(S);
     for (int k = 0; k < S; k++) {
-        PsvOmega o;
-        psv_omega_setup(s, freqs[k], o);
-        omg[k] = o.omega;
-        psv_pack_band(s, o, (double *)(band.data() + 16 * (size_t)nn * k));
-        for (int q = 0; q < 160; q++) {
-            src[k].anum[q] = make_double2(o.anum[q].real(), o.anum[q].imag());
-            src[k].bnum[q] = make_double2(o.bnum[q].real(), o.bnum[q].imag());
-        }
-        src[k].coef1 = make_double2(o.coef1_spn.real(), o.coef1_spn.imag());
-        src[k].coef2 = make_double2(o.coef2_spn.real(), o.coef2_spn.imag());
-        for (int r = 0; r < R; r++) fk[(size_t)k * R + r] = make_double2(o.fluidK[r].real(), o.fluidK[r].imag());
-        cut[k].lsuf = o.lsuf;
+        cut[k].lsuf = p.lsuf[k];
         cut[k].llog = s.maxlmax;
         slotout[k] = freqs[k] - h->ipos0;
     }
-    HIPCHK(h, hipMemcpyAsync(h->d_band.p, band.data(), sizeof(double2) * band.size(), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipMemcpyAsync(h->d_band.p, p.band, sizeof(double2) * 16 * (size_t)nn * S, hipMemcpyHostToDevice,
+                             h->stream));
     HIPCHK(h, hipMemcpyAsync(h->d_src.p, src.data(), sizeof(PsvSrc) * S, hipMemcpyHostToDevice, h->stream));
     HIPCHK(h, hipMemcpyAsync(h->d_fluidK.p, fk.data(), sizeof(double2) * fk.size(), hipMemcpyHostToDevice, h->stream));
     HIPCHK(h, hipMemcpyAsync(h->d_omg.p, omg.data(), sizeof(double) * S, hipMemcpyHostToDevice, h->stream));
@@ -1179,10 +1234,29 @@ int dsm_sgt_run(dsm_sgt_t *h, int32_t i_begin, int32_t i_end)
     std::vector<int> all;
     for (int i = i_begin; i <= i_end; i++)
         if (i != 0) all.push_back(i);
-    for (size_t b = 0; b < all.size(); b += h->nslot) {
-        std::vector<int> batch(all.begin() + b, all.begin() + std::min(all.size(), b + (size_t)h->nslot));
-        int rc = (h->mode == DSM_SGT_SH) ? sh_run_batch(h, batch) : psv_run_batch(h, batch);
-        if (rc) return rc;
+    std::vector<std::vector<int>> batches;
+    for (size_t b = 0; b < all.size(); b += h->nslot)
+        batches.emplace_back(all.begin() + b, all.begin() + std::min(all.size(), b + (size_t)h->nslot));
+    if (h->mode == DSM_SGT_SH) {
+        for (auto &batch : batches) {
+            int rc = sh_run_batch(h, batch);
+            if (rc) return rc;
+        }
+        return DSM_SGT_OK;
+    }
+    // P-SV: the host set-up of batch b+1 runs while batch b is on the device
+    const int nb = (int)batches.size();
+    if (nb == 0) return DSM_SGT_OK;
+    auto prepare = [h, &batches](int b) { return psv_prepare(h->psv, batches[b], h->prep[b & 1]); };
+    std::future<bool> next = std::async(std::launch::async, prepare, 0);
+    for (int b = 0; b < nb; b++) {
+        if (!next.get()) return fail(h, DSM_SGT_ENOMEM, "pinned host buffer");
+        if (b + 1 < nb) next = std::async(std::launch::async, prepare, b + 1);
+        int rc = psv_run_batch(h, h->prep[b & 1]);
+        if (rc) {
+            if (b + 1 < nb) next.wait();
+            return rc;
+        }
     }
     return DSM_SGT_OK;
 }
@@ -1338,6 +1412,8 @@ void dsm_sgt_destroy(dsm_sgt_t *h)
         ev_collect(h);
         (void)hipStreamDestroy(h->stream);
     }
+    for (auto &p : h->prep)
+        if (p.band) (void)hipHostFree(p.band);
     delete h;
 }
 
