@@ -132,9 +132,14 @@ def main():
     mode = dsm_sgt.PSV if args.mode == "psv" else dsm_sgt.SH
     prec = getattr(dsm_sgt, FORMS[args.form][0])
 
+    detail = []
+
     def step(pr):
+        t0 = time.perf_counter()
         eng = dsm_sgt.Engine(inp, mode=mode, precision=pr)
+        t1 = time.perf_counter()
         eng.run(1, p["imax"])  # returns with the device synchronised
+        detail.append({"create_s": t1 - t0, "run_s": time.perf_counter() - t1})
         return eng
 
     for _ in range(args.warmup):
@@ -146,6 +151,8 @@ def main():
         t0 = time.perf_counter()
         eng = step(prec)
         times.append(time.perf_counter() - t0)
+        print("step %.3f s: %s" % (times[-1], detail[-1]), file=sys.stderr, flush=True)
+    steps_detail = detail[-args.steps:]
     st = eng.stats()
     cuts = [eng.lcut(i) for i in (1, p["imax"])]
     info = eng.info
@@ -210,6 +217,7 @@ def main():
                      "flops_per_launch": flops, "note": bound_note},
         "stages_ms": {"solve+coef": st.solve_ms, "legendre": st.legendre_ms, "synth": st.synth_ms,
                       "other": st.other_ms},
+        "steps_detail": steps_detail,
         "solve": {"systems": st.systems, "bytes_design": st.solve_bytes},
     }
     if fp64 is not None:

@@ -18,6 +18,8 @@
 #include <cstdlib>
 #include <cstring>
 #include <future>
+#include <map>
+#include <mutex>
 #include <string>
 #include <sys/stat.h>
 #include <thread>
@@ -36,21 +38,79 @@ namespace {
 
 constexpr double PI = 3.1415926535897932;
 
+// Device blocks released by destroyed engines, kept for the next engine of
+// the process: an engine holds ~100 GB, and freeing it and allocating it
+// again has stalled engine creation for seconds.  Per device; a released
+// block is reused for a request of at least half its size; all blocks of
+// a device are freed when an allocation fails.  Never torn down (the
+// process exit returns the memory).
+class BufCache {
+    std::mutex m_;
+    std::multimap<std::pair<int, size_t>, void *> free_;
+
+public:
+    void *take(int dev, size_t bytes, size_t &got)
+    {
+        std::lock_guard<std::mutex> g(m_);
+        auto it = free_.lower_bound({dev, bytes});
+        if (it == free_.end() || it->first.first != dev || it->first.second / 2 > bytes) return nullptr;
+        void *p = it->second;
+        got = it->first.second;
+        free_.erase(it);
+        return p;
+    }
+    void give(int dev, void *p, size_t n)
+    {
+        std::lock_guard<std::mutex> g(m_);
+        free_.insert({{dev, n}, p});
+    }
+    void flush(int dev)
+    {
+        std::lock_guard<std::mutex> g(m_);
+        for (auto it = free_.begin(); it != free_.end();) {
+            if (it->first.first == dev) {
+                (void)hipFree(it->second);
+                it = free_.erase(it);
+            } else {
+                ++it;
+            }
+        }
+    }
+};
+BufCache &buf_cache()
+{
+    static BufCache *c = new BufCache();
+    return *c;
+}
+
 struct DevBuf {
     void *p = nullptr;
     size_t n = 0;
+    int dev = 0;
     ~DevBuf()
     {
-        if (p) (void)hipFree(p);
+        if (p) buf_cache().give(dev, p, n);
     }
     hipError_t ensure(size_t bytes)
     {
         if (bytes <= n) return hipSuccess;
-        if (p) (void)hipFree(p);
+        if (p) buf_cache().give(dev, p, n);
         p = nullptr;
         n = 0;
+        if (hipGetDevice(&dev) != hipSuccess) dev = 0;
+        size_t got = 0;
+        if ((p = buf_cache().take(dev, bytes, got))) {
+            n = got;
+            return hipSuccess;
+        }
         hipError_t e = hipMalloc(&p, bytes);
+        if (e == hipErrorOutOfMemory) {
+            (void)hipGetLastError();
+            buf_cache().flush(dev);
+            e = hipMalloc(&p, bytes);
+        }
         if (e == hipSuccess) n = bytes;
+        else p = nullptr;
         return e;
     }
     template <class T> T *as() const { return (T *)p; }
@@ -111,7 +171,7 @@ struct dsm_sgt {
     // depth cut of the batched forms (0: off): per-row radius / wave speed,
     // per-slot angular frequency, first solved row per (slot, l block)
     double dcut = 0.0;
-    DevBuf d_rrow, d_vrow, d_omg, d_kstart;
+    DevBuf d_rrow, d_vrow, d_omg, d_kstart, d_rlow, d_rneed;
     // fused FP64 family (mixed forms): compact band and its selector bits
     DevBuf d_cband, d_csel;
     std::vector<SynthJob> jobs;
@@ -777,10 +837,17 @@ static int psv_create(dsm_sgt_t *h)
             HIPCHK(h, h->d_cband.ensure(sizeof(double2) * 8 * (size_t)nn * S));
         }
         HIPCHK(h, h->d_kstart.ensure(sizeof(int) * (size_t)S * (psv_solve_blocks(0, lch - 1) + 1)));
+        std::vector<int> rlow(R);
+        for (int r = 0; r < R; r++) rlow[r] = s.ksta[r] - s.row0;
+        HIPCHK(h, h->d_rlow.ensure(sizeof(int) * R));
+        HIPCHK(h, hipMemcpy(h->d_rlow.p, rlow.data(), sizeof(int) * R, hipMemcpyHostToDevice));
+        HIPCHK(h, h->d_rneed.ensure((size_t)R * (psv_solve_blocks(0, lch - 1) + 1)));
     }
     HIPCHK(h, h->d_xn.ensure(sizeof(double2) * (size_t)S * h->nneed * PSV_NSOL * LC));
-    HIPCHK(h, h->d_E.ensure(sizeof(double2) * (size_t)S * PSV_NQ * R * LC));
-    HIPCHK(h, h->d_Es.ensure(sizeof(double2) * (size_t)S * PSV_NQS * LC));
+    if (!h->ns) {  // the bf16 forms write split rows directly (alloc_b16)
+        HIPCHK(h, h->d_E.ensure(sizeof(double2) * (size_t)S * PSV_NQ * R * LC));
+        HIPCHK(h, h->d_Es.ensure(sizeof(double2) * (size_t)S * PSV_NQS * LC));
+    }
     HIPCHK(h, h->d_amp.ensure(sizeof(double2) * 2 * (size_t)S * LC));
     const size_t np = (size_t)std::max(1, h->npos);
     HIPCHK(h, h->d_cat_t.ensure(sizeof(double2) * np * R * s.theta_n * 20));
@@ -984,6 +1051,9 @@ static int psv_run_batch(dsm_sgt_t *h, const PsvPrep &p)
             da.kstart = h->d_kstart.as<int>();
             launch_psv_depth(da, h->stream);
             if (h->ns && l0 == 0 && lend == s.maxlmax) {
+                PsvNeedArgs na{h->d_kstart.as<int>(), h->d_rlow.as<int>(), (int)active.size(), da.nblk, R,
+                               h->d_rneed.as<unsigned char>()};
+                launch_psv_need(na, h->stream);
                 kst_nblk = da.nblk;
                 kst_host.resize((size_t)S * kst_nblk);
                 HIPCHK(h, hipMemcpyAsync(kst_host.data(), h->d_kstart.p, sizeof(int) * kst_host.size(),
@@ -1018,6 +1088,7 @@ static int psv_run_batch(dsm_sgt_t *h, const PsvPrep &p)
         if (h->ns) {
             ca.eb = B16Rows{h->d_eb.as<unsigned short>(), h->ns, S, R, LC};
             ca.esb = B16Rows{h->d_esb.as<unsigned short>(), h->ns, S, 1, LC};
+            if (h->dcut > 0.0 && l0 == 0 && lend == s.maxlmax) ca.rneed = h->d_rneed.as<unsigned char>();
         }
         launch_psv_coef(ca, nblocks, h->stream);
         HIPCHK(h, hipGetLastError());

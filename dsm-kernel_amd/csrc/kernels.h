@@ -20,6 +20,7 @@ void launch_psv_excite(const PsvSolveArgs &a, int nslots, hipStream_t s);
 void launch_psv_solve(const PsvSolveArgs &a, int nslots, bool fast, hipStream_t s);
 int psv_solve_blocks(int l0, int lend);  // l blocks per frequency of launch_psv_solve
 void launch_psv_depth(const PsvDepthArgs &a, hipStream_t s);
+void launch_psv_need(const PsvNeedArgs &a, hipStream_t s);
 // compact band of the fused family (PsvSolveArgs::cband)
 void launch_psv_compact(const double2 *band, const int *kind, int nn, const int *slot_list, int nslots,
                         double2 *cband, hipStream_t s);

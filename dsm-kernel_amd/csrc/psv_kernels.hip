@@ -49,6 +49,27 @@ __global__ __launch_bounds__(64) void psv_depth_kernel(PsvDepthArgs a)
     a.kstart[t] = max(1, kst - a.row0 + 1);
 }
 
+__global__ __launch_bounds__(64) void psv_need_kernel(PsvNeedArgs a)
+{
+    const int t = blockIdx.x * 64 + threadIdx.x;
+    if (t >= a.nblk * a.r_n) return;
+    const int b = t / a.r_n, r = t - b * a.r_n;
+    const int S = a.nslots, npair = a.r_n * S;
+    const int lo = ((r * S) / 64) * 64, hi = min(npair, ((r * S + S - 1) / 64 + 1) * 64) - 1;
+    unsigned char f = 0;
+    for (int p = lo; p <= hi && !f; p++) {
+        const int rr = p / S, sl = p - rr * S;
+        if (a.kstart[sl * a.nblk + b] <= a.rlow[rr]) f = 1;
+    }
+    a.rneed[t] = f;
+}
+
+void launch_psv_need(const PsvNeedArgs &a, hipStream_t s)
+{
+    const int n = a.nblk * a.r_n;
+    hipLaunchKernelGGL(psv_need_kernel, dim3((n + 63) / 64), dim3(64), 0, s, a);
+}
+
 void launch_psv_depth(const PsvDepthArgs &a, hipStream_t s)
 {
     const int n = a.nslots * a.nblk;

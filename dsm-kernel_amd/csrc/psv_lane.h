@@ -69,6 +69,17 @@ struct PsvSolveArgs {
 // operator decays by at most ~e^-1.3 per node whatever kappa is.  Rows
 // below are dropped from the system: their share of the solution is below
 // e^-dcut.
+// Receivers whose coefficient rows some synthesis tile reads in each
+// block of 256 degrees: a tile holds the rows (r', slot) of 64 consecutive
+// receiver-major pairs, and reads a block if for one of them the block's
+// solved rows reach the receiver's stencil (kstart <= rlow(r')).
+struct PsvNeedArgs {
+    const int *kstart;     // [slot][nblk]
+    const int *rlow;       // [r]: local row of ksta - 1
+    int nslots, nblk, r_n;
+    unsigned char *rneed;  // [nblk][r_n]
+};
+
 struct PsvDepthArgs {
     const double *rrow, *vrow;  // [nn] (psv_row_profile)
     const double *omega;        // [slot] real angular frequency
@@ -93,6 +104,7 @@ struct PsvCoefArgs {
     double2 *Es;           // [slot][PSV_NQS][lcap]
     double2 *ampuv;        // [slot][lcap][2]
     B16Rows eb, esb;       // mixed forms (eb.Eb non-null): bf16 rows instead of E, Es
+    const unsigned char *rneed;  // batched forms: [l block of 256][r] rows some synthesis tile reads (null: all)
 };
 
 struct PsvL0Args {
@@ -789,6 +801,7 @@ __host__ __device__ inline void psv_coef_lane(const PsvCoefArgs &a, int b, int t
     const int R = a.r_n, LC = a.lcap;
     const double il = 1.0 / sqrt((double)l * (double)(l + 1));
     const bool mu1 = true, mu2 = (l >= 2);
+    const unsigned char *rneed = a.rneed ? a.rneed + (size_t)((l - a.l0) >> 8) * R : nullptr;
     const double2 *X = a.xn + (size_t)sl * a.nneed * PSV_NSOL * LC + l;
     auto xv = [&](int p, int s) { return X[(size_t)(PSV_NSOL * p + s) * LC]; };
     // amplitude rows of the top receiver (calamp on imt 1, m = 0)
@@ -810,6 +823,7 @@ __host__ __device__ inline void psv_coef_lane(const PsvCoefArgs &a, int b, int t
     }
     double2 *E = a.E + (size_t)sl * PSV_NQ * R * LC + l;
     for (int r = 0; r < R; r++) {
+        if (rneed && !rneed[r]) continue;  // below the solved rows of every slot the tile holds
         const int *rr = a.rec_row + 6 * r;
         const double *w0 = a.w0 + 3 * r, *w1 = a.w1 + 3 * r;
         PsvVals z[PSV_NSOL];
