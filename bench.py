@@ -95,13 +95,14 @@ BF16_PEAK = 2.5e15
 FP32_MFMA_PEAK = 157.3e12
 
 
-def catalog_errors(eng, ref, freqs):
-    """Worst relative L2 per catalog component (parity.py) of `eng` against
-    `ref` over the frequencies `freqs`, per kind."""
+def catalog_errors(got_all, ref, freqs):
+    """Worst relative L2 per catalog component (parity.py) of the fetched
+    records `got_all[i]` against the engine `ref` over the frequencies
+    `freqs`, per kind."""
     from parity import component_errors
     worst = {"tsgt": 0.0, "rsgt": 0.0, "synn": 0.0}
     for i in freqs:
-        got, want = eng.fetch_all_fp64(i), ref.fetch_all_fp64(i)
+        got, want = got_all[i], ref.fetch_all_fp64(i)
         for k, kind in enumerate(("tsgt", "rsgt", "synn")):
             worst[kind] = max(worst[kind], float(component_errors(got[k], want[k]).max()))
     return worst
@@ -157,21 +158,23 @@ def main():
     cuts = [eng.lcut(i) for i in (1, p["imax"])]
     info = eng.info
     ms = 1e3 * sum(times) / len(times)
+    freqs = sorted(set(int(round(x)) for x in [1 + k * (p["imax"] - 1) / 16 for k in range(17)]))
+    compare = args.form != "fp64" and not args.no_fp64
+    got_all = {i: eng.fetch_all_fp64(i) for i in freqs} if compare else {}
+    eng.close()
 
     # the plain FP64 form once, and the form's error against it
     fp64 = None
-    if args.form != "fp64" and not args.no_fp64:
+    if compare:
         t0 = time.perf_counter()
         ref = step(dsm_sgt.FP64)
         fp64_s = time.perf_counter() - t0
         fst = ref.stats()
-        freqs = sorted(set(int(round(x)) for x in [1 + k * (p["imax"] - 1) / 16 for k in range(17)]))
-        errs = catalog_errors(eng, ref, freqs)
+        errs = catalog_errors(got_all, ref, freqs)
         fp64 = {"wall_s": fp64_s, "speedup": fp64_s / (ms / 1e3),
                 "stages_ms": {"solve+coef": fst.solve_ms, "synth": fst.synth_ms, "other": fst.other_ms},
                 "max_rel_l2_vs_fp64": errs, "error_frequencies": freqs, "bar": 1e-6}
         ref.close()
-    eng.close()
 
     r_n, th_n, nn = info.r_n, info.theta_n, info.nn
     launches = max(st.synth_launches, 1)

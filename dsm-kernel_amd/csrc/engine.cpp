@@ -171,7 +171,7 @@ struct dsm_sgt {
     // depth cut of the batched forms (0: off): per-row radius / wave speed,
     // per-slot angular frequency, first solved row per (slot, l block)
     double dcut = 0.0;
-    DevBuf d_rrow, d_vrow, d_omg, d_kstart, d_rlow, d_rneed;
+    DevBuf d_rrow, d_vrow, d_omg, d_kstart, d_rlow, d_rneed, d_zs, d_loff, d_lsuf, d_llog;
     // fused FP64 family (mixed forms): compact band and its selector bits
     DevBuf d_cband, d_csel;
     std::vector<SynthJob> jobs;
@@ -765,7 +765,10 @@ static int psv_create(dsm_sgt_t *h)
     h->theta_pad = ((s.theta_n + 127) / 128) * 128;
     h->lcap = ((s.maxlmax + 1 + 63) / 64) * 64;
     h->nneed = (int)g.need.size();
-    h->nslot = 32;
+    // the batched forms store the factors of the depth-cut systems only,
+    // which leaves room for four times more frequencies per batch
+    h->nslot = (h->precision == DSM_SGT_FP64) ? 32 : 128;
+    if (const char *e = std::getenv("DSM_SGT_NSLOT")) h->nslot = std::max(1, std::atoi(e));
     // one solve launch covers the whole l range unless told otherwise
     h->lchunk = s.maxlmax + 1;
     if (const char *e = std::getenv("DSM_SGT_LCHUNK")) h->lchunk = std::max(64, std::atoi(e));
@@ -813,7 +816,10 @@ static int psv_create(dsm_sgt_t *h)
     const int n = nn - s.row0 + 1, klo = g.rowlo - s.row0 + 1, fs = s.jtmp - s.row0 + 1;
     const int lch = std::min(h->lchunk, s.maxlmax + 1);
     const size_t nthreads = (size_t)S * (psv_solve_blocks(0, lch - 1) + 1) * 256;
-    HIPCHK(h, h->d_scratch.ensure(sizeof(double2) * (size_t)(3 * (n - klo + 1) + 4 * (n - fs + 1)) * nthreads));
+    HIPCHK(h, h->d_zs.ensure(sizeof(double2) * (size_t)4 * (n - fs + 1) * nthreads));
+    if (h->precision == DSM_SGT_FP64)  // full systems; the batched forms size it per batch
+        HIPCHK(h, h->d_scratch.ensure(sizeof(double2) * (size_t)3 * (n - klo + 1) * nthreads));
+    HIPCHK(h, h->d_loff.ensure(sizeof(long long) * (size_t)S * (psv_solve_blocks(0, lch - 1) + 1) + 16));
     HIPCHK(h, h->d_gexc.ensure(sizeof(double2) * (size_t)S * 16 * LC));
     HIPCHK(h, h->d_ends.ensure(sizeof(double2) * 3 * nthreads));
     // batched forms: depth cut (DSM_SGT_DEPTH_CUT overrides; 0 disables)
@@ -842,6 +848,8 @@ static int psv_create(dsm_sgt_t *h)
         HIPCHK(h, h->d_rlow.ensure(sizeof(int) * R));
         HIPCHK(h, hipMemcpy(h->d_rlow.p, rlow.data(), sizeof(int) * R, hipMemcpyHostToDevice));
         HIPCHK(h, h->d_rneed.ensure((size_t)R * (psv_solve_blocks(0, lch - 1) + 1)));
+        HIPCHK(h, h->d_lsuf.ensure(sizeof(int) * S));
+        HIPCHK(h, h->d_llog.ensure(sizeof(int) * S));
     }
     HIPCHK(h, h->d_xn.ensure(sizeof(double2) * (size_t)S * h->nneed * PSV_NSOL * LC));
     if (!h->ns) {  // the bf16 forms write split rows directly (alloc_b16)
@@ -953,6 +961,8 @@ static int psv_run_batch(dsm_sgt_t *h, const PsvPrep &p)
     HIPCHK(h, hipMemcpyAsync(h->d_src.p, src.data(), sizeof(PsvSrc) * S, hipMemcpyHostToDevice, h->stream));
     HIPCHK(h, hipMemcpyAsync(h->d_fluidK.p, fk.data(), sizeof(double2) * fk.size(), hipMemcpyHostToDevice, h->stream));
     HIPCHK(h, hipMemcpyAsync(h->d_omg.p, omg.data(), sizeof(double) * S, hipMemcpyHostToDevice, h->stream));
+    if (h->d_lsuf.p)
+        HIPCHK(h, hipMemcpyAsync(h->d_lsuf.p, p.lsuf.data(), sizeof(int) * S, hipMemcpyHostToDevice, h->stream));
     HIPCHK(h, hipMemcpyAsync(h->d_slotout.p, slotout.data(), sizeof(int) * S, hipMemcpyHostToDevice, h->stream));
     std::vector<int> active(S);
     for (int k = 0; k < S; k++) active[k] = k;
@@ -1000,10 +1010,13 @@ static int psv_run_batch(dsm_sgt_t *h, const PsvPrep &p)
     std::vector<int> kst_host;  // depth cut per (slot, block of 256 l), single-launch case
     int kst_nblk = 0;
     int l0 = 0;
+    bool dev_cut = false;
     while (!active.empty()) {
         const int lend = std::min(l0 + h->lchunk - 1, s.maxlmax);
         const int nlb = (lend - l0 + 64) / 64;
         const int nblocks = (int)active.size() * nlb;
+        // batched forms in one launch: the cut-off is replayed on the device
+        dev_cut = h->precision != DSM_SGT_FP64 && h->d_lsuf.p && l0 == 0 && lend == s.maxlmax;
         HIPCHK(h, hipMemcpyAsync(h->d_slots.p, active.data(), sizeof(int) * active.size(), hipMemcpyHostToDevice,
                                  h->stream));
         PsvSolveArgs sa{};
@@ -1030,6 +1043,8 @@ static int psv_run_batch(dsm_sgt_t *h, const PsvPrep &p)
         sa.kstart = (h->dcut > 0.0) ? h->d_kstart.as<int>() : nullptr;
         sa.cband = h->d_cband.as<double2>();
         sa.csel = h->d_csel.as<int>();
+        sa.zs = h->d_zs.as<double2>();
+        sa.loff = nullptr;
         TimedLaunch t0;
         ev_begin(h, t0, 0);
         if (h->precision != DSM_SGT_FP64 && l0 == 0)
@@ -1050,14 +1065,28 @@ static int psv_run_batch(dsm_sgt_t *h, const PsvPrep &p)
             da.dcap = 0.35;
             da.kstart = h->d_kstart.as<int>();
             launch_psv_depth(da, h->stream);
-            if (h->ns && l0 == 0 && lend == s.maxlmax) {
-                PsvNeedArgs na{h->d_kstart.as<int>(), h->d_rlow.as<int>(), (int)active.size(), da.nblk, R,
-                               h->d_rneed.as<unsigned char>()};
-                launch_psv_need(na, h->stream);
+            if (l0 == 0 && lend == s.maxlmax) {
+                // factor storage of the depth-cut systems: offsets per block, total
+                const int nbt = (int)active.size() * da.nblk;
+                long long *loff = h->d_loff.as<long long>();
+                launch_psv_loff(h->d_kstart.as<int>(), nbt, n, klo, loff, loff + nbt, h->stream);
+                long long total = 0;
+                HIPCHK(h, hipMemcpyAsync(&total, loff + nbt, sizeof(long long), hipMemcpyDeviceToHost, h->stream));
                 kst_nblk = da.nblk;
                 kst_host.resize((size_t)S * kst_nblk);
                 HIPCHK(h, hipMemcpyAsync(kst_host.data(), h->d_kstart.p, sizeof(int) * kst_host.size(),
                                          hipMemcpyDeviceToHost, h->stream));
+                HIPCHK(h, hipStreamSynchronize(h->stream));
+                HIPCHK(h, h->d_scratch.ensure(sizeof(double2) * (size_t)std::max(total, 1LL)));
+                sa.scratch = h->d_scratch.as<double2>();
+                sa.loff = loff;
+            } else if (h->precision != DSM_SGT_FP64) {
+                return fail(h, DSM_SGT_EUNSUPPORTED, "the depth cut needs one solve launch per batch (DSM_SGT_LCHUNK)");
+            }
+            if (h->ns && l0 == 0 && lend == s.maxlmax) {
+                PsvNeedArgs na{h->d_kstart.as<int>(), h->d_rlow.as<int>(), (int)active.size(), da.nblk, R,
+                               h->d_rneed.as<unsigned char>()};
+                launch_psv_need(na, h->stream);
             }
         }
         launch_psv_excite(sa, (int)active.size(), h->stream);
@@ -1092,6 +1121,12 @@ static int psv_run_batch(dsm_sgt_t *h, const PsvPrep &p)
         }
         launch_psv_coef(ca, nblocks, h->stream);
         HIPCHK(h, hipGetLastError());
+        if (dev_cut) {
+            PsvCutArgs cu{h->d_amp.as<double2>(), h->d_lsuf.as<int>(), S, LC, s.maxlmax, s.ratl, h->d_nl.as<int>(),
+                          h->d_llog.as<int>()};
+            launch_psv_cut(cu, h->stream);
+            HIPCHK(h, hipGetLastError());
+        }
         ev_end(h, t0);
         h->stats.solve_launches++;
         const long nsys = (long)active.size() * (lend - std::max(l0, 1) + 1);
@@ -1102,6 +1137,10 @@ static int psv_run_batch(dsm_sgt_t *h, const PsvPrep &p)
         // recorded rows
         h->stats.solve_bytes += (double)nsys * 16.0 * (2.0 * 3 * (n - klo + 1) + 2.0 * 4 * (n - fs + 1) +
                                                        (double)PSV_NSOL * h->nneed);
+        if (dev_cut) {  // the whole l range in one launch, cut-off replayed on the device
+            active.clear();
+            break;
+        }
         HIPCHK(h, hipMemcpy2DAsync(&ampuv[2 * (size_t)l0], sizeof(double2) * 2 * LC,
                                    h->d_amp.as<double2>() + 2 * (size_t)l0, sizeof(double2) * 2 * LC,
                                    sizeof(double2) * 2 * (lend - l0 + 1), S, hipMemcpyDeviceToHost, h->stream));
@@ -1137,12 +1176,14 @@ static int psv_run_batch(dsm_sgt_t *h, const PsvPrep &p)
     }
     std::vector<int> nl(S), all(S);
     for (int k = 0; k < S; k++) {
-        nl[k] = cut[k].nl;
+        nl[k] = dev_cut ? s.maxlmax + 1 : cut[k].nl;  // device cut-off: known after the batch
         all[k] = k;
-        h->llog[slotout[k]] = cut[k].llog - 1;
-        h->done[slotout[k]] = 1;
+        if (!dev_cut) {
+            h->llog[slotout[k]] = cut[k].llog - 1;
+            h->done[slotout[k]] = 1;
+        }
     }
-    HIPCHK(h, hipMemcpyAsync(h->d_nl.p, nl.data(), sizeof(int) * S, hipMemcpyHostToDevice, h->stream));
+    if (!dev_cut) HIPCHK(h, hipMemcpyAsync(h->d_nl.p, nl.data(), sizeof(int) * S, hipMemcpyHostToDevice, h->stream));
     HIPCHK(h, hipMemcpyAsync(h->d_slots.p, all.data(), sizeof(int) * S, hipMemcpyHostToDevice, h->stream));
     TimedLaunch tz;
     ev_begin(h, tz, 3);
@@ -1151,12 +1192,11 @@ static int psv_run_batch(dsm_sgt_t *h, const PsvPrep &p)
         // bf16 rows written by the coefficient kernels: zero the cut-off
         // tails; l extent per (receiver, slot): the l cut-off, and the
         // blocks of l whose solved rows reach the receiver's stencil
-        const int lmin = *std::min_element(nl.begin(), nl.end());
-        launch_zero_tail_b16(h->d_eb.as<unsigned short>(), PSV_NQ, h->ns, S, R, LC, h->d_nl.as<int>(), lmin,
-                             s.maxlmax, h->stream);
-        launch_zero_tail_b16(h->d_esb.as<unsigned short>(), PSV_NQS, h->ns, S, 1, LC, h->d_nl.as<int>(), lmin,
-                             s.maxlmax, h->stream);
-        std::vector<int> nlr((size_t)R * S);
+        launch_zero_tail_b16(h->d_eb.as<unsigned short>(), PSV_NQ, h->ns, S, R, LC, h->d_nl.as<int>(), s.maxlmax,
+                             h->stream);
+        launch_zero_tail_b16(h->d_esb.as<unsigned short>(), PSV_NQS, h->ns, S, 1, LC, h->d_nl.as<int>(), s.maxlmax,
+                             h->stream);
+        std::vector<int> nlr((size_t)R * S);  // host estimate (the cut-off taken from nl) for the tile order
         for (int r = 0; r < R; r++) {
             const int rlow = s.ksta[r] - s.row0;  // local row of ksta-1
             for (int k = 0; k < S; k++) {
@@ -1169,7 +1209,12 @@ static int psv_run_batch(dsm_sgt_t *h, const PsvPrep &p)
                 nlr[(size_t)r * S + k] = lr;
             }
         }
-        HIPCHK(h, hipMemcpyAsync(h->d_nlr.p, nlr.data(), sizeof(int) * nlr.size(), hipMemcpyHostToDevice, h->stream));
+        if (h->dcut > 0.0 && !kst_host.empty())
+            launch_psv_nlr(h->d_kstart.as<int>(), h->d_rlow.as<int>(), h->d_nl.as<int>(), S, kst_nblk, R,
+                           h->d_nlr.as<int>(), h->stream);
+        else
+            HIPCHK(h, hipMemcpyAsync(h->d_nlr.p, nlr.data(), sizeof(int) * nlr.size(), hipMemcpyHostToDevice,
+                                     h->stream));
         rc = b16_tile_order(h, nlr, S, R);
         if (rc) return rc;
     } else {
@@ -1213,6 +1258,16 @@ static int psv_run_batch(dsm_sgt_t *h, const PsvPrep &p)
     HIPCHK(h, hipGetLastError());
     ev_end(h, tg2);
     h->stats.synth_launches += 2;
+    if (dev_cut) {  // the cut-off replayed on the device
+        std::vector<int> llog(S);
+        HIPCHK(h, hipMemcpyAsync(nl.data(), h->d_nl.p, sizeof(int) * S, hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(h, hipMemcpyAsync(llog.data(), h->d_llog.p, sizeof(int) * S, hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(h, hipStreamSynchronize(h->stream));
+        for (int k = 0; k < S; k++) {
+            h->llog[slotout[k]] = llog[k] - 1;
+            h->done[slotout[k]] = 1;
+        }
+    }
     double sumL = 0.0;
     for (int k = 0; k < S; k++) sumL += nl[k];
     // 4 flops per (complex coefficient x real function) per (r,theta)
@@ -1306,8 +1361,16 @@ int dsm_sgt_run(dsm_sgt_t *h, int32_t i_begin, int32_t i_end)
     for (int i = i_begin; i <= i_end; i++)
         if (i != 0) all.push_back(i);
     std::vector<std::vector<int>> batches;
-    for (size_t b = 0; b < all.size(); b += h->nslot)
-        batches.emplace_back(all.begin() + b, all.begin() + std::min(all.size(), b + (size_t)h->nslot));
+    {
+        // a short first batch when batches are large: its host set-up is the
+        // only one not hidden behind device work
+        size_t b = 0, first = (h->mode == DSM_SGT_PSV && h->nslot > 32) ? 16 : (size_t)h->nslot;
+        while (b < all.size()) {
+            const size_t len = batches.empty() ? first : (size_t)h->nslot;
+            batches.emplace_back(all.begin() + b, all.begin() + std::min(all.size(), b + len));
+            b += len;
+        }
+    }
     if (h->mode == DSM_SGT_SH) {
         for (auto &batch : batches) {
             int rc = sh_run_batch(h, batch);

@@ -21,6 +21,13 @@ void launch_psv_solve(const PsvSolveArgs &a, int nslots, bool fast, hipStream_t 
 int psv_solve_blocks(int l0, int lend);  // l blocks per frequency of launch_psv_solve
 void launch_psv_depth(const PsvDepthArgs &a, hipStream_t s);
 void launch_psv_need(const PsvNeedArgs &a, hipStream_t s);
+// l cut-off replay on the device (batched forms) and the bf16 row extents
+void launch_psv_cut(const PsvCutArgs &a, hipStream_t s);
+void launch_psv_nlr(const int *kstart, const int *rlow, const int *nl, int nslots, int nblk, int r_n, int *nlr,
+                    hipStream_t s);
+// per-block factor storage of the batched forms (PsvSolveArgs::loff)
+void launch_psv_loff(const int *kstart, int nblocks, int n, int klo, long long *loff, long long *total,
+                     hipStream_t s);
 // compact band of the fused family (PsvSolveArgs::cband)
 void launch_psv_compact(const double2 *band, const int *kind, int nn, const int *slot_list, int nslots,
                         double2 *cband, hipStream_t s);
@@ -87,9 +94,9 @@ struct SynthB16Args {
     const int *tiles;          // optional [grid]: tile (job ny + ty) nx + tx per workgroup, longest first
     int ntiles;
 };
-// zero degrees l in [max(lmin, nl(slot)), lend] of every bf16 row
-void launch_zero_tail_b16(unsigned short *Eb, int nq, int ns, int nslot, int r_eff, int lcap, const int *nl, int lmin,
-                          int lend, hipStream_t s);
+// zero degrees l in [nl(slot), lend] of every bf16 row
+void launch_zero_tail_b16(unsigned short *Eb, int nq, int ns, int nslot, int r_eff, int lcap, const int *nl, int lend,
+                          hipStream_t s);
 void launch_split_rows(const double2 *E, int nslot, int nq, int r_eff, int lcap, int ns, unsigned short *Eb,
                        hipStream_t s);
 void launch_split_table(const double *tab, int nplanes, int lcap, int theta_pad, int nh, int nhp, int ns,

@@ -6,6 +6,8 @@
 
 namespace dsm {
 
+constexpr int SV_NT = 256, SV_CH = 32;  // lanes per solve workgroup, band columns per LDS chunk
+
 // ---------------------------------------------------------------------------
 // Source excitation g[s][i] (calya + calg, psv_excitation) of every
 // (frequency, l): grid (l blocks of 64, active frequencies).
@@ -70,6 +72,94 @@ void launch_psv_need(const PsvNeedArgs &a, hipStream_t s)
     hipLaunchKernelGGL(psv_need_kernel, dim3((n + 63) / 64), dim3(64), 0, s, a);
 }
 
+// Offsets of the per-block factor storage: block b stores the rows
+// max(klo, kstart[b]) .. n of its 256 lanes (3 complex per row and lane);
+// out[0] = total in double2.  One workgroup of 1024 threads.
+__global__ __launch_bounds__(1024) void psv_loff_kernel(const int *kstart, int nblocks, int n, int klo,
+                                                        long long *loff, long long *total)
+{
+    __shared__ long long part[1024];
+    const int t = threadIdx.x, c = (nblocks + 1023) / 1024;
+    const int b0 = min(nblocks, t * c), b1 = min(nblocks, b0 + c);
+    long long sum = 0;
+    for (int b = b0; b < b1; b++) sum += 3LL * SV_NT * (n - max(klo, kstart[b]) + 1);
+    part[t] = sum;
+    __syncthreads();
+    for (int d = 1; d < 1024; d <<= 1) {  // inclusive scan
+        const long long v = (t >= d) ? part[t - d] : 0;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    long long off = part[t] - sum;
+    for (int b = b0; b < b1; b++) {
+        loff[b] = off;
+        off += 3LL * SV_NT * (n - max(klo, kstart[b]) + 1);
+    }
+    if (t == 1023) total[0] = part[1023];
+}
+
+void launch_psv_loff(const int *kstart, int nblocks, int n, int klo, long long *loff, long long *total,
+                     hipStream_t s)
+{
+    hipLaunchKernelGGL(psv_loff_kernel, dim3(1), dim3(1024), 0, s, kstart, nblocks, n, klo, loff, total);
+}
+
+__global__ __launch_bounds__(64) void psv_cut_kernel(PsvCutArgs a)
+{
+    const int k = blockIdx.x * 64 + threadIdx.x;
+    if (k >= a.nslots) return;
+    const int lsuf = a.lsuf[k];
+    int ismall = 0, nl = a.maxlmax + 1, llog = a.maxlmax;
+    double maxamp = -1.0;
+    for (int l = 1; l <= a.maxlmax; l++) {
+        if (ismall > 20) {
+            if (llog > l) llog = l;
+            nl = l;
+            break;
+        }
+        const double2 u = a.amp[((size_t)k * a.lcap + l) * 2], v = a.amp[((size_t)k * a.lcap + l) * 2 + 1];
+        const double a1 = hypot(u.x, u.y), a2 = hypot(v.x, v.y);
+        const double a0 = sqrt(a1 * a1 + a2 * a2);
+        for (int m = -2; m <= 2; m++) {
+            if (abs(m) > l) continue;
+            const double am = (m == 0) ? a0 : 0.0;
+            if (am > maxamp) maxamp = am;
+            const double ratio = (am != 0.0 && maxamp != 0.0) ? am / maxamp : 0.0;
+            if (ratio < a.ratl && l >= lsuf) ismall++;
+            else ismall = 0;
+        }
+    }
+    a.nl[k] = nl;
+    a.llog[k] = llog;
+}
+
+void launch_psv_cut(const PsvCutArgs &a, hipStream_t s)
+{
+    hipLaunchKernelGGL(psv_cut_kernel, dim3((a.nslots + 63) / 64), dim3(64), 0, s, a);
+}
+
+// l extent of every (receiver, slot) row of the bf16 synthesis: the l
+// cut-off and the leading blocks of 256 degrees whose solved rows reach
+// the receiver's stencil
+__global__ __launch_bounds__(256) void psv_nlr_kernel(const int *kstart, const int *rlow, const int *nl, int nslots,
+                                                      int nblk, int r_n, int *nlr)
+{
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    if (t >= nslots * r_n) return;
+    const int r = t / nslots, k = t - r * nslots;
+    int nb = 0;
+    while (nb < nblk && kstart[k * nblk + nb] <= rlow[r]) nb++;
+    nlr[t] = min(nl[k], 256 * nb);
+}
+
+void launch_psv_nlr(const int *kstart, const int *rlow, const int *nl, int nslots, int nblk, int r_n, int *nlr,
+                    hipStream_t s)
+{
+    hipLaunchKernelGGL(psv_nlr_kernel, dim3((nslots * r_n + 255) / 256), dim3(256), 0, s, kstart, rlow, nl, nslots,
+                       nblk, r_n, nlr);
+}
+
 void launch_psv_depth(const PsvDepthArgs &a, hipStream_t s)
 {
     const int n = a.nslots * a.nblk;
@@ -93,7 +183,6 @@ void launch_psv_depth(const PsvDepthArgs &a, hipStream_t s)
 // Lanes past the l range shadow a valid l and store nothing, so every lane
 // reaches every barrier.
 // ---------------------------------------------------------------------------
-constexpr int SV_NT = 256, SV_CH = 32;
 
 // Compact band of the fused family: entry ii of column j is A + s B with
 // A = e0 (+ e2), B = e1 (+ e3) folded from the full column (two zones'
@@ -149,11 +238,16 @@ __global__ __launch_bounds__(SV_NT) void psv_factor_kernel(PsvSolveArgs a)
     // local column j (1..n) is band row row0+j-1
     const double2 *band0 = (F ? a.cband : a.band) + ((size_t)sl * a.nn + (a.row0 - 1)) * CW;
     const int *kind0 = (F ? a.csel : a.kind) + (a.row0 - 1);
-    double2 *Ls = a.scratch;                                   // [k - klo][3]
-    double2 *Zs = a.scratch + (size_t)3 * (n - klo + 1) * ss;  // [k - fs][4]
     const double2 *gx = a.gexc + (size_t)sl * 16 * a.lcap + l;
     // first solved local row of this block (depth cut of the batched forms)
-    const int kst = a.kstart ? a.kstart[blockIdx.y * gridDim.x + blockIdx.x] : 1;
+    const int blk = blockIdx.y * gridDim.x + blockIdx.x;
+    const int kst = a.kstart ? a.kstart[blk] : 1;
+    // factor rows k >= kL at Lb[(3 (k - kL) + j) Lst + Lln], forward values
+    // at Zs[(4 (k - fs) + s) ss + gid]
+    double2 *Lb = a.loff ? a.scratch + a.loff[blk] : a.scratch;
+    const int kL = a.loff ? max(klo, kst) : klo;
+    const long Lst = a.loff ? SV_NT : ss, Lln = a.loff ? tid : gid;
+    double2 *Zs = a.zs;
 
     const int nch = (n + SV_CH - 1) / SV_CH;
     constexpr int PER = SV_CH * CW / SV_NT;  // staged values per thread
@@ -212,9 +306,9 @@ __global__ __launch_bounds__(SV_NT) void psv_factor_kernel(PsvSolveArgs a)
     W.w4m3 = W.w4m2 = W.w4m1 = W.w3m2 = W.w3m1 = W.w2m1 = cz();
     auto storeL = [&](int k, double2 w2, double2 w3, double2 w4) {
         if (act && k >= klo) {
-            Ls[(size_t)(3 * (k - klo) + 0) * ss + gid] = w2;
-            Ls[(size_t)(3 * (k - klo) + 1) * ss + gid] = w3;
-            Ls[(size_t)(3 * (k - klo) + 2) * ss + gid] = w4;
+            Lb[(3 * (k - kL) + 0) * Lst + Lln] = w2;
+            Lb[(3 * (k - kL) + 1) * Lst + Lln] = w3;
+            Lb[(3 * (k - kL) + 2) * Lst + Lln] = w4;
         }
     };
     // phase 1: rows below the source, factorization only (fs <= n - 3);
@@ -308,11 +402,14 @@ __global__ __launch_bounds__(64) void psv_back_kernel(PsvSolveArgs a)
     const int fs = a.jtmp - a.row0 + 1;
     const int klo = a.rowlo - a.row0 + 1;
     const int nsrc = (l >= 2) ? 4 : 3;
-    const double2 *Ls = a.scratch;
-    const double2 *Zs = a.scratch + (size_t)3 * (n - klo + 1) * ss;
+    const double2 *Zs = a.zs;
     const double2 Dn1 = a.ends[gid], Dn = a.ends[ss + gid], w2n1 = a.ends[2 * ss + gid];
-    const int kst = a.kstart ? a.kstart[blockIdx.y * (gridDim.x >> 2) + xb] : 1;
+    const int blk = blockIdx.y * (gridDim.x >> 2) + xb;
+    const int kst = a.kstart ? a.kstart[blk] : 1;
     const int kbot = max(klo, kst);  // rows below kst are outside the solved system
+    const double2 *Lb = a.loff ? a.scratch + a.loff[blk] : a.scratch;
+    const int kL = a.loff ? kbot : klo;
+    const long Lst = a.loff ? SV_NT : ss, Lln = a.loff ? sub * 64 + threadIdx.x : gid;
 
     // back-propagated forces: forward values at rows n-1, n only
     const double c0f = (double)(float)sqrt((double)(2 * l + 1) / (4.0 * DSM_PI) / 1.0);
@@ -335,9 +432,9 @@ __global__ __launch_bounds__(64) void psv_back_kernel(PsvSolveArgs a)
     auto ld = [&](int k) {
         Row r{cz(), cz(), cz()};
         if (k >= kbot) {
-            r.L2 = (k < n) ? Ls[(size_t)(3 * (k - klo) + 0) * ss + gid] : cz();
-            r.L3 = (k < n - 1) ? Ls[(size_t)(3 * (k - klo) + 1) * ss + gid] : cz();
-            r.L4 = (k < n - 2) ? Ls[(size_t)(3 * (k - klo) + 2) * ss + gid] : cz();
+            r.L2 = (k < n) ? Lb[(3 * (k - kL) + 0) * Lst + Lln] : cz();
+            r.L3 = (k < n - 1) ? Lb[(3 * (k - kL) + 1) * Lst + Lln] : cz();
+            r.L4 = (k < n - 2) ? Lb[(3 * (k - kL) + 2) * Lst + Lln] : cz();
         }
         return r;
     };

@@ -57,6 +57,10 @@ struct PsvSolveArgs {
     const int *kstart;     // [active slot][l block of 256]: first solved local row (null: 1)
     const double2 *cband;  // fused family: [slot][nn][4][A, B] (psv_compact_kernel)
     const int *csel;       // [nn]: bit ii set where entry ii scales with sqrt(l(l+1))
+    double2 *zs;           // forward values: [k - fs][4][sstride]
+    // batched forms: factors of (active slot, l block) blk from scratch + loff[blk],
+    // [k - max(klo, kstart)][3][256]; null: [k - klo][3][sstride]
+    const long long *loff;
 };
 
 // Depth cut of the batched forms: for the smallest l of every block of 256
@@ -78,6 +82,18 @@ struct PsvNeedArgs {
     const int *rlow;       // [r]: local row of ksta - 1
     int nslots, nblk, r_n;
     unsigned char *rneed;  // [nblk][r_n]
+};
+
+// l cut-off of every slot (calamp replay, SGTpsv-MPI.f90:616-634,
+// 884-888): from the amplitudes of the top receiver's rows, stop after
+// more than 20 consecutive degrees whose amplitude ratio is below ratl
+// past lsuf; m = +-1, +-2 contribute zero amplitudes.
+struct PsvCutArgs {
+    const double2 *amp;  // [slot][lcap][2]
+    const int *lsuf;     // [slot]
+    int nslots, lcap, maxlmax;
+    double ratl;
+    int *nl, *llog;      // [slot]: number of degrees, the reference's llog
 };
 
 struct PsvDepthArgs {

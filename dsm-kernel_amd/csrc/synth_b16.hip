@@ -87,19 +87,15 @@ __global__ void split_rows_kernel(const double2 *E, int nslot, int nq, int R, in
     }
 }
 
-// zero the degrees l in [nl(slot), lend] of every row (the l cut-off)
-__global__ void zero_tail_b16_kernel(u16 *Eb, int nq, int ns, int nslot, int R, int lcap, const int *nl, int lmin,
-                                     int lend)
+// zero the degrees l in [nl(slot), lend] of every row (the l cut-off):
+// one row per thread, nothing to do for slots that run to lend
+__global__ void zero_tail_b16_kernel(u16 *Eb, int nq, int ns, int nslot, int R, int lcap, const int *nl, int lend)
 {
     const size_t rows = (size_t)nq * ns * nslot * R * 2;
-    const int span = lend + 1 - lmin;
-    for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < rows * span;
-         idx += (size_t)gridDim.x * blockDim.x) {
-        const int l = lmin + (int)(idx % span);
-        const size_t row = idx / span;
-        const size_t m = row % ((size_t)nslot * R * 2);
-        const int sl = (int)((m >> 1) % nslot);
-        if (l >= nl[sl]) Eb[row * lcap + lperm(l, lcap)] = 0;
+    const size_t M = (size_t)nslot * R * 2;
+    for (size_t row = (size_t)blockIdx.x * blockDim.x + threadIdx.x; row < rows; row += (size_t)gridDim.x * blockDim.x) {
+        const int sl = (int)(((row % M) >> 1) % nslot);
+        for (int l = nl[sl]; l <= lend; l++) Eb[row * lcap + lperm(l, lcap)] = 0;
     }
 }
 
@@ -312,12 +308,10 @@ void launch_split_rows(const double2 *E, int nslot, int nq, int r_eff, int lcap,
     hipLaunchKernelGGL(split_rows_kernel, dim3(8192), dim3(256), 0, s, E, nslot, nq, r_eff, lcap, ns, Eb);
 }
 
-void launch_zero_tail_b16(unsigned short *Eb, int nq, int ns, int nslot, int r_eff, int lcap, const int *nl, int lmin,
-                          int lend, hipStream_t s)
+void launch_zero_tail_b16(unsigned short *Eb, int nq, int ns, int nslot, int r_eff, int lcap, const int *nl, int lend,
+                          hipStream_t s)
 {
-    if (lmin > lend) return;
-    hipLaunchKernelGGL(zero_tail_b16_kernel, dim3(8192), dim3(256), 0, s, Eb, nq, ns, nslot, r_eff, lcap, nl, lmin,
-                       lend);
+    hipLaunchKernelGGL(zero_tail_b16_kernel, dim3(4096), dim3(256), 0, s, Eb, nq, ns, nslot, r_eff, lcap, nl, lend);
 }
 
 void launch_split_table(const double *tab, int nplanes, int lcap, int theta_pad, int nh, int nhp, int ns,
