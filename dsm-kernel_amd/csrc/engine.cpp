@@ -320,18 +320,45 @@ static int b16_tile_order(dsm_sgt_t *h, const std::vector<int> &nlr, int S, int 
     for (int ty = 0; ty < ny; ty++)
         for (int pr = (ty * 128) >> 1; pr <= (std::min(M - 1, ty * 128 + 127) >> 1); pr++)
             kt[ty] = std::max(kt[ty], nlr[pr]);
+    // Tile order.  A workgroup's blockIdx goes to XCD blockIdx % 8, and the
+    // XCDs have separate L2s.  The nx column tiles of one (job, row block)
+    // group read the same coefficient rows, and row blocks of one job read
+    // the same table columns: chunks of XG groups (sorted longest series
+    // first, so a chunk's groups are of one job and finish together) go to
+    // one XCD, XG nx workgroups that share their operands through its L2.
+    // Chunks are dealt to the least loaded XCD; slots past an XCD's queue
+    // hold -1 (no tile).
+    constexpr int NXCD = 8;
+    const int XG = std::max(1, 32 / nx);
     auto order = [&](int j0, int nj, std::vector<int> &out) {
-        std::vector<int> rows;
+        std::vector<int> groups;
+        std::vector<double> w(nj * ny);
         for (int j = 0; j < nj; j++)
-            for (int ty = 0; ty < ny; ty++) rows.push_back(j * ny + ty);
-        std::stable_sort(rows.begin(), rows.end(), [&](int a, int b) { return kt[a % ny] > kt[b % ny]; });
-        out.clear();
-        for (int jt : rows)
-            for (int tx = 0; tx < nx; tx++) out.push_back(jt * nx + tx);
-        (void)j0;
+            for (int ty = 0; ty < ny; ty++) {
+                groups.push_back(j * ny + ty);
+                const int ce = ((kt[ty] + 1) / 2 + 31) / 32, co = (kt[ty] / 2 + 31) / 32;
+                w[j * ny + ty] = (double)h->jobs[j0 + j].nb * (ce + co);
+            }
+        std::stable_sort(groups.begin(), groups.end(), [&](int a, int b) { return w[a] > w[b]; });
+        std::vector<std::vector<int>> q(NXCD);
+        double load[NXCD] = {};
+        for (size_t c = 0; c < groups.size(); c += XG) {
+            int x = 0;
+            for (int y = 1; y < NXCD; y++)
+                if (load[y] < load[x]) x = y;
+            for (size_t g = c; g < std::min(groups.size(), c + XG); g++) {
+                load[x] += w[groups[g]] * nx;
+                for (int tx = 0; tx < nx; tx++) q[x].push_back(groups[g] * nx + tx);
+            }
+        }
+        size_t len = 0;
+        for (auto &v : q) len = std::max(len, v.size());
+        out.assign(len * NXCD, -1);
+        for (int x = 0; x < NXCD; x++)
+            for (size_t i = 0; i < q[x].size(); i++) out[i * NXCD + x] = q[x][i];
     };
     order(0, h->njobs_main, h->tiles_main);
-    order(0, h->njobs_acc, h->tiles_acc);
+    order(h->njobs_main, h->njobs_acc, h->tiles_acc);
     // MFMA flops issued: per tile and K step of 32 degrees, 2*128*128*32
     // per cross product
     const double per = 2.0 * 128 * 128 * 32 * (h->ns == 3 ? 6 : 3);
@@ -1114,6 +1141,11 @@ static int psv_run_batch(dsm_sgt_t *h, const PsvPrep &p)
         ca.E = h->d_E.as<double2>();
         ca.Es = h->d_Es.as<double2>();
         ca.ampuv = h->d_amp.as<double2>();
+        if (h->dcut > 0.0) {  // receivers below a block's solved rows: zero rows
+            ca.kstart = h->d_kstart.as<int>();
+            ca.rlow = h->d_rlow.as<int>();
+            ca.nblk = psv_solve_blocks(l0, lend);
+        }
         if (h->ns) {
             ca.eb = B16Rows{h->d_eb.as<unsigned short>(), h->ns, S, R, LC};
             ca.esb = B16Rows{h->d_esb.as<unsigned short>(), h->ns, S, 1, LC};

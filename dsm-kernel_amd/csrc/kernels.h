@@ -91,7 +91,7 @@ struct SynthB16Args {
     const SynthJob *jobs;
     int njobs;
     double *stash;             // [workgroup][8 waves][32][64 lanes]
-    const int *tiles;          // optional [grid]: tile (job ny + ty) nx + tx per workgroup, longest first
+    const int *tiles;          // optional [grid]: tile (job ny + ty) nx + tx per workgroup, -1 none
     int ntiles;
 };
 // zero degrees l in [nl(slot), lend] of every bf16 row

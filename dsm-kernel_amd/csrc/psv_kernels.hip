@@ -105,38 +105,58 @@ void launch_psv_loff(const int *kstart, int nblocks, int n, int klo, long long *
     hipLaunchKernelGGL(psv_loff_kernel, dim3(1), dim3(1024), 0, s, kstart, nblocks, n, klo, loff, total);
 }
 
+// One 64-lane workgroup per slot: the lanes form the amplitudes of 64
+// degrees at a time, lane 0 replays the count over them in order.
 __global__ __launch_bounds__(64) void psv_cut_kernel(PsvCutArgs a)
 {
-    const int k = blockIdx.x * 64 + threadIdx.x;
-    if (k >= a.nslots) return;
+    __shared__ double amp0[64];
+    __shared__ int stop;
+    const int k = blockIdx.x, tx = threadIdx.x;
     const int lsuf = a.lsuf[k];
     int ismall = 0, nl = a.maxlmax + 1, llog = a.maxlmax;
     double maxamp = -1.0;
-    for (int l = 1; l <= a.maxlmax; l++) {
-        if (ismall > 20) {
-            if (llog > l) llog = l;
-            nl = l;
-            break;
+    if (tx == 0) stop = 0;
+    for (int l0 = 1; l0 <= a.maxlmax; l0 += 64) {
+        const int l = l0 + tx;
+        double a0 = 0.0;
+        if (l <= a.maxlmax) {
+            const double2 u = a.amp[((size_t)k * a.lcap + l) * 2], v = a.amp[((size_t)k * a.lcap + l) * 2 + 1];
+            const double a1 = hypot(u.x, u.y), a2 = hypot(v.x, v.y);
+            a0 = sqrt(a1 * a1 + a2 * a2);
         }
-        const double2 u = a.amp[((size_t)k * a.lcap + l) * 2], v = a.amp[((size_t)k * a.lcap + l) * 2 + 1];
-        const double a1 = hypot(u.x, u.y), a2 = hypot(v.x, v.y);
-        const double a0 = sqrt(a1 * a1 + a2 * a2);
-        for (int m = -2; m <= 2; m++) {
-            if (abs(m) > l) continue;
-            const double am = (m == 0) ? a0 : 0.0;
-            if (am > maxamp) maxamp = am;
-            const double ratio = (am != 0.0 && maxamp != 0.0) ? am / maxamp : 0.0;
-            if (ratio < a.ratl && l >= lsuf) ismall++;
-            else ismall = 0;
+        amp0[tx] = a0;
+        __syncthreads();
+        if (tx == 0) {
+            for (int t = 0; t < 64 && l0 + t <= a.maxlmax; t++) {
+                const int ll = l0 + t;
+                if (ismall > 20) {
+                    if (llog > ll) llog = ll;
+                    nl = ll;
+                    stop = 1;
+                    break;
+                }
+                for (int m = -2; m <= 2; m++) {
+                    if (abs(m) > ll) continue;
+                    const double am = (m == 0) ? amp0[t] : 0.0;
+                    if (am > maxamp) maxamp = am;
+                    const double ratio = (am != 0.0 && maxamp != 0.0) ? am / maxamp : 0.0;
+                    if (ratio < a.ratl && ll >= lsuf) ismall++;
+                    else ismall = 0;
+                }
+            }
         }
+        __syncthreads();
+        if (stop) break;
     }
-    a.nl[k] = nl;
-    a.llog[k] = llog;
+    if (tx == 0) {
+        a.nl[k] = nl;
+        a.llog[k] = llog;
+    }
 }
 
 void launch_psv_cut(const PsvCutArgs &a, hipStream_t s)
 {
-    hipLaunchKernelGGL(psv_cut_kernel, dim3((a.nslots + 63) / 64), dim3(64), 0, s, a);
+    hipLaunchKernelGGL(psv_cut_kernel, dim3(a.nslots), dim3(64), 0, s, a);
 }
 
 // l extent of every (receiver, slot) row of the bf16 synthesis: the l
@@ -515,9 +535,8 @@ __global__ __launch_bounds__(64) void psv_back_kernel(PsvSolveArgs a)
     if (k >= kbot) step(k, r0);
     if (k - 1 >= kbot) step(k - 1, r1);
     if (k - 2 >= kbot) step(k - 2, r2);
-    // recorded rows below the cut: zero
-    for (; p < a.nneed; p++)
-        for (int s = 0; s < PSV_NSOL; s++) xo[(size_t)(PSV_NSOL * p + s) * a.lcap] = cz();
+    // recorded rows below the cut are not written: the coefficient kernel
+    // reads them as zero (PsvCoefArgs::kstart)
 }
 
 void launch_psv_excite(const PsvSolveArgs &a, int nslots, hipStream_t s)

@@ -121,6 +121,11 @@ struct PsvCoefArgs {
     double2 *ampuv;        // [slot][lcap][2]
     B16Rows eb, esb;       // mixed forms (eb.Eb non-null): bf16 rows instead of E, Es
     const unsigned char *rneed;  // batched forms: [l block of 256][r] rows some synthesis tile reads (null: all)
+    // batched forms: first solved local row per (slot, l block) and the
+    // receivers' lowest stencil row (local); a receiver below its block's
+    // first row has zero solutions there (not stored by the back substitution)
+    const int *kstart, *rlow;
+    int nblk;
 };
 
 struct PsvL0Args {
@@ -838,8 +843,16 @@ __host__ __device__ inline void psv_coef_lane(const PsvCoefArgs &a, int b, int t
         }
     }
     double2 *E = a.E + (size_t)sl * PSV_NQ * R * LC + l;
+    const int kst = a.kstart ? a.kstart[b / a.nlb * a.nblk + ((l - a.l0) >> 8)] : 1;
     for (int r = 0; r < R; r++) {
         if (rneed && !rneed[r]) continue;  // below the solved rows of every slot the tile holds
+        if (a.kstart && kst > a.rlow[r]) {  // below this block's solved rows: zero rows
+            if (a.eb.Eb)
+                for (int q = 0; q < PSV_NQ; q++) a.eb.put(q, r, sl, l, cz());
+            else
+                for (int q = 0; q < PSV_NQ; q++) E[(size_t)(q * R + r) * LC] = cz();
+            continue;
+        }
         const int *rr = a.rec_row + 6 * r;
         const double *w0 = a.w0 + 3 * r, *w1 = a.w1 + 3 * r;
         PsvVals z[PSV_NSOL];

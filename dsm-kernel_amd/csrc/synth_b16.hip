@@ -152,11 +152,14 @@ __global__ __launch_bounds__(512) void synth_b16_kernel(SynthB16Args a, int nx, 
 {
     __shared__ __attribute__((aligned(16))) u16 As[2][NS * BM * LDSW];
     __shared__ __attribute__((aligned(16))) u16 Bs[2][NS * BN * LDSW];
-    int tx, ty, tz;
+    int tx, ty, tz, tile = blockIdx.x;
     if (a.tiles) {
-        // host order: longest l series first, dealt round-robin over the XCDs
+        // host order: groups of tiles that share operands on one XCD,
+        // longest l series first; -1 marks an empty slot
         if ((int)blockIdx.x >= a.ntiles) return;
         const int t = a.tiles[blockIdx.x];
+        if (t < 0) return;
+        tile = t;
         tx = t % nx;
         ty = (t / nx) % ny;
         tz = t / (nx * ny);
@@ -183,7 +186,8 @@ __global__ __launch_bounds__(512) void synth_b16_kernel(SynthB16Args a, int nx, 
     for (int i = 0; i < 4; i++)
 #pragma unroll
         for (int j = 0; j < 2; j++) acc64[i][j] = (v4d){0.0, 0.0, 0.0, 0.0};
-    double *stash = a.stash + (((size_t)blockIdx.x * 8 + wave) * 32) * 64 + lane;
+    // stash of tile t (< nx ny njobs, within synth_b16_grid)
+    double *stash = a.stash + (((size_t)tile * 8 + wave) * 32) * 64 + lane;
     bool stashed = false;
 
     auto itmap = [&](int it, int &jb, int &lp0) {
