@@ -1434,24 +1434,32 @@ static int fetch_common(dsm_sgt_t *h, int32_t i, int32_t kind, int32_t ir, std::
     if (pos < 0 || pos >= h->npos || !h->done[pos])
         return fail(h, DSM_SGT_ESTATE, "frequency " + std::to_string(i) + " not computed");
     const int R = h->r_n, T = h->theta_n;
+    // device catalogs: [pos][comp][r][theta] (r only for TSGT, RSGT)
     const double2 *src;
+    int rstride = R;
     if (kind == DSM_SGT_TSGT) {
         num = h->ncomp[0];
         if (ir < 0 || ir >= R) return fail(h, DSM_SGT_EINVAL, "receiver index out of range");
-        src = h->d_cat_t.as<double2>() + ((size_t)pos * R + ir) * T * num;
+        src = h->d_cat_t.as<double2>() + ((size_t)pos * num * R + ir) * T;
     } else if (kind == DSM_SGT_RSGT) {
         num = h->ncomp[1];
         if (ir < 0 || ir >= R) return fail(h, DSM_SGT_EINVAL, "receiver index out of range");
-        src = h->d_cat_r.as<double2>() + ((size_t)pos * R + ir) * T * num;
+        src = h->d_cat_r.as<double2>() + ((size_t)pos * num * R + ir) * T;
     } else if (kind == DSM_SGT_SYNN) {
         num = h->ncomp[2];
-        src = h->d_cat_s.as<double2>() + (size_t)pos * T * num;
+        rstride = 1;
+        src = h->d_cat_s.as<double2>() + (size_t)pos * num * T;
     } else {
         return fail(h, DSM_SGT_EINVAL, "unknown catalog kind");
     }
-    buf.resize((size_t)num * T);
+    std::vector<double2> cm((size_t)num * T);
     if (hipSetDevice(h->device) != hipSuccess) return fail(h, DSM_SGT_EHIP, "hipSetDevice failed");
-    HIPCHK(h, hipMemcpy(buf.data(), src, sizeof(double2) * buf.size(), hipMemcpyDeviceToHost));
+    HIPCHK(h, hipMemcpy2D(cm.data(), sizeof(double2) * T, src, sizeof(double2) * T * rstride, sizeof(double2) * T, num,
+                          hipMemcpyDeviceToHost));
+    // records per distance: [theta][comp]
+    buf.resize((size_t)num * T);
+    for (int c = 0; c < num; c++)
+        for (int t = 0; t < T; t++) buf[(size_t)t * num + c] = cm[(size_t)c * T + t];
     return DSM_SGT_OK;
 }
 

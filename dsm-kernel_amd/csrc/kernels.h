@@ -58,7 +58,7 @@ struct SynthJob {
     double sign[2];        // +-1
     int accumulate;        // add into the targets instead of writing them
     int ncomp;             // record width
-    double2 *out;          // catalog of this kind: [pos][r_eff][theta_n][ncomp]
+    double2 *out;          // catalog of this kind: [pos][ncomp][r_eff][theta_n]
     double msign;          // F(l, pi - theta) = msign (-1)^l F(l, theta) for all the job's functions
 };
 struct SynthArgs {

@@ -146,7 +146,7 @@ __global__ __launch_bounds__(512) void synth_f64_kernel(SynthArgs a, int nx, int
         if (it + 1 < niter) store_tile(As[st ^ 1], Bs[st ^ 1], tid, tl);
         __syncthreads();
     }
-    // epilogue: row = (slot, r, re|im) -> out[pos][r][theta][comp].
+    // epilogue: row = (slot, r, re|im) -> out[pos][comp][r][theta].
     // v_mfma_f64_16x16x4f64 leaves D[i][j] in lane j + 16*(i%4), register i/4.
     double *out = (double *)job.out;
     const int ntgt = job.comp[1] >= 0 ? 2 : 1;
@@ -158,13 +158,13 @@ __global__ __launch_bounds__(512) void synth_f64_kernel(SynthArgs a, int nx, int
             if (row >= M) continue;
             const int pr = row >> 1, reim = row & 1;
             const int sl = pr / R, r = pr - sl * R;
-            const size_t base = ((size_t)a.slot_out[sl] * R + r) * a.theta_n;
+            const size_t pos = a.slot_out[sl];
 #pragma unroll
             for (int j = 0; j < 2; j++) {
                 const int col = n0 + wc * 32 + j * 16 + (lane & 15);
                 if (col >= a.theta_n) continue;
                 for (int t = 0; t < ntgt; t++) {
-                    double *o = out + 2 * ((base + col) * job.ncomp + job.comp[t]) + reim;
+                    double *o = out + 2 * (((pos * job.ncomp + job.comp[t]) * R + r) * a.theta_n + col) + reim;
                     const double v = job.sign[t] * acc[i][j][q];
                     *o = job.accumulate ? *o + v : v;
                 }

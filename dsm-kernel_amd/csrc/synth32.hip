@@ -150,25 +150,27 @@ __global__ __launch_bounds__(512) void synth_f32_kernel(SynthArgs a, const float
         __syncthreads();
     }
     // epilogue: v_mfma_f32_16x16x4f32 leaves D[i][j] in lane j + 16*(i/4), register i%4
-    double *out = (double *)job.out;
+    // out[pos][comp][r][theta] (complex): registers 2h, 2h+1 are re, im of
+    // one (slot, r) row, 16 lanes write 16 consecutive distances
+    double2 *out = job.out;
     const int ntgt = job.comp[1] >= 0 ? 2 : 1;
 #pragma unroll
     for (int i = 0; i < 4; i++)
 #pragma unroll
-        for (int q = 0; q < 4; q++) {
-            const int row = m0 + wr * 64 + i * 16 + 4 * (lane >> 4) + q;
+        for (int h2 = 0; h2 < 2; h2++) {
+            const int row = m0 + wr * 64 + i * 16 + 4 * (lane >> 4) + 2 * h2;
             if (row >= M) continue;
-            const int pr = row >> 1, reim = row & 1;
+            const int pr = row >> 1;
             const int sl = pr / R, r = pr - sl * R;
-            const size_t base = ((size_t)a.slot_out[sl] * R + r) * a.theta_n;
+            const size_t pos = a.slot_out[sl];
 #pragma unroll
             for (int j = 0; j < 2; j++) {
                 const int col = n0 + wc * 32 + j * 16 + (lane & 15);
                 if (col >= a.theta_n) continue;
                 for (int t = 0; t < ntgt; t++) {
-                    double *o = out + 2 * ((base + col) * job.ncomp + job.comp[t]) + reim;
-                    const double v = job.sign[t] * acc64[i][j][q];
-                    *o = job.accumulate ? *o + v : v;
+                    double2 *o = out + ((pos * job.ncomp + job.comp[t]) * R + r) * a.theta_n + col;
+                    const double2 v = make_double2(job.sign[t] * acc64[i][j][2 * h2], job.sign[t] * acc64[i][j][2 * h2 + 1]);
+                    *o = job.accumulate ? make_double2(o->x + v.x, o->y + v.y) : v;
                 }
             }
         }

@@ -264,42 +264,44 @@ __global__ __launch_bounds__(512) void synth_b16_kernel(SynthB16Args a, int nx, 
         if (it + 1 < nit) store_stage<NS>(As[st ^ 1], Bs[st ^ 1], tid, stg);
         __syncthreads();
     }
-    // epilogue: v_mfma_f32_16x16x32_bf16 leaves D[i][j] in lane j + 16*(i/4), register i%4
-    double *out = (double *)job->out;
+    // epilogue: v_mfma_f32_16x16x32_bf16 leaves D[i][j] in lane j + 16*(i/4),
+    // register i%4: registers 2h, 2h+1 are re, im of one (receiver, slot)
+    // row, written as one complex to out[pos][comp][r][theta], 16 lanes on
+    // 16 consecutive distances (the mirror columns likewise, descending)
+    double2 *out = job->out;
     const int ncomp = job->ncomp, accum = job->accumulate;
     const int comp0 = job->comp[0], comp1 = job->comp[1];
     const double sign0 = job->sign[0], sign1 = job->sign[1], msign = job->msign;
 #pragma unroll
     for (int i = 0; i < 4; i++)
 #pragma unroll
-        for (int q = 0; q < 4; q++) {
-            const int row = m0 + wr * 64 + i * 16 + 4 * (lane >> 4) + q;
+        for (int h2 = 0; h2 < 2; h2++) {
+            const int row = m0 + wr * 64 + i * 16 + 4 * (lane >> 4) + 2 * h2;  // even; M is even
             if (row >= M) continue;
-            const int pr = row >> 1, reim = row & 1;
+            const int pr = row >> 1;
             const int r = pr / a.nslot, sl = pr - r * a.nslot;
-            const size_t base = ((size_t)a.slot_out[sl] * R + r) * a.theta_n;
+            const size_t pos = a.slot_out[sl];
 #pragma unroll
             for (int j = 0; j < 2; j++) {
                 const int col = n0 + wc * 32 + j * 16 + fr;
                 if (col >= a.nh) continue;
-                const double o = acc64[i][j][q];
-                const double e = stashed ? stash[(size_t)((i * 2 + j) * 4 + q) * 64] : 0.0;
-                const double dv = e + o;
+                const int q0 = 2 * h2, q1 = q0 + 1;
+                const double o0 = acc64[i][j][q0], o1 = acc64[i][j][q1];
+                const double e0 = stashed ? stash[(size_t)((i * 2 + j) * 4 + q0) * 64] : 0.0;
+                const double e1 = stashed ? stash[(size_t)((i * 2 + j) * 4 + q1) * 64] : 0.0;
                 const int mc = a.theta_n - 1 - col;
                 const bool mir = a.mirror && mc != col;
-                const double mv = msign * (e - o);
 #pragma unroll
                 for (int t = 0; t < 2; t++) {
                     const int comp = t ? comp1 : comp0;
                     if (comp < 0) continue;
                     const double sg = t ? sign1 : sign0;
-                    double *od = out + 2 * ((base + col) * ncomp + comp) + reim;
-                    const double v = sg * dv;
-                    *od = accum ? *od + v : v;
+                    double2 *od = out + ((pos * ncomp + comp) * R + r) * a.theta_n;
+                    const double2 v = make_double2(sg * (e0 + o0), sg * (e1 + o1));
+                    od[col] = accum ? make_double2(od[col].x + v.x, od[col].y + v.y) : v;
                     if (mir) {
-                        double *om = out + 2 * ((base + mc) * ncomp + comp) + reim;
-                        const double w = sg * mv;
-                        *om = accum ? *om + w : w;
+                        const double2 w = make_double2(sg * msign * (e0 - o0), sg * msign * (e1 - o1));
+                        od[mc] = accum ? make_double2(od[mc].x + w.x, od[mc].y + w.y) : w;
                     }
                 }
             }
