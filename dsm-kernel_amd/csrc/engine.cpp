@@ -385,6 +385,9 @@ static int alloc_b16(dsm_sgt_t *h, int nslot, int nq, int nqs, int R)
 {
     if (!h->ns) return DSM_SGT_OK;
     const size_t M = (size_t)nslot * R * 2, Ms = (size_t)nslot * 2;
+    // the synthesis kernel addresses a split's rows with 32-bit offsets
+    if (M * h->lcap >= (size_t)1 << 32)
+        return fail(h, DSM_SGT_EUNSUPPORTED, "too many receivers x frequencies per batch for the bf16 synthesis");
     HIPCHK(h, h->d_eb.ensure(sizeof(unsigned short) * nq * h->ns * M * h->lcap));
     HIPCHK(h, h->d_esb.ensure(sizeof(unsigned short) * nqs * h->ns * Ms * h->lcap));
     // degrees above maxlmax are never written: zero once

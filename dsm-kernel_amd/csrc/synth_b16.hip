@@ -32,9 +32,13 @@
 // longest series among its rows.
 //
 // Tile: 128 x 128 per 512-thread workgroup, 8 waves in 2 x 4, each wave
-// 4 x 2 blocks of 16 x 16; operands staged through LDS with a register-
-// staged double buffer, rows padded to 80 bytes (conflict-free 16-byte
-// fragment reads).
+// 4 x 2 blocks of 16 x 16.  Operands go from global memory straight into
+// LDS (global_load_lds_dwordx4: a wave writes 1 KiB, 16 rows of 32
+// degrees) in a ring of three stages, two K steps ahead of the MFMAs; a
+// wave waits for its own loads of the current stage (s_waitcnt vmcnt) and
+// the barrier makes every wave's loads visible.  Rows are 64 bytes, their
+// four 16-byte degree segments permuted per row (sw_seg), so the 16-byte
+// fragment reads of 8 and of 16 consecutive rows hit distinct banks.
 #include "dev_common.h"
 #include "kernels.h"
 
@@ -46,7 +50,22 @@ typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 typedef unsigned short u16;
 
 constexpr int BM = 128, BN = 128, BK = 32, NT = 512;
-constexpr int LDSW = 40;  // u16 per LDS row: 32 degrees + 8 pad
+constexpr int NSTAGE = 3;
+
+// degree segment (8 bf16) seg of tile row `row` sits at LDS segment
+// seg ^ sw_seg(row)
+__device__ __forceinline__ int sw_seg(int row) { return ((row >> 1) ^ (row >> 3)) & 3; }
+
+#define LDS_PTR(x) ((__attribute__((address_space(3))) void *)(x))
+
+// s_waitcnt: vmcnt <= N (the loads of later stages stay in flight), all LDS
+// reads returned (lgkmcnt 0), expcnt not waited (gfx9 encoding)
+template <int N>
+__device__ __forceinline__ void wait_vm_lgkm0()
+{
+    static_assert(N >= 0 && N < 16, "vmcnt field");
+    __builtin_amdgcn_s_waitcnt(N | (0x7 << 4));
+}
 
 // XCD-sliced tile order, as in synth.hip
 __device__ __forceinline__ bool xcd_tile_b16(int nx, int ny, int &tx, int &ty, int &tz)
@@ -61,11 +80,6 @@ __device__ __forceinline__ bool xcd_tile_b16(int nx, int ny, int &tx, int &ty, i
     tx = t - ty * nx;
     return true;
 }
-
-template <int NS>
-struct StageB16 {
-    uint4 a[NS], b[NS];
-};
 
 }  // namespace
 
@@ -118,40 +132,31 @@ __global__ void split_table_kernel(const double *tab, int nplanes, int lcap, int
     }
 }
 
-// job: the workgroup's job in global memory (read with uniform loads; a
-// private copy indexed by jb would live in scratch)
+// One K step of the tile into LDS stage `dst` (u16 offset base): 2 NS 8
+// groups of 16 rows (A: splits of the coefficient rows, then B: splits of
+// the table columns), group g by wave g % 8, 1 KiB per group.  loff[u]:
+// the lane's row offset (degree segment included) in group wave + 8 u;
+// ea, tb: the job's coefficient-row and table-plane bases at degree lp0.
+// Nothing here may wait on a vector load: the loads of the later stages
+// are in flight.
 template <int NS>
-__device__ __forceinline__ void load_stage(const SynthB16Args &a, const SynthJob *job, int M, int m0, int n0, int jb,
-                                           int lp0, int tid, StageB16<NS> &st)
+__device__ __forceinline__ void issue_stage(const u16 *ea, const u16 *tb, size_t a_split, size_t t_split,
+                                            const unsigned *loff, int wave, u16 *dst)
 {
-    const int row = tid >> 2, seg = tid & 3;
-    const bool okr = (m0 + row) < M;
-    const int plane = job->b[jb];
 #pragma unroll
-    for (int s = 0; s < NS; s++) {
-        const u16 *pa = a.Eb + (((size_t)(job->q0 + jb) * NS + s) * M + m0 + row) * a.lcap + lp0 + 8 * seg;
-        st.a[s] = okr ? *(const uint4 *)pa : make_uint4(0, 0, 0, 0);
-        const u16 *pb = a.Tb + (((size_t)s * a.nplanes + plane) * a.nhp + n0 + row) * a.lcap + lp0 + 8 * seg;
-        st.b[s] = *(const uint4 *)pb;
-    }
-}
-
-template <int NS>
-__device__ __forceinline__ void store_stage(u16 *As, u16 *Bs, int tid, const StageB16<NS> &st)
-{
-    const int row = tid >> 2, seg = tid & 3;
-#pragma unroll
-    for (int s = 0; s < NS; s++) {
-        *(uint4 *)&As[(s * BM + row) * LDSW + 8 * seg] = st.a[s];
-        *(uint4 *)&Bs[(s * BN + row) * LDSW + 8 * seg] = st.b[s];
+    for (int u = 0; u < NS * 2; u++) {
+        const int g = wave + 8 * u;
+        const int s = (g >> 3) % NS;
+        const u16 *src = (g < NS * 8) ? ea + s * a_split : tb + s * t_split;
+        __builtin_amdgcn_global_load_lds((const void *)(src + loff[u]), LDS_PTR(dst + g * 512), 16, 0, 0);
     }
 }
 
 template <int NS, int KIND>  // KIND 0: receiver rows, 1: surface rows (separate symbol for profiling)
 __global__ __launch_bounds__(512) void synth_b16_kernel(SynthB16Args a, int nx, int ny)
 {
-    __shared__ __attribute__((aligned(16))) u16 As[2][NS * BM * LDSW];
-    __shared__ __attribute__((aligned(16))) u16 Bs[2][NS * BN * LDSW];
+    constexpr int STAGE = NS * (BM + BN) * BK;  // u16
+    __shared__ __attribute__((aligned(1024))) u16 lds[NSTAGE * STAGE];
     int tx, ty, tz, tile = blockIdx.x;
     if (a.tiles) {
         // host order: groups of tiles that share operands on one XCD,
@@ -168,7 +173,7 @@ __global__ __launch_bounds__(512) void synth_b16_kernel(SynthB16Args a, int nx, 
     }
     const SynthJob *job = a.jobs + tz;
     const int nb = job->nb;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int R = a.r_eff;
     const int M = a.nslot * R * 2;
     const int m0 = ty * BM, n0 = tx * BN;
@@ -179,7 +184,7 @@ __global__ __launch_bounds__(512) void synth_b16_kernel(SynthB16Args a, int nx, 
     const int LH = a.lcap >> 1;
     const int nit_e = nb * ce, nit = nit_e + nb * co;
     const int wr = wave >> 2, wc = wave & 3;
-    const int fr = lane & 15, fk = 8 * (lane >> 4);
+    const int fr = lane & 15;
 
     v4d acc64[4][2];
 #pragma unroll
@@ -201,30 +206,54 @@ __global__ __launch_bounds__(512) void synth_b16_kernel(SynthB16Args a, int nx, 
         }
     };
 
-    StageB16<NS> stg;
-    if (nit > 0) {
-        int jb, lp0;
-        itmap(0, jb, lp0);
-        load_stage<NS>(a, job, M, m0, n0, jb, lp0, tid, stg);
-        store_stage<NS>(As[0], Bs[0], tid, stg);
+    // loads: the job's fields in scalar registers, the lanes' row offsets
+    // (rows past M read row M - 1: their products land in rows not stored)
+    const int q0 = __builtin_amdgcn_readfirstlane(job->q0);
+    const int pl0 = __builtin_amdgcn_readfirstlane(job->b[0]), pl1 = __builtin_amdgcn_readfirstlane(job->b[1]);
+    const int pl2 = __builtin_amdgcn_readfirstlane(job->b[2]), pl3 = __builtin_amdgcn_readfirstlane(job->b[3]);
+    unsigned loff[2 * NS];
+#pragma unroll
+    for (int u = 0; u < 2 * NS; u++) {
+        const int g = wave + 8 * u;
+        const int row = (g & 7) * 16 + (lane >> 2);
+        const int seg = (lane & 3) ^ sw_seg(row);
+        const int src_row = (g < NS * 8) ? min(m0 + row, M - 1) : n0 + row;
+        loff[u] = (unsigned)src_row * (unsigned)a.lcap + 8 * seg;
     }
-    __syncthreads();
+    const size_t a_split = (size_t)M * a.lcap, t_split = (size_t)a.nplanes * a.nhp * a.lcap;
+    auto issue = [&](int it2) {
+        int jb, lp0;
+        itmap(it2, jb, lp0);
+        const int plane = jb == 0 ? pl0 : jb == 1 ? pl1 : jb == 2 ? pl2 : pl3;
+        const u16 *ea = a.Eb + (size_t)(q0 + jb) * NS * a_split + lp0;
+        const u16 *tb = a.Tb + (size_t)plane * a.nhp * a.lcap + lp0;
+        issue_stage<NS>(ea, tb, a_split, t_split, loff, wave, lds + (it2 % NSTAGE) * STAGE);
+    };
+    if (nit > 0) issue(0);
+    if (nit > 1) issue(1);
     for (int it = 0; it < nit; it++) {
-        const int st = it & 1;
-        if (it + 1 < nit) {
-            int jb, lp0;
-            itmap(it + 1, jb, lp0);
-            load_stage<NS>(a, job, M, m0, n0, jb, lp0, tid, stg);
-        }
+        // this stage's loads landed (each wave its own 2 NS), and every wave's
+        // fragment reads of the stage refilled next are done
+        if (it + 1 < nit) wait_vm_lgkm0<2 * NS>();
+        else wait_vm_lgkm0<0>();
+        asm volatile("" ::: "memory");
+        __builtin_amdgcn_s_barrier();
+        asm volatile("" ::: "memory");
+        if (it + 2 < nit) issue(it + 2);
+        const u16 *As = lds + (it % NSTAGE) * STAGE, *Bs = As + NS * BM * BK;
         v8bf af[NS][4], bfg[NS][2];
 #pragma unroll
         for (int s = 0; s < NS; s++) {
 #pragma unroll
-            for (int i = 0; i < 4; i++)
-                af[s][i] = *(const v8bf *)&As[st][(s * BM + wr * 64 + i * 16 + fr) * LDSW + fk];
+            for (int i = 0; i < 4; i++) {
+                const int row = wr * 64 + i * 16 + fr;
+                af[s][i] = *(const v8bf *)&As[(s * BM + row) * BK + 8 * ((lane >> 4) ^ sw_seg(row))];
+            }
 #pragma unroll
-            for (int j = 0; j < 2; j++)
-                bfg[s][j] = *(const v8bf *)&Bs[st][(s * BN + wc * 32 + j * 16 + fr) * LDSW + fk];
+            for (int j = 0; j < 2; j++) {
+                const int row = wc * 32 + j * 16 + fr;
+                bfg[s][j] = *(const v8bf *)&Bs[(s * BN + row) * BK + 8 * ((lane >> 4) ^ sw_seg(row))];
+            }
         }
         v4f acc[4][2];
 #pragma unroll
@@ -261,8 +290,6 @@ __global__ __launch_bounds__(512) void synth_b16_kernel(SynthB16Args a, int nx, 
                     }
             stashed = true;
         }
-        if (it + 1 < nit) store_stage<NS>(As[st ^ 1], Bs[st ^ 1], tid, stg);
-        __syncthreads();
     }
     // epilogue: v_mfma_f32_16x16x32_bf16 leaves D[i][j] in lane j + 16*(i/4),
     // register i%4: registers 2h, 2h+1 are re, im of one (receiver, slot)
