@@ -359,16 +359,6 @@ static int b16_tile_order(dsm_sgt_t *h, const std::vector<int> &nlr, int S, int 
     };
     order(0, h->njobs_main, h->tiles_main);
     order(h->njobs_main, h->njobs_acc, h->tiles_acc);
-    // MFMA flops issued: per tile and K step of 32 degrees, 2*128*128*32
-    // per cross product
-    const double per = 2.0 * 128 * 128 * 32 * (h->ns == 3 ? 6 : 3);
-    double fl = 0.0;
-    for (int j = 0; j < h->njobs_main + h->njobs_acc; j++)
-        for (int ty = 0; ty < ny; ty++) {
-            const int ce = ((kt[ty] + 1) / 2 + 31) / 32, co = (kt[ty] / 2 + 31) / 32;
-            fl += per * nx * h->jobs[j].nb * (ce + co);
-        }
-    h->stats.synth_mfma_flops += fl;
     h->tiles_main_n = (int)h->tiles_main.size();
     h->tiles_acc_n = (int)h->tiles_acc.size();
     HIPCHK(h, h->d_tiles.ensure(sizeof(int) * (h->tiles_main.size() + h->tiles_acc.size())));
@@ -377,6 +367,25 @@ static int b16_tile_order(dsm_sgt_t *h, const std::vector<int> &nlr, int S, int 
     HIPCHK(h, hipMemcpyAsync(h->d_tiles.as<int>() + h->tiles_main.size(), h->tiles_acc.data(),
                              sizeof(int) * h->tiles_acc.size(), hipMemcpyHostToDevice, h->stream));
     return DSM_SGT_OK;
+}
+
+// MFMA flops the bf16 synthesis issued for the batch, from the rows' l
+// extents the kernels used (nlr, device values): per tile and K step of 32
+// degrees 2 x 128 x 128 x 32 per cross product.
+static double b16_issued_flops(const dsm_sgt_t *h, const std::vector<int> &nlr, int S, int R)
+{
+    int nx, ny;
+    synth_b16_tiles(S, R, h->nhp, nx, ny);
+    const int M = S * R * 2;
+    const double per = 2.0 * 128 * 128 * 32 * (h->ns == 3 ? 6 : 3);
+    double fl = 0.0;
+    for (int ty = 0; ty < ny; ty++) {
+        int kt = 0;
+        for (int pr = (ty * 128) >> 1; pr <= (std::min(M - 1, ty * 128 + 127) >> 1); pr++) kt = std::max(kt, nlr[pr]);
+        const int ce = ((kt + 1) / 2 + 31) / 32, co = (kt / 2 + 31) / 32;
+        for (int j = 0; j < h->njobs_main + h->njobs_acc; j++) fl += per * nx * h->jobs[j].nb * (ce + co);
+    }
+    return fl;
 }
 
 // Device buffers of the BF16 forms for nslot frequencies: split rows and
@@ -1293,6 +1302,12 @@ static int psv_run_batch(dsm_sgt_t *h, const PsvPrep &p)
     HIPCHK(h, hipGetLastError());
     ev_end(h, tg2);
     h->stats.synth_launches += 2;
+    std::vector<int> nlr_dev;
+    if (h->ns) {  // the l extents the synthesis used, for its issued-flop count
+        nlr_dev.resize((size_t)R * S);
+        HIPCHK(h, hipMemcpyAsync(nlr_dev.data(), h->d_nlr.p, sizeof(int) * nlr_dev.size(), hipMemcpyDeviceToHost,
+                                 h->stream));
+    }
     if (dev_cut) {  // the cut-off replayed on the device
         std::vector<int> llog(S);
         HIPCHK(h, hipMemcpyAsync(nl.data(), h->d_nl.p, sizeof(int) * S, hipMemcpyDeviceToHost, h->stream));
@@ -1308,6 +1323,7 @@ static int psv_run_batch(dsm_sgt_t *h, const PsvPrep &p)
     // 4 flops per (complex coefficient x real function) per (r,theta)
     h->stats.synth_flops += 4.0 * sumL * s.theta_n * (double)PSV_NQ * R;
     HIPCHK(h, hipStreamSynchronize(h->stream));
+    if (h->ns) h->stats.synth_mfma_flops += b16_issued_flops(h, nlr_dev, S, R);
     ev_collect(h);
     return DSM_SGT_OK;
 }

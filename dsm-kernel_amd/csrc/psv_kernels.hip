@@ -105,17 +105,21 @@ void launch_psv_loff(const int *kstart, int nblocks, int n, int klo, long long *
     hipLaunchKernelGGL(psv_loff_kernel, dim3(1), dim3(1024), 0, s, kstart, nblocks, n, klo, loff, total);
 }
 
-// One 64-lane workgroup per slot: the lanes form the amplitudes of 64
-// degrees at a time, lane 0 replays the count over them in order.
+// One 64-lane workgroup per slot, 64 degrees at a time.  In the count
+// (calamp, src/SGTsh/others.f90:805-824, over m = -2..2, |m| <= l; the
+// host form is calamp in engine.cpp) the entries m != 0 have amplitude 0:
+// they never raise the running maximum (which starts at -1 and so is
+// max(0, amplitudes so far)), their ratio is 0, so they count as small
+// exactly when 0 < ratl and l >= lsuf.  The lanes form their degree's amplitude,
+// the running maximum through it (wave prefix maximum) and the m = 0 test;
+// lane 0 replays the integer count from the 64-bit mask of those tests.
 __global__ __launch_bounds__(64) void psv_cut_kernel(PsvCutArgs a)
 {
-    __shared__ double amp0[64];
-    __shared__ int stop;
     const int k = blockIdx.x, tx = threadIdx.x;
     const int lsuf = a.lsuf[k];
     int ismall = 0, nl = a.maxlmax + 1, llog = a.maxlmax;
-    double maxamp = -1.0;
-    if (tx == 0) stop = 0;
+    const bool zero_small = 0.0 < a.ratl;  // a ratio of 0 counts as small
+    double carry = 0.0;  // running maximum over the degrees of earlier chunks (and the m = -1 entry of l = 1)
     for (int l0 = 1; l0 <= a.maxlmax; l0 += 64) {
         const int l = l0 + tx;
         double a0 = 0.0;
@@ -124,29 +128,33 @@ __global__ __launch_bounds__(64) void psv_cut_kernel(PsvCutArgs a)
             const double a1 = hypot(u.x, u.y), a2 = hypot(v.x, v.y);
             a0 = sqrt(a1 * a1 + a2 * a2);
         }
-        amp0[tx] = a0;
-        __syncthreads();
+        double pm = a0;  // inclusive prefix maximum over the chunk
+        for (int d = 1; d < 64; d <<= 1) {
+            const double o = __shfl_up(pm, d, 64);
+            if (tx >= d) pm = fmax(pm, o);
+        }
+        const double mx = fmax(carry, pm);
+        const double ratio = (a0 != 0.0 && mx != 0.0) ? a0 / mx : 0.0;
+        const unsigned long long small0 = __ballot(l <= a.maxlmax && ratio < a.ratl && l >= lsuf);
+        carry = fmax(carry, __shfl(pm, 63, 64));
+        int stop = 0;
         if (tx == 0) {
             for (int t = 0; t < 64 && l0 + t <= a.maxlmax; t++) {
                 const int ll = l0 + t;
                 if (ismall > 20) {
-                    if (llog > ll) llog = ll;
+                    llog = min(llog, ll);
                     nl = ll;
                     stop = 1;
                     break;
                 }
-                for (int m = -2; m <= 2; m++) {
-                    if (abs(m) > ll) continue;
-                    const double am = (m == 0) ? amp0[t] : 0.0;
-                    if (am > maxamp) maxamp = am;
-                    const double ratio = (am != 0.0 && maxamp != 0.0) ? am / maxamp : 0.0;
-                    if (ratio < a.ratl && ll >= lsuf) ismall++;
-                    else ismall = 0;
-                }
+                const bool s = zero_small && ll >= lsuf, f0 = (small0 >> t) & 1ull;
+                const int side = (ll >= 2) ? 2 : 1;  // entries m < 0 (and m > 0)
+                ismall = s ? ismall + side : 0;
+                ismall = f0 ? ismall + 1 : 0;
+                ismall = s ? ismall + side : 0;
             }
         }
-        __syncthreads();
-        if (stop) break;
+        if (__shfl(stop, 0, 64)) break;
     }
     if (tx == 0) {
         a.nl[k] = nl;
