@@ -140,7 +140,9 @@ def main():
         eng = dsm_sgt.Engine(inp, mode=mode, precision=pr)
         t1 = time.perf_counter()
         eng.run(1, p["imax"])  # returns with the device synchronised
-        detail.append({"create_s": t1 - t0, "run_s": time.perf_counter() - t1})
+        st = eng.stats()
+        detail.append({"create_s": t1 - t0, "run_s": time.perf_counter() - t1,
+                       "prep_wait_s": st.prep_wait_ms * 1e-3, "device_wait_s": st.device_wait_ms * 1e-3})
         return eng
 
     for _ in range(args.warmup):

@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <complex>
 #include <cstdio>
@@ -123,13 +124,13 @@ struct TimedLaunch {
 
 }  // namespace
 
-// Host part of one P-SV batch (per-frequency set-up of SGTpsv-MPI.f90:
-// 566-625 for every slot), prepared on host threads while the previous
-// batch runs on the device.
+// Host part of one P-SV batch: the per-frequency scalars of
+// SGTpsv-MPI.f90:566-625 for every slot (the band pieces themselves are
+// assembled on the device, psv_band.hip), prepared on a host thread while
+// the previous batch runs on the device.
 struct PsvPrep {
     std::vector<int> freqs;
-    double2 *band = nullptr;  // pinned [slot][nn][16]
-    size_t band_cap = 0;      // in double2
+    std::vector<double2> scal;  // [slot][1 + 3 nzone]: the band assembly's scalars (PsvBandArgs::scal)
     std::vector<double2> fk;
     std::vector<PsvSrc> src;
     std::vector<double> omg;
@@ -164,6 +165,7 @@ struct dsm_sgt {
     DevBuf d_tb, d_eb, d_esb, d_stash, d_nlr, d_tiles;
     std::vector<int> tiles_main, tiles_acc;  // host copies of the tile orders (launch sizes)
     int tiles_main_n = 0, tiles_acc_n = 0;
+    DevBuf d_bfe, d_bzbase, d_bzsolid, d_bznl, d_bzkkdr, d_bcolz, d_bovrii, d_bovr, d_bscal;  // device band assembly
     DevBuf d_band, d_om, d_slots, d_nl, d_slotout, d_scratch, d_xn, d_E, d_Es, d_amp, d_jobs;
     DevBuf d_cat_t, d_cat_r, d_cat_s;
     // P-SV only
@@ -211,6 +213,22 @@ static void ev_end(dsm_sgt_t *h, TimedLaunch &t)
     (void)hipEventRecord(t.b, h->stream);
     h->pending.push_back(t);
 }
+// Host wall time since t0, in ms.
+static double ms_since(std::chrono::steady_clock::time_point t0)
+{
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
+// hipStreamSynchronize of the engine stream inside a batch, its wait
+// counted in stats.device_wait_ms.
+static hipError_t dev_sync(dsm_sgt_t *h)
+{
+    const auto t0 = std::chrono::steady_clock::now();
+    const hipError_t e = hipStreamSynchronize(h->stream);
+    h->stats.device_wait_ms += ms_since(t0);
+    return e;
+}
+
 static void ev_collect(dsm_sgt_t *h)
 {
     for (auto &t : h->pending) {
@@ -846,6 +864,34 @@ static int psv_create(dsm_sgt_t *h)
     HIPCHK(h, hipMemcpy(h->d_jsta0.p, s.jsta0.data(), sizeof(int) * R, hipMemcpyHostToDevice));
 
     HIPCHK(h, h->d_band.ensure(sizeof(double2) * 16 * (size_t)nn * S));
+    {
+        // resident tables of the device band assembly
+        PsvBandTables bt;
+        psv_band_tables(s, bt);
+        for (int j = 0; j < nn; j++) {
+            int z = 0;
+            for (int iz = 0; iz < s.nzone; iz++) {
+                const int ncol = s.fe[iz].solid ? 2 * s.fe[iz].nlayer + 2 : s.fe[iz].nlayer + 1;
+                if (j + 1 >= s.kkdr[iz] && j + 1 < s.kkdr[iz] + ncol) z++;
+            }
+            if (z > 2) return fail(h, DSM_SGT_EUNSUPPORTED, "a band column shared by more than two zones");
+        }
+        auto up = [&](DevBuf &d, const void *src, size_t bytes) -> int {
+            HIPCHK(h, d.ensure(std::max<size_t>(bytes, 16)));
+            HIPCHK(h, hipMemcpy(d.p, src, bytes, hipMemcpyHostToDevice));
+            return DSM_SGT_OK;
+        };
+        int rc2 = up(h->d_bfe, bt.fe.data(), sizeof(double) * bt.fe.size());
+        if (!rc2) rc2 = up(h->d_bzbase, bt.zbase.data(), sizeof(long long) * bt.zbase.size());
+        if (!rc2) rc2 = up(h->d_bzsolid, bt.zsolid.data(), sizeof(int) * bt.zsolid.size());
+        if (!rc2) rc2 = up(h->d_bznl, bt.znl.data(), sizeof(int) * bt.znl.size());
+        if (!rc2) rc2 = up(h->d_bzkkdr, bt.zkkdr.data(), sizeof(int) * bt.zkkdr.size());
+        if (!rc2) rc2 = up(h->d_bcolz, bt.colz.data(), sizeof(int) * bt.colz.size());
+        if (!rc2) rc2 = up(h->d_bovrii, bt.ovr_ii.data(), sizeof(int) * bt.ovr_ii.size());
+        if (!rc2) rc2 = up(h->d_bovr, bt.ovr.data(), sizeof(double) * bt.ovr.size());
+        if (rc2) return rc2;
+        HIPCHK(h, h->d_bscal.ensure(sizeof(double2) * (size_t)S * (1 + 3 * s.nzone)));
+    }
     HIPCHK(h, h->d_src.ensure(sizeof(PsvSrc) * S));
     HIPCHK(h, h->d_fluidK.ensure(sizeof(double2) * (size_t)S * R));
     HIPCHK(h, h->d_slots.ensure(sizeof(int) * S));
@@ -934,49 +980,61 @@ static int psv_create(dsm_sgt_t *h)
 // Host set-up of the frequencies `freqs` into `p`, split over host threads
 // (psv_omega_setup only reads the set-up).  Returns false if the pinned
 // buffer could not be allocated.
-static bool psv_prepare(const PsvSetup &s, const std::vector<int> &freqs, PsvPrep &p)
+static void psv_prepare(const PsvSetup &s, const std::vector<int> &freqs, PsvPrep &p)
 {
-    const int S = (int)freqs.size(), nn = s.nn, R = s.r_n;
-    const size_t need = 16 * (size_t)nn * S;
-    if (p.band_cap < need) {
-        if (p.band) (void)hipHostFree(p.band);
-        p.band = nullptr;
-        p.band_cap = 0;
-        if (hipHostMalloc((void **)&p.band, sizeof(double2) * need, hipHostMallocPortable) != hipSuccess) return false;
-        p.band_cap = need;
-    }
+    const int S = (int)freqs.size(), R = s.r_n, nz = s.nzone;
     p.freqs = freqs;
+    p.scal.assign((size_t)S * (1 + 3 * nz), cz());
     p.fk.assign((size_t)S * R, cz());
     p.src.assign(S, PsvSrc{});
     p.omg.assign(S, 0.0);
     p.lsuf.assign(S, 0);
-    auto work = [&](int k0, int k1) {
-        for (int k = k0; k < k1; k++) {
-            PsvOmega o;
-            psv_omega_setup(s, freqs[k], o);
-            p.omg[k] = o.omega;
-            psv_pack_band(s, o, (double *)(p.band + 16 * (size_t)nn * k));
-            for (int q = 0; q < 160; q++) {
-                p.src[k].anum[q] = make_double2(o.anum[q].real(), o.anum[q].imag());
-                p.src[k].bnum[q] = make_double2(o.bnum[q].real(), o.bnum[q].imag());
-            }
-            p.src[k].coef1 = make_double2(o.coef1_spn.real(), o.coef1_spn.imag());
-            p.src[k].coef2 = make_double2(o.coef2_spn.real(), o.coef2_spn.imag());
-            for (int r = 0; r < R; r++) p.fk[(size_t)k * R + r] = make_double2(o.fluidK[r].real(), o.fluidK[r].imag());
-            p.lsuf[k] = o.lsuf;
+    auto d2 = [](const cd &c) { return make_double2(c.real(), c.imag()); };
+    for (int k = 0; k < S; k++) {
+        PsvOmega o;
+        psv_omega_scalars(s, freqs[k], o);
+        p.omg[k] = o.omega;
+        double2 *sc = &p.scal[(size_t)k * (1 + 3 * nz)];
+        sc[0] = d2(o.comega2);
+        for (int iz = 0; iz < nz; iz++) {
+            sc[1 + 3 * iz] = d2(o.zc1[iz]);
+            sc[2 + 3 * iz] = d2(o.zc2[iz]);
+            sc[3 + 3 * iz] = d2(o.zcf[iz]);
         }
-    };
-    // one thread by default: the set-up already runs beside the device
-    // work of the previous batch, and more threads have slowed the whole
-    // run on hosts with a CPU quota (DSM_SGT_HOST_THREADS overrides)
-    int nt = 1;
-    if (const char *e = std::getenv("DSM_SGT_HOST_THREADS")) nt = std::atoi(e);
-    nt = std::max(1, std::min(nt, S));
-    std::vector<std::thread> pool;
-    for (int t = 1; t < nt; t++) pool.emplace_back(work, S * t / nt, S * (t + 1) / nt);
-    work(0, S / nt);
-    for (auto &th : pool) th.join();
-    return true;
+        for (int q = 0; q < 160; q++) {
+            p.src[k].anum[q] = d2(o.anum[q]);
+            p.src[k].bnum[q] = d2(o.bnum[q]);
+        }
+        p.src[k].coef1 = d2(o.coef1_spn);
+        p.src[k].coef2 = d2(o.coef2_spn);
+        for (int r = 0; r < R; r++) p.fk[(size_t)k * R + r] = d2(o.fluidK[r]);
+        p.lsuf[k] = o.lsuf;
+    }
+}
+
+// Band pieces of S frequencies into d_band slots slot0.. (psv_band_kernel).
+static int psv_device_band(dsm_sgt_t *h, const std::vector<double2> &scal, int S, int slot0)
+{
+    const PsvSetup &s = h->psv;
+    HIPCHK(h, hipMemcpyAsync(h->d_bscal.p, scal.data(), sizeof(double2) * scal.size(), hipMemcpyHostToDevice,
+                             h->stream));
+    PsvBandArgs ba{};
+    ba.fe = h->d_bfe.as<double>();
+    ba.zbase = h->d_bzbase.as<long long>();
+    ba.zsolid = h->d_bzsolid.as<int>();
+    ba.znl = h->d_bznl.as<int>();
+    ba.zkkdr = h->d_bzkkdr.as<int>();
+    ba.colz = h->d_bcolz.as<int>();
+    ba.ovr_ii = h->d_bovrii.as<int>();
+    ba.ovr = h->d_bovr.as<double2>();
+    ba.nz = s.nzone;
+    ba.nn = s.nn;
+    ba.scal = h->d_bscal.as<double2>();
+    ba.slot0 = slot0;
+    ba.band = h->d_band.as<double2>();
+    launch_psv_band(ba, S, h->stream);
+    HIPCHK(h, hipGetLastError());
+    return DSM_SGT_OK;
 }
 
 static int psv_run_batch(dsm_sgt_t *h, const PsvPrep &p)
@@ -995,8 +1053,10 @@ static int psv_run_batch(dsm_sgt_t *h, const PsvPrep &p)
         cut[k].llog = s.maxlmax;
         slotout[k] = freqs[k] - h->ipos0;
     }
-    HIPCHK(h, hipMemcpyAsync(h->d_band.p, p.band, sizeof(double2) * 16 * (size_t)nn * S, hipMemcpyHostToDevice,
-                             h->stream));
+    {
+        const int rc = psv_device_band(h, p.scal, S, 0);
+        if (rc) return rc;
+    }
     HIPCHK(h, hipMemcpyAsync(h->d_src.p, src.data(), sizeof(PsvSrc) * S, hipMemcpyHostToDevice, h->stream));
     HIPCHK(h, hipMemcpyAsync(h->d_fluidK.p, fk.data(), sizeof(double2) * fk.size(), hipMemcpyHostToDevice, h->stream));
     HIPCHK(h, hipMemcpyAsync(h->d_omg.p, omg.data(), sizeof(double) * S, hipMemcpyHostToDevice, h->stream));
@@ -1115,7 +1175,7 @@ static int psv_run_batch(dsm_sgt_t *h, const PsvPrep &p)
                 kst_host.resize((size_t)S * kst_nblk);
                 HIPCHK(h, hipMemcpyAsync(kst_host.data(), h->d_kstart.p, sizeof(int) * kst_host.size(),
                                          hipMemcpyDeviceToHost, h->stream));
-                HIPCHK(h, hipStreamSynchronize(h->stream));
+                HIPCHK(h, dev_sync(h));
                 HIPCHK(h, h->d_scratch.ensure(sizeof(double2) * (size_t)std::max(total, 1LL)));
                 sa.scratch = h->d_scratch.as<double2>();
                 sa.loff = loff;
@@ -1188,7 +1248,7 @@ static int psv_run_batch(dsm_sgt_t *h, const PsvPrep &p)
         HIPCHK(h, hipMemcpy2DAsync(&ampuv[2 * (size_t)l0], sizeof(double2) * 2 * LC,
                                    h->d_amp.as<double2>() + 2 * (size_t)l0, sizeof(double2) * 2 * LC,
                                    sizeof(double2) * 2 * (lend - l0 + 1), S, hipMemcpyDeviceToHost, h->stream));
-        HIPCHK(h, hipStreamSynchronize(h->stream));
+        HIPCHK(h, dev_sync(h));
         // replay the cut-off in the reference's order (SGTpsv-MPI.f90:616-634, 884-888)
         std::vector<int> still;
         for (int k : active) {
@@ -1312,7 +1372,7 @@ static int psv_run_batch(dsm_sgt_t *h, const PsvPrep &p)
         std::vector<int> llog(S);
         HIPCHK(h, hipMemcpyAsync(nl.data(), h->d_nl.p, sizeof(int) * S, hipMemcpyDeviceToHost, h->stream));
         HIPCHK(h, hipMemcpyAsync(llog.data(), h->d_llog.p, sizeof(int) * S, hipMemcpyDeviceToHost, h->stream));
-        HIPCHK(h, hipStreamSynchronize(h->stream));
+        HIPCHK(h, dev_sync(h));
         for (int k = 0; k < S; k++) {
             h->llog[slotout[k]] = llog[k] - 1;
             h->done[slotout[k]] = 1;
@@ -1322,7 +1382,7 @@ static int psv_run_batch(dsm_sgt_t *h, const PsvPrep &p)
     for (int k = 0; k < S; k++) sumL += nl[k];
     // 4 flops per (complex coefficient x real function) per (r,theta)
     h->stats.synth_flops += 4.0 * sumL * s.theta_n * (double)PSV_NQ * R;
-    HIPCHK(h, hipStreamSynchronize(h->stream));
+    HIPCHK(h, dev_sync(h));
     if (h->ns) h->stats.synth_mfma_flops += b16_issued_flops(h, nlr_dev, S, R);
     ev_collect(h);
     return DSM_SGT_OK;
@@ -1433,9 +1493,11 @@ int dsm_sgt_run(dsm_sgt_t *h, int32_t i_begin, int32_t i_end)
     const int nb = (int)batches.size();
     if (nb == 0) return DSM_SGT_OK;
     auto prepare = [h, &batches](int b) { return psv_prepare(h->psv, batches[b], h->prep[b & 1]); };
-    std::future<bool> next = std::async(std::launch::async, prepare, 0);
+    std::future<void> next = std::async(std::launch::async, prepare, 0);
     for (int b = 0; b < nb; b++) {
-        if (!next.get()) return fail(h, DSM_SGT_ENOMEM, "pinned host buffer");
+        const auto tw = std::chrono::steady_clock::now();
+        next.get();
+        h->stats.prep_wait_ms += ms_since(tw);
         if (b + 1 < nb) next = std::async(std::launch::async, prepare, b + 1);
         int rc = psv_run_batch(h, h->prep[b & 1]);
         if (rc) {
@@ -1503,6 +1565,22 @@ int dsm_sgt_fetch(dsm_sgt_t *h, int32_t i, int32_t kind, int32_t ir, void *out)
         o[2 * k] = (float)buf[k].x;
         o[2 * k + 1] = (float)buf[k].y;
     }
+    return DSM_SGT_OK;
+}
+
+int dsm_sgt_psv_band(dsm_sgt_t *h, int32_t i, double *band)
+{
+    if (!h || !band) return DSM_SGT_EINVAL;
+    if (h->mode != DSM_SGT_PSV || !h->ready) return fail(h, DSM_SGT_ESTATE, "needs a created P-SV handle");
+    if (i < 0) return fail(h, DSM_SGT_EINVAL, "negative frequency index");
+    if (hipSetDevice(h->device) != hipSuccess) return fail(h, DSM_SGT_EHIP, "hipSetDevice failed");
+    PsvPrep p;
+    psv_prepare(h->psv, std::vector<int>{i}, p);
+    int rc = psv_device_band(h, p.scal, 1, 0);
+    if (rc) return rc;
+    HIPCHK(h, hipMemcpyAsync(band, h->d_band.p, sizeof(double2) * 16 * (size_t)h->psv.nn, hipMemcpyDeviceToHost,
+                             h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
     return DSM_SGT_OK;
 }
 
@@ -1605,8 +1683,6 @@ void dsm_sgt_destroy(dsm_sgt_t *h)
         ev_collect(h);
         (void)hipStreamDestroy(h->stream);
     }
-    for (auto &p : h->prep)
-        if (p.band) (void)hipHostFree(p.band);
     delete h;
 }
 

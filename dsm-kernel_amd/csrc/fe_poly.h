@@ -5,6 +5,7 @@
 #pragma once
 
 #include <algorithm>
+#include <cmath>
 #include <complex>
 
 namespace dsm {
@@ -16,6 +17,24 @@ using cd = std::complex<double>;
 static inline cd cmul(cd a, cd b)
 {
     return cd(a.real() * b.real() - a.imag() * b.imag(), a.real() * b.imag() + a.imag() * b.real());
+}
+
+// Complex division by Smith's method, the reference's (gfortran's complex
+// division and the GNU runtime's __divdc3 on finite operands), written out:
+// the __divdc3 a shared library binds depends on its linker (clang's
+// runtime scales by powers of two and rounds differently), and the device
+// assembly (dev_common.h cdiv) uses these operations.
+static inline cd cdiv(cd p, cd q)
+{
+    const double a = p.real(), b = p.imag(), c = q.real(), d = q.imag();
+    if (std::fabs(c) < std::fabs(d)) {
+        const double ratio = c / d;
+        const double denom = (c * ratio) + d;
+        return cd(((a * ratio) + b) / denom, ((b * ratio) - a) / denom);
+    }
+    const double ratio = d / c;
+    const double denom = (d * ratio) + c;
+    return cd(((b * ratio) + a) / denom, (b - (a * ratio)) / denom);
 }
 
 // pmulti: c = a*b for polynomials of n and m coefficients (l = n+m-1).

@@ -117,8 +117,40 @@ struct PsvOmega {
     // (P0,P1), (Q0,Q1); kinds in PsvSetup::term_kind
     std::vector<cd> P0, P1, Q0, Q1;   // [nn][4]
     std::vector<cd> fluidK;           // [r_n]: (omega - i omegai)/3 / lambda~ (calupfluid)
+    // scalars of the band assembly: per zone the moduli factors coef1,
+    // coef2 and 1/coef2 (calcoef), and (omega - i omegai)^2
+    std::vector<cd> zc1, zc2, zcf;
+    cd comega2;
 };
+// psv_omega_scalars: everything but the band pieces (lsuf, calcoef,
+// calabnum, calupfluid); psv_omega_band: the band pieces P0..Q1 and calbc
+// from those scalars; psv_omega_setup: both.
+void psv_omega_scalars(const PsvSetup &s, int i, PsvOmega &o);
+void psv_omega_band(const PsvSetup &s, PsvOmega &o);
 void psv_omega_setup(const PsvSetup &s, int i, PsvOmega &o);
+
+// Frequency-independent tables of the device band assembly
+// (psv_band_kernel, the operations of psv_omega_band): the element
+// integrals of every zone in one array, per zone its base offset there,
+// kind, layer count and first band column, per band column the (up to
+// two) zones contributing to it in zone order, and calbc's overrides.
+// Layout of a solid zone's block: PSV_FE_NSOLID4 arrays of 4 nlayer
+// (PsvFeSolid order), then PSV_FE_NSOLIDM arrays of 4 (nlayer + 1)
+// (PsvFeSolidM order); a fluid zone: p1, p2, p3, 4 nlayer each.
+enum PsvFeSolid {
+    FE_T, FE_H1X, FE_H2N, FE_H3AY, FE_H5AY, FE_H4AN, FE_H6AN, FE_H7Z, FE_H7Y, FE_H8N, FE_H2L, FE_H4AL, FE_H6AL, FE_H8L,
+    FE_H3Y, FE_H4N, FE_H6L, FE_H4L, FE_H5Y, FE_H6N, PSV_FE_NSOLID4
+};
+enum PsvFeSolidM { FE_H3MY, FE_H4M2N, FE_H6M2L, FE_H4M1L, FE_H5MY, FE_H6M1N, PSV_FE_NSOLIDM };
+struct PsvBandTables {
+    std::vector<double> fe;
+    std::vector<long long> zbase;
+    std::vector<int> zsolid, znl, zkkdr;
+    std::vector<int> colz;     // [nn][2], -1: none
+    std::vector<int> ovr_ii;   // [nn]: band row (0-based) calbc sets, -1: none
+    std::vector<double> ovr;   // [nn][2]: the value of P0 there (re, im)
+};
+void psv_band_tables(const PsvSetup &s, PsvBandTables &t);
 
 // Device layouts shared by the engine and the host test build:
 //  band  [nn][4 band rows][P0, P1, Q0, Q1] complex (interleaved re/im),

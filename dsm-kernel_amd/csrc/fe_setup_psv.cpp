@@ -682,7 +682,7 @@ void overlapb(int nlayer, const cd *b, cd *b2)
 
 }  // namespace
 
-void psv_omega_setup(const PsvSetup &s, int i, PsvOmega &o)
+void psv_omega_scalars(const PsvSetup &s, int i, PsvOmega &o)
 {
     const int nz = s.nzone;
     o.omega = 2.0 * PI * (double)i / s.tlen;
@@ -694,7 +694,10 @@ void psv_omega_setup(const PsvSetup &s, int i, PsvOmega &o)
         o.lsuf = (int)(omega * s.vrmax[nz - 1] / tvs - 0.5) + 1;
     }
     // calcoef (calmat.f90:409-447)
-    std::vector<cd> coef1(nz), coef2(nz), coef(nz);
+    o.zc1.assign(nz, 0.0);
+    o.zc2.assign(nz, 0.0);
+    o.zcf.assign(nz, 0.0);
+    std::vector<cd> &coef1 = o.zc1, &coef2 = o.zc2, &coef = o.zcf;
     for (int k = 0; k < nz; k++) {
         if (s.qmu[k] <= 0.0) coef1[k] = 1.0;
         else {
@@ -709,14 +712,15 @@ void psv_omega_setup(const PsvSetup &s, int i, PsvOmega &o)
             const double bb = 1.0 / (2.0 * s.qkappa[k]);
             coef2[k] = cmul(cd(aa, bb), cd(aa, bb));
         }
-        coef[k] = cd(1.0, 0.0) / coef2[k];
+        coef[k] = cdiv(cd(1.0, 0.0), coef2[k]);
     }
     const int spn = s.spn;
     o.coef1_spn = coef1[spn - 1];
     o.coef2_spn = coef2[spn - 1];
     const cd w(omega, -omegai);
     o.womega = w;
-    const cd comega2 = cmul(w, w);
+    o.comega2 = cmul(w, w);
+    const cd comega2 = o.comega2;
     // calabnum (calmat.f90:815-903)
     o.anum.assign(160, 0.0);
     o.bnum.assign(160, 0.0);
@@ -754,17 +758,17 @@ void psv_omega_setup(const PsvSetup &s, int i, PsvOmega &o)
             const cd xLL = cd(2.0 * xecL.real(), 2.0 * xecL.imag());
             const cd xAmL = xecA - xLL;
             const cd xecF = cd(m.eta * xAmL.real(), m.eta * xAmL.imag());
-            const cd xFdC = xecF / xecC;
+            const cd xFdC = cdiv(xecF, xecC);
             const cd xAmF2dC = xecA - cmul(xecF, xFdC);
             const double r2 = r * r;
             auto rs = [](double a, cd b) { return cd(a * b.real(), a * b.imag()); };
             AN(o.anum, 1, 1, k) = rs(-(2.0 / r), xFdC);
-            AN(o.anum, 1, 2, k) = cd(1.0, 0.0) / xecC;
+            AN(o.anum, 1, 2, k) = cdiv(cd(1.0, 0.0), xecC);
             AN(o.anum, 2, 1, k) = cmul(-xrho, comega2) + rs(4.0 / r2, xAmF2dC - xecN);
             AN(o.anum, 2, 2, k) = rs(2.0 / r, xFdC - 1.0);
             AN(o.anum, 3, 1, k) = cd(-(1.0 / r), 0.0);
             AN(o.anum, 3, 3, k) = cd(1.0 / r, 0.0);
-            AN(o.anum, 3, 4, k) = cd(1.0, 0.0) / xecL;
+            AN(o.anum, 3, 4, k) = cdiv(cd(1.0, 0.0), xecL);
             AN(o.anum, 4, 1, k) = rs(-(2.0 / r2), xAmF2dC - xecN);
             AN(o.anum, 4, 2, k) = rs(-(1.0 / r), xFdC);
             AN(o.anum, 4, 3, k) = cmul(-xrho, comega2) - rs(2.0 / r2, xecN);
@@ -775,6 +779,26 @@ void psv_omega_setup(const PsvSetup &s, int i, PsvOmega &o)
             AN(o.bnum, 4, 3, k) = rs(1.0 / r2, xAmF2dC);
         }
     }
+    // calupfluid (others.f90:1009-1031): omega/3 / lambda~ per receiver
+    o.fluidK.assign(s.r_n, 0.0);
+    {
+        const double deuxsurpi = 0.63661977236, unsurdeuxpi = 0.15915494309,
+                     unsurtrois = 0.33333333333333333333;
+        for (int ir = 0; ir < s.r_n; ir++) {
+            if (!s.fluid_rec[ir]) continue;
+            const double qk = s.qkp[ir];
+            cd fl = cmul(cd(s.lambda[ir], 0.0), cd(1.0, 1.0 / qk));
+            fl = cmul(fl, cd(1 + deuxsurpi / qk * std::log(unsurdeuxpi * w.real()), 0.0));
+            o.fluidK[ir] = cdiv(cd(w.real() * unsurtrois, w.imag() * unsurtrois), fl);
+        }
+    }
+}
+
+void psv_omega_band(const PsvSetup &s, PsvOmega &o)
+{
+    const int nz = s.nzone;
+    const std::vector<cd> &coef1 = o.zc1, &coef2 = o.zc2, &coef = o.zcf;
+    const cd comega2 = o.comega2;
     // per-zone band blocks a0/a1/a2 (SGTpsv-MPI.f90:572-604)
     const size_t NN = (size_t)s.nn;
     o.P0.assign(4 * NN, 0.0);
@@ -898,13 +922,13 @@ void psv_omega_setup(const PsvSetup &s, int i, PsvOmega &o)
             for (auto &x : cw) x = 0.0;
             for (int k = 1; k <= 4 * nl; k++) {
                 if (k % 4 == 3) continue;
-                cw[k - 1] = cd(-z.p1[k - 1], 0.0) / comega2 + cmul(cf, cd(z.p3[k - 1], 0.0));
+                cw[k - 1] = cdiv(cd(-z.p1[k - 1], 0.0), comega2) + cmul(cf, cd(z.p3[k - 1], 0.0));
             }
             overlapb(nl, cw.data(), a0z.data());
             for (auto &x : cw) x = 0.0;
             for (int k = 1; k <= 4 * nl; k++) {
                 if (k % 4 == 3) continue;
-                cw[k - 1] = cd(-z.p2[k - 1], 0.0) / comega2;
+                cw[k - 1] = cdiv(cd(-z.p2[k - 1], 0.0), comega2);
             }
             overlapb(nl, cw.data(), a2z.data());
             for (int c = 0; c <= nl; c++) {
@@ -927,17 +951,66 @@ void psv_omega_setup(const PsvSetup &s, int i, PsvOmega &o)
             o.P1[q] = o.Q0[q] = o.Q1[q] = 0.0;
         }
     }
-    // calupfluid (others.f90:1009-1031): omega/3 / lambda~ per receiver
-    o.fluidK.assign(s.r_n, 0.0);
-    {
-        const double deuxsurpi = 0.63661977236, unsurdeuxpi = 0.15915494309,
-                     unsurtrois = 0.33333333333333333333;
-        for (int ir = 0; ir < s.r_n; ir++) {
-            if (!s.fluid_rec[ir]) continue;
-            const double qk = s.qkp[ir];
-            cd fl = cmul(cd(s.lambda[ir], 0.0), cd(1.0, 1.0 / qk));
-            fl = cmul(fl, cd(1 + deuxsurpi / qk * std::log(unsurdeuxpi * w.real()), 0.0));
-            o.fluidK[ir] = cd(w.real() * unsurtrois, w.imag() * unsurtrois) / fl;
+}
+
+void psv_omega_setup(const PsvSetup &s, int i, PsvOmega &o)
+{
+    psv_omega_scalars(s, i, o);
+    psv_omega_band(s, o);
+}
+
+void psv_band_tables(const PsvSetup &s, PsvBandTables &t)
+{
+    const int nz = s.nzone, nn = s.nn;
+    t = PsvBandTables{};
+    for (int iz = 0; iz < nz; iz++) {
+        const PsvZoneFE &z = s.fe[iz];
+        const int nl = z.nlayer;
+        t.zbase.push_back((long long)t.fe.size());
+        t.zsolid.push_back(z.solid ? 1 : 0);
+        t.znl.push_back(nl);
+        t.zkkdr.push_back(s.kkdr[iz]);
+        auto app = [&](const std::vector<double> &v, size_t n) {
+            for (size_t k = 0; k < n; k++) t.fe.push_back(k < v.size() ? v[k] : 0.0);
+        };
+        if (z.solid) {
+            const std::vector<double> *a4[PSV_FE_NSOLID4] = {&z.t,   &z.h1x,  &z.h2N,  &z.h3ay, &z.h5ay, &z.h4aN, &z.h6aN,
+                                                             &z.h7z, &z.h7y,  &z.h8N,  &z.h2L,  &z.h4aL, &z.h6aL, &z.h8L,
+                                                             &z.h3y, &z.h4N,  &z.h6L,  &z.h4L,  &z.h5y,  &z.h6N};
+            const std::vector<double> *am[PSV_FE_NSOLIDM] = {&z.h3my, &z.h4m2N, &z.h6m2L, &z.h4m1L, &z.h5my, &z.h6m1N};
+            for (auto *v : a4) app(*v, 4 * (size_t)nl);
+            for (auto *v : am) app(*v, 4 * (size_t)(nl + 1));
+        } else {
+            app(z.p1, 4 * (size_t)nl);
+            app(z.p2, 4 * (size_t)nl);
+            app(z.p3, 4 * (size_t)nl);
+        }
+    }
+    t.colz.assign(2 * (size_t)nn, -1);
+    for (int iz = 0; iz < nz; iz++) {
+        const int ncol = s.fe[iz].solid ? 2 * s.fe[iz].nlayer + 2 : s.fe[iz].nlayer + 1;
+        for (int c = 0; c < ncol; c++) {
+            const int col = s.kkdr[iz] + c;  // 1-based
+            int *cz = &t.colz[2 * (size_t)(col - 1)];
+            if (cz[0] < 0) cz[0] = iz;
+            else cz[1] = iz;
+        }
+    }
+    t.ovr_ii.assign(nn, -1);
+    t.ovr.assign(2 * (size_t)nn, 0.0);
+    for (int k = 0; k + 1 < nz; k++) {  // calbc, as in psv_omega_band
+        const double rr = s.vrmax[k] * s.vrmax[k];
+        const int j = s.kkdr[k + 1] - 1;
+        if (s.iphase[k] == 1 && s.iphase[k + 1] == 2) {
+            t.ovr_ii[j] = 1;
+            t.ovr[2 * (size_t)j] = rr;
+            t.ovr[2 * (size_t)j + 1] = 0.0;
+        }
+        if (s.iphase[k] == 2 && s.iphase[k + 1] == 1) {
+            const cd v = -cd(rr, 0.0);
+            t.ovr_ii[j] = 2;
+            t.ovr[2 * (size_t)j] = v.real();
+            t.ovr[2 * (size_t)j + 1] = v.imag();
         }
     }
 }

@@ -14,6 +14,7 @@
 #include "fe_setup.h"
 #include "legendre_lane.h"
 #include "psv_jobs.h"
+#include "psv_band_lane.h"
 #include "psv_lane.h"
 #include "sh_jobs.h"
 #include "sh_lane.h"
@@ -160,6 +161,59 @@ extern "C" int dsm_hostcheck_sh(const dsm_sgt_input_t *in, int32_t i, double *ts
                 out[2 * o2 + 1] = im;
             }
     }
+    return 0;
+}
+
+// P-SV band pieces of frequency i as the host set-up forms them
+// (psv_omega_setup + psv_pack_band): band[nn][4][4] complex.
+extern "C" int dsm_hostcheck_psv_band(const dsm_sgt_input_t *in, int32_t i, double *band)
+{
+    PsvSetup s;
+    std::string err;
+    int rc = psv_setup(*in, s, err);
+    if (rc) return rc;
+    PsvOmega o;
+    psv_omega_setup(s, i, o);
+    psv_pack_band(s, o, band);
+    return 0;
+}
+
+// The same band through psv_band_column (the device assembly's code, run
+// on the host), from psv_band_tables and psv_omega_scalars.
+extern "C" int dsm_hostcheck_psv_band_lane(const dsm_sgt_input_t *in, int32_t i, double *band)
+{
+    PsvSetup s;
+    std::string err;
+    int rc = psv_setup(*in, s, err);
+    if (rc) return rc;
+    PsvBandTables bt;
+    psv_band_tables(s, bt);
+    PsvOmega o;
+    psv_omega_scalars(s, i, o);
+    const int nz = s.nzone;
+    std::vector<double2> scal(1 + 3 * nz);
+    auto d2 = [](const cd &c) { return make_double2(c.real(), c.imag()); };
+    scal[0] = d2(o.comega2);
+    for (int iz = 0; iz < nz; iz++) {
+        scal[1 + 3 * iz] = d2(o.zc1[iz]);
+        scal[2 + 3 * iz] = d2(o.zc2[iz]);
+        scal[3 + 3 * iz] = d2(o.zcf[iz]);
+    }
+    PsvBandArgs a{};
+    a.fe = bt.fe.data();
+    a.zbase = bt.zbase.data();
+    a.zsolid = bt.zsolid.data();
+    a.znl = bt.znl.data();
+    a.zkkdr = bt.zkkdr.data();
+    a.colz = bt.colz.data();
+    a.ovr_ii = bt.ovr_ii.data();
+    a.ovr = (const double2 *)bt.ovr.data();
+    a.nz = nz;
+    a.nn = s.nn;
+    a.scal = scal.data();
+    a.slot0 = 0;
+    a.band = (double2 *)band;
+    for (int j = 0; j < s.nn; j++) psv_band_column(a, 0, j);
     return 0;
 }
 

@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include "psv_band_lane.h"
 #include "psv_lane.h"
 #include "sh_lane.h"
 
@@ -23,6 +24,9 @@ void launch_psv_depth(const PsvDepthArgs &a, hipStream_t s);
 void launch_psv_need(const PsvNeedArgs &a, hipStream_t s);
 // l cut-off replay on the device (batched forms) and the bf16 row extents
 void launch_psv_cut(const PsvCutArgs &a, hipStream_t s);
+
+// Band pieces on the device (psv_band.hip; PsvBandArgs in psv_band_lane.h).
+void launch_psv_band(const PsvBandArgs &a, int nslots, hipStream_t s);
 void launch_psv_nlr(const int *kstart, const int *rlow, const int *nl, int nslots, int nblk, int r_n, int *nlr,
                     hipStream_t s);
 // per-block factor storage of the batched forms (PsvSolveArgs::loff)

@@ -33,7 +33,8 @@ class Stats(C.Structure):
     _fields_ = [("solve_ms", C.c_double), ("legendre_ms", C.c_double), ("synth_ms", C.c_double),
                 ("other_ms", C.c_double), ("solve_launches", C.c_int64), ("synth_launches", C.c_int64),
                 ("systems", C.c_int64), ("solve_bytes", C.c_double), ("synth_flops", C.c_double),
-                ("synth_bytes", C.c_double), ("synth_mfma_flops", C.c_double)]
+                ("synth_bytes", C.c_double), ("synth_mfma_flops", C.c_double),
+                ("prep_wait_ms", C.c_double), ("device_wait_ms", C.c_double)]
 
 
 class DsmSgtError(RuntimeError):
@@ -47,7 +48,7 @@ _lib = None
 # every entry point declared in include/dsm_sgt.h
 EXPORTS = ("dsm_sgt_create", "dsm_sgt_get_info", "dsm_sgt_get_grid", "dsm_sgt_run", "dsm_sgt_fetch",
            "dsm_sgt_fetch_fp64", "dsm_sgt_lcut", "dsm_sgt_write_catalog", "dsm_sgt_get_stats",
-           "dsm_sgt_reset_stats", "dsm_sgt_clear", "dsm_sgt_last_error", "dsm_sgt_destroy")
+           "dsm_sgt_reset_stats", "dsm_sgt_clear", "dsm_sgt_last_error", "dsm_sgt_destroy", "dsm_sgt_psv_band")
 
 
 def lib():
@@ -65,6 +66,7 @@ def lib():
         L.dsm_sgt_fetch.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_void_p]
         L.dsm_sgt_fetch_fp64.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_void_p]
         L.dsm_sgt_lcut.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_int32)]
+        L.dsm_sgt_psv_band.argtypes = [C.c_void_p, C.c_int32, C.c_void_p]
         L.dsm_sgt_write_catalog.argtypes = [C.c_void_p, C.c_int32, C.c_char_p, C.c_char_p]
         L.dsm_sgt_get_stats.argtypes = [C.c_void_p, C.POINTER(Stats)]
         L.dsm_sgt_reset_stats.argtypes = [C.c_void_p]
@@ -126,6 +128,13 @@ class Engine:
         r = np.stack([self.fetch_fp64(i, RSGT, ir) for ir in range(R)], axis=1)
         s = self.fetch_fp64(i, SYNN)
         return t, r, s
+
+    def psv_band(self, i):
+        """Band pieces the device assembles for frequency i (P-SV):
+        complex128 [nn][4 band rows][P0, P1, Q0, Q1]."""
+        out = np.zeros((self.info.nn, 4, 4), np.complex128)
+        self._check(lib().dsm_sgt_psv_band(self._h, i, out.ctypes.data))
+        return out
 
     def lcut(self, i):
         v = C.c_int32()

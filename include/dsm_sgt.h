@@ -96,6 +96,8 @@ typedef struct dsm_sgt_stats {
     double synth_flops;       /* algorithmic flops of the synthesis contraction */
     double synth_bytes;       /* operand bytes streamed by the synthesis */
     double synth_mfma_flops;  /* MFMA flops the synthesis kernels issued (bf16 forms: all cross products) */
+    double prep_wait_ms;      /* host time the run waited for a batch's host set-up */
+    double device_wait_ms;    /* host time the run blocked on the device inside batches */
 } dsm_sgt_stats_t;
 
 /* Create an engine on HIP device `device` for `mode` and `precision`.
@@ -124,6 +126,13 @@ int dsm_sgt_fetch_fp64(dsm_sgt_t *h, int32_t i, int32_t kind, int32_t ir, void *
 /* The l cut-off logged by the reference for frequency i (its llog-1,
  * SGTsh-MPI.f90:559, SGTpsv-MPI.f90:1098). */
 int dsm_sgt_lcut(dsm_sgt_t *h, int32_t i, int32_t *llog_minus_1);
+
+/* Diagnostic, no counterpart in the reference: the P-SV band pieces the
+ * device assembles for frequency i (the per-zone band blocks and calbc of
+ * SGTpsv-MPI.f90:572-604), band[nn][4 band rows][P0, P1, Q0, Q1] complex,
+ * interleaved (re, im): 32 doubles per band column.  For checking the
+ * device assembly against the host set-up; P-SV handles, between runs. */
+int dsm_sgt_psv_band(dsm_sgt_t *h, int32_t i, double *band);
 /* Write the catalog files of frequency i under outputDir with the
  * reference's names and byte layout (raw little-endian complex(4),
  * component fastest, no record markers). */
