@@ -1168,7 +1168,9 @@ static int psv_run_batch(dsm_sgt_t *h, const PsvPrep &p)
                 // factor storage of the depth-cut systems: offsets per block, total
                 const int nbt = (int)active.size() * da.nblk;
                 long long *loff = h->d_loff.as<long long>();
-                launch_psv_loff(h->d_kstart.as<int>(), nbt, n, klo, loff, loff + nbt, h->stream);
+                // mixed forms: checkpoints of the factorization below the source
+                sa.ckpt = h->precision != DSM_SGT_FP64 ? 1 : 0;
+                launch_psv_loff(h->d_kstart.as<int>(), nbt, n, klo, fs, sa.ckpt, loff, loff + nbt, h->stream);
                 long long total = 0;
                 HIPCHK(h, hipMemcpyAsync(&total, loff + nbt, sizeof(long long), hipMemcpyDeviceToHost, h->stream));
                 kst_nblk = da.nblk;

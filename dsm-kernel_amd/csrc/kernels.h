@@ -30,8 +30,8 @@ void launch_psv_band(const PsvBandArgs &a, int nslots, hipStream_t s);
 void launch_psv_nlr(const int *kstart, const int *rlow, const int *nl, int nslots, int nblk, int r_n, int *nlr,
                     hipStream_t s);
 // per-block factor storage of the batched forms (PsvSolveArgs::loff)
-void launch_psv_loff(const int *kstart, int nblocks, int n, int klo, long long *loff, long long *total,
-                     hipStream_t s);
+void launch_psv_loff(const int *kstart, int nblocks, int n, int klo, int fs, int ckpt, long long *loff,
+                     long long *total, hipStream_t s);
 // compact band of the fused family (PsvSolveArgs::cband)
 void launch_psv_compact(const double2 *band, const int *kind, int nn, const int *slot_list, int nslots,
                         double2 *cband, hipStream_t s);

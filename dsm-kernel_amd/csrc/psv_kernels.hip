@@ -72,17 +72,17 @@ void launch_psv_need(const PsvNeedArgs &a, hipStream_t s)
     hipLaunchKernelGGL(psv_need_kernel, dim3((n + 63) / 64), dim3(64), 0, s, a);
 }
 
-// Offsets of the per-block factor storage: block b stores the rows
-// max(klo, kstart[b]) .. n of its 256 lanes (3 complex per row and lane);
-// out[0] = total in double2.  One workgroup of 1024 threads.
-__global__ __launch_bounds__(1024) void psv_loff_kernel(const int *kstart, int nblocks, int n, int klo,
-                                                        long long *loff, long long *total)
+// Offsets of the per-block factor storage: block b stores
+// psv_block_store(n, fs, max(klo, kstart[b]), ckpt) complex per lane of its
+// 256; out[0] = total in double2.  One workgroup of 1024 threads.
+__global__ __launch_bounds__(1024) void psv_loff_kernel(const int *kstart, int nblocks, int n, int klo, int fs,
+                                                        int ckpt, long long *loff, long long *total)
 {
     __shared__ long long part[1024];
     const int t = threadIdx.x, c = (nblocks + 1023) / 1024;
     const int b0 = min(nblocks, t * c), b1 = min(nblocks, b0 + c);
     long long sum = 0;
-    for (int b = b0; b < b1; b++) sum += 3LL * SV_NT * (n - max(klo, kstart[b]) + 1);
+    for (int b = b0; b < b1; b++) sum += SV_NT * psv_block_store(n, fs, max(klo, kstart[b]), ckpt != 0);
     part[t] = sum;
     __syncthreads();
     for (int d = 1; d < 1024; d <<= 1) {  // inclusive scan
@@ -94,15 +94,15 @@ __global__ __launch_bounds__(1024) void psv_loff_kernel(const int *kstart, int n
     long long off = part[t] - sum;
     for (int b = b0; b < b1; b++) {
         loff[b] = off;
-        off += 3LL * SV_NT * (n - max(klo, kstart[b]) + 1);
+        off += SV_NT * psv_block_store(n, fs, max(klo, kstart[b]), ckpt != 0);
     }
     if (t == 1023) total[0] = part[1023];
 }
 
-void launch_psv_loff(const int *kstart, int nblocks, int n, int klo, long long *loff, long long *total,
-                     hipStream_t s)
+void launch_psv_loff(const int *kstart, int nblocks, int n, int klo, int fs, int ckpt, long long *loff,
+                     long long *total, hipStream_t s)
 {
-    hipLaunchKernelGGL(psv_loff_kernel, dim3(1), dim3(1024), 0, s, kstart, nblocks, n, klo, loff, total);
+    hipLaunchKernelGGL(psv_loff_kernel, dim3(1), dim3(1024), 0, s, kstart, nblocks, n, klo, fs, ckpt, loff, total);
 }
 
 // One 64-lane workgroup per slot, 64 degrees at a time.  In the count
@@ -244,7 +244,7 @@ void launch_psv_compact(const double2 *band, const int *kind, int nn, const int 
                        slot_list, cband);
 }
 
-template <bool F>
+template <bool F, bool CK>  // CK: checkpointed storage (batched mixed forms, F only)
 __global__ __launch_bounds__(SV_NT) void psv_factor_kernel(PsvSolveArgs a)
 {
     constexpr int CW = F ? 8 : 16;  // double2 per staged column
@@ -332,11 +332,27 @@ __global__ __launch_bounds__(SV_NT) void psv_factor_kernel(PsvSolveArgs a)
         W.d2 = e[3];
     }
     W.w4m3 = W.w4m2 = W.w4m1 = W.w3m2 = W.w3m1 = W.w2m1 = cz();
+    // checkpointed storage (batched mixed forms): windows of the segments
+    // below the source, then the factors of rows xb = max(fs, kL) .. n
+    const int nseg = (CK && fs > kL) ? (fs - kL + PSV_CKPT - 1) / PSV_CKPT : 0;
+    const int xb = CK ? max(fs, kL) : kL;
+    double2 *Lx = CK ? Lb + (size_t)6 * nseg * SV_NT : Lb;
     auto storeL = [&](int k, double2 w2, double2 w3, double2 w4) {
-        if (act && k >= klo) {
-            Lb[(3 * (k - kL) + 0) * Lst + Lln] = w2;
-            Lb[(3 * (k - kL) + 1) * Lst + Lln] = w3;
-            Lb[(3 * (k - kL) + 2) * Lst + Lln] = w4;
+        if (act && k >= klo && k >= xb) {
+            Lx[(3 * (k - xb) + 0) * Lst + Lln] = w2;
+            Lx[(3 * (k - xb) + 1) * Lst + Lln] = w3;
+            Lx[(3 * (k - xb) + 2) * Lst + Lln] = w4;
+        }
+    };
+    auto storeW = [&](int k) {  // window before the step of row k, at the start of a segment
+        if (act && ((k - kL) % PSV_CKPT) == 0) {
+            double2 *c = Lb + (size_t)6 * ((k - kL) / PSV_CKPT) * SV_NT + Lln;
+            c[0 * SV_NT] = W.d0;
+            c[1 * SV_NT] = W.e01;
+            c[2 * SV_NT] = W.e02;
+            c[3 * SV_NT] = W.d1;
+            c[4 * SV_NT] = W.e12;
+            c[5 * SV_NT] = W.d2;
         }
     };
     // phase 1: rows below the source, factorization only (fs <= n - 3);
@@ -349,8 +365,9 @@ __global__ __launch_bounds__(SV_NT) void psv_factor_kernel(PsvSolveArgs a)
     for (int k = max(kst, klo); k < fs; k++) {
         double2 e[4], w2, w3, w4;
         column(k + 3, e);
+        if (CK) storeW(k);
         psv_factor_step<F>(W, k, n, e, crecip_t<F>(W.d0), w2, w3, w4);
-        storeL(k, w2, w3, w4);
+        if (!CK) storeL(k, w2, w3, w4);
     }
     // phase 2: forward values of the source solutions at row k (their
     // right-hand sides sit at rows fs..fs+3), then the factorization step
@@ -547,6 +564,131 @@ __global__ __launch_bounds__(64) void psv_back_kernel(PsvSolveArgs a)
     // reads them as zero (PsvCoefArgs::kstart)
 }
 
+// Backward substitution of the checkpointed storage (batched mixed forms):
+// rows n .. fs from the stored factors as in psv_back_kernel; below the
+// source, segment by segment from the top, the factors of a segment's
+// PSV_CKPT rows recomputed from its window (psv_factor_step on the compact
+// band, the factor kernel's operations on the same values, so the same
+// bits) into registers, then the substitution down through them.
+__global__ __launch_bounds__(64) void psv_back_ckpt_kernel(PsvSolveArgs a)
+{
+    constexpr bool F = true;
+    const int sub = blockIdx.x & 3, xb_ = blockIdx.x >> 2;
+    const int sl = a.slot_list[blockIdx.y];
+    const int lraw = a.l0 + xb_ * SV_NT + sub * 64 + threadIdx.x;
+    if (lraw > a.lend || lraw == 0) return;
+    const int l = lraw;
+    const long gid = ((long)blockIdx.y * (gridDim.x >> 2) + xb_) * SV_NT + sub * 64 + threadIdx.x;
+    const long ss = a.sstride;
+    const double l2 = (double)l * (double)(l + 1);
+    const double lsq = sqrt(l2);
+    const int n = a.nn - a.row0 + 1;
+    const int fs = a.jtmp - a.row0 + 1;
+    const int klo = a.rowlo - a.row0 + 1;
+    const int nsrc = (l >= 2) ? 4 : 3;
+    const double2 *Zs = a.zs;
+    const double2 Dn1 = a.ends[gid], Dn = a.ends[ss + gid], w2n1 = a.ends[2 * ss + gid];
+    const int blk = blockIdx.y * (gridDim.x >> 2) + xb_;
+    const int kL = max(klo, a.kstart[blk]);  // rows below kL are outside the solved system or not needed
+    const int nseg = fs > kL ? (fs - kL + PSV_CKPT - 1) / PSV_CKPT : 0;
+    const int xb = max(fs, kL);
+    const int lane = sub * 64 + threadIdx.x;
+    const double2 *Ck = a.scratch + a.loff[blk];
+    const double2 *Lx = Ck + (size_t)6 * nseg * SV_NT;
+    const double2 *cb0 = a.cband + ((size_t)sl * a.nn + (a.row0 - 1)) * 8;
+    const int *cs0 = a.csel + (a.row0 - 1);
+
+    const double c0f = (double)(float)sqrt((double)(2 * l + 1) / (4.0 * DSM_PI) / 1.0);
+    const double cm1 = sqrt((double)(2 * l + 1) / (4.0 * DSM_PI) / (1.0 * (double)l * (double)(l + 1)));
+    const double hs = l2 * cm1 / 2.0;
+    const double2 yb_rhs = make_double2(-c0f, 0.0);
+    const double2 s_rhs = cdiv(make_double2(hs, 0.0), make_double2(lsq, 0.0));
+    const double2 zb_n1 = cdiv(yb_rhs, Dn1);
+    const double2 zb_n = cdiv(csub(cz(), cadd(cadd(cz(), cz()), cmul(w2n1, yb_rhs))), Dn);
+    const double2 zs_n = cdiv(s_rhs, Dn);
+
+    double2 x1[PSV_NSOL], x2[PSV_NSOL], x3[PSV_NSOL];  // b(k+1), b(k+2), b(k+3)
+    for (int s = 0; s < PSV_NSOL; s++) x1[s] = x2[s] = x3[s] = cz();
+    int p = 0;
+    double2 *xo = a.xn + (size_t)sl * a.nneed * PSV_NSOL * a.lcap + l;
+    auto record = [&](int k, const double2 *bk) {
+        const int row = a.row0 + k - 1;
+        if (p < a.nneed && a.need[p] == row) {
+            for (int s = 0; s < PSV_NSOL; s++) xo[(size_t)(PSV_NSOL * p + s) * a.lcap] = bk[s];
+            p++;
+        }
+    };
+    auto shift = [&](const double2 *bk) {
+#pragma unroll
+        for (int s = 0; s < PSV_NSOL; s++) {
+            x3[s] = x2[s];
+            x2[s] = x1[s];
+            x1[s] = bk[s];
+        }
+    };
+    // rows n .. xb: forward values present, factors stored
+    for (int k = n; k >= xb; k--) {
+        double2 L2 = cz(), L3 = cz(), L4 = cz();
+        if (k < n) L2 = Lx[(3 * (k - xb) + 0) * SV_NT + lane];
+        if (k < n - 1) L3 = Lx[(3 * (k - xb) + 1) * SV_NT + lane];
+        if (k < n - 2) L4 = Lx[(3 * (k - xb) + 2) * SV_NT + lane];
+        double2 zd[PSV_NSOL];
+#pragma unroll
+        for (int s = 0; s < 4; s++) zd[s] = (s < nsrc && k >= fs) ? Zs[(size_t)(4 * (k - fs) + s) * ss + gid] : cz();
+        zd[4] = (k == n) ? zb_n : (k == n - 1) ? zb_n1 : cz();
+        zd[5] = (k == n) ? zs_n : cz();
+        double2 bk[PSV_NSOL];
+        if (k == n) {
+#pragma unroll
+            for (int s = 0; s < PSV_NSOL; s++) bk[s] = zd[s];
+        } else {
+            const double2 m2 = cneg(L2), m3 = cneg(L3), m4 = cneg(L4);
+#pragma unroll
+            for (int s = 0; s < PSV_NSOL; s++)
+                bk[s] = cmadd_t<F>(cmadd_t<F>(cmadd_t<F>(zd[s], m4, x3[s]), m3, x2[s]), m2, x1[s]);
+        }
+        shift(bk);
+        record(k, bk);
+    }
+    // below the source: segments from the top
+    for (int sg = nseg - 1; sg >= 0; sg--) {
+        const int k0 = kL + sg * PSV_CKPT;
+        const double2 *c = Ck + (size_t)6 * sg * SV_NT + lane;
+        PsvWin W;
+        W.d0 = c[0 * SV_NT];
+        W.e01 = c[1 * SV_NT];
+        W.e02 = c[2 * SV_NT];
+        W.d1 = c[3 * SV_NT];
+        W.e12 = c[4 * SV_NT];
+        W.d2 = c[5 * SV_NT];
+        double2 w2[PSV_CKPT], w3[PSV_CKPT], w4[PSV_CKPT];
+#pragma unroll
+        for (int u = 0; u < PSV_CKPT; u++) {
+            const int k = k0 + u;
+            if (k < fs) {
+                double2 e[4];
+                psv_col_fast(cb0 + (size_t)(k + 2) * 8, cs0[k + 2], l2, lsq, e);  // column k + 3
+                psv_factor_step<F>(W, k, n, e, crecip_t<F>(W.d0), w2[u], w3[u], w4[u]);
+            }
+        }
+#pragma unroll
+        for (int u = PSV_CKPT - 1; u >= 0; u--) {
+            const int k = k0 + u;
+            if (k < fs) {
+                const double2 m2 = cneg(w2[u]), m3 = cneg(w3[u]), m4 = cneg(w4[u]);
+                double2 bk[PSV_NSOL];
+#pragma unroll
+                for (int s = 0; s < PSV_NSOL; s++)
+                    bk[s] = cmadd_t<F>(cmadd_t<F>(cmul_t<F>(m4, x3[s]), m3, x2[s]), m2, x1[s]);
+                shift(bk);
+                record(k, bk);
+            }
+        }
+    }
+    // recorded rows below the cut are not written: the coefficient kernel
+    // reads them as zero (PsvCoefArgs::kstart)
+}
+
 void launch_psv_excite(const PsvSolveArgs &a, int nslots, hipStream_t s)
 {
     const int nb = (a.lend - a.l0 + 64) / 64;
@@ -560,10 +702,15 @@ void launch_psv_solve(const PsvSolveArgs &a, int nslots, bool fast, hipStream_t 
     const int nb = psv_solve_blocks(a.l0, a.lend);
     const dim3 g1(nb, nslots), g2(4 * nb, nslots);
     if (fast) {
-        hipLaunchKernelGGL(psv_factor_kernel<true>, g1, dim3(SV_NT), 0, s, a);
-        hipLaunchKernelGGL(psv_back_kernel<true>, g2, dim3(64), 0, s, a);
+        if (a.ckpt && a.loff && a.kstart) {
+            hipLaunchKernelGGL((psv_factor_kernel<true, true>), g1, dim3(SV_NT), 0, s, a);
+            hipLaunchKernelGGL(psv_back_ckpt_kernel, g2, dim3(64), 0, s, a);
+        } else {
+            hipLaunchKernelGGL((psv_factor_kernel<true, false>), g1, dim3(SV_NT), 0, s, a);
+            hipLaunchKernelGGL(psv_back_kernel<true>, g2, dim3(64), 0, s, a);
+        }
     } else {
-        hipLaunchKernelGGL(psv_factor_kernel<false>, g1, dim3(SV_NT), 0, s, a);
+        hipLaunchKernelGGL((psv_factor_kernel<false, false>), g1, dim3(SV_NT), 0, s, a);
         hipLaunchKernelGGL(psv_back_kernel<false>, g2, dim3(64), 0, s, a);
     }
 }

@@ -61,7 +61,25 @@ struct PsvSolveArgs {
     // batched forms: factors of (active slot, l block) blk from scratch + loff[blk],
     // [k - max(klo, kstart)][3][256]; null: [k - klo][3][sstride]
     const long long *loff;
+    // batched mixed forms: below the source row the factorization's window
+    // every PSV_CKPT rows instead of the factors (psv_block_store); the
+    // backward pass recomputes a segment's factors from its checkpoint
+    int ckpt;
 };
+
+constexpr int PSV_CKPT = 8;
+
+// Factor storage of one block of the batched forms, double2 per lane, the
+// lowest stored row kL = max(klo, kstart): explicit, the factors (3 per
+// row) of rows kL .. n; checkpointed, the windows (6) of the segments of
+// PSV_CKPT rows from kL below the source row fs, then the factors of rows
+// max(fs, kL) .. n.
+__host__ __device__ inline long long psv_block_store(int n, int fs, int kL, bool ckpt)
+{
+    if (!ckpt) return 3LL * (n - kL + 1);
+    const int nseg = fs > kL ? (fs - kL + PSV_CKPT - 1) / PSV_CKPT : 0;
+    return 6LL * nseg + 3LL * (n - (fs > kL ? fs : kL) + 1);
+}
 
 // Depth cut of the batched forms: for the smallest l of every block of 256
 // degrees, the deepest band row whose evanescent decay below the source,
