@@ -166,6 +166,7 @@ struct dsm_sgt {
     std::vector<int> tiles_main, tiles_acc;  // host copies of the tile orders (launch sizes)
     int tiles_main_n = 0, tiles_acc_n = 0;
     DevBuf d_bfe, d_bzbase, d_bzsolid, d_bznl, d_bzkkdr, d_bcolz, d_bovrii, d_bovr, d_bscal;  // device band assembly
+    DevBuf d_plm;  // P_l^m scratch of the P-SV Legendre table
     DevBuf d_band, d_om, d_slots, d_nl, d_slotout, d_scratch, d_xn, d_E, d_Es, d_amp, d_jobs;
     DevBuf d_cat_t, d_cat_r, d_cat_s;
     // P-SV only
@@ -408,7 +409,9 @@ static double b16_issued_flops(const dsm_sgt_t *h, const std::vector<int> &nlr, 
 
 // Device buffers of the BF16 forms for nslot frequencies: split rows and
 // the stash of the largest launch.
-static int alloc_b16(dsm_sgt_t *h, int nslot, int nq, int nqs, int R)
+// lpad >= 0: only the degrees l >= lpad are zeroed (the caller writes every
+// lower degree a synthesis tile reads in each batch); < 0: all
+static int alloc_b16(dsm_sgt_t *h, int nslot, int nq, int nqs, int R, int lpad)
 {
     if (!h->ns) return DSM_SGT_OK;
     const size_t M = (size_t)nslot * R * 2, Ms = (size_t)nslot * 2;
@@ -417,9 +420,14 @@ static int alloc_b16(dsm_sgt_t *h, int nslot, int nq, int nqs, int R)
         return fail(h, DSM_SGT_EUNSUPPORTED, "too many receivers x frequencies per batch for the bf16 synthesis");
     HIPCHK(h, h->d_eb.ensure(sizeof(unsigned short) * nq * h->ns * M * h->lcap));
     HIPCHK(h, h->d_esb.ensure(sizeof(unsigned short) * nqs * h->ns * Ms * h->lcap));
-    // degrees above maxlmax are never written: zero once
-    HIPCHK(h, hipMemset(h->d_eb.p, 0, h->d_eb.n));
-    HIPCHK(h, hipMemset(h->d_esb.p, 0, h->d_esb.n));
+    if (lpad >= 0) {
+        launch_zero_pad_b16(h->d_eb.as<unsigned short>(), (size_t)nq * h->ns * M, h->lcap, lpad, h->stream);
+        launch_zero_pad_b16(h->d_esb.as<unsigned short>(), (size_t)nqs * h->ns * Ms, h->lcap, lpad, h->stream);
+        HIPCHK(h, hipGetLastError());
+    } else {
+        HIPCHK(h, hipMemset(h->d_eb.p, 0, h->d_eb.n));
+        HIPCHK(h, hipMemset(h->d_esb.p, 0, h->d_esb.n));
+    }
     HIPCHK(h, h->d_nlr.ensure(sizeof(int) * (size_t)nslot * R));
     size_t g = synth_b16_grid(nslot, R, h->nhp, std::max(h->njobs_main, h->njobs_acc));
     g = std::max(g, synth_b16_grid(nslot, 1, h->nhp, h->njobs_syn));
@@ -588,7 +596,7 @@ static int sh_create(dsm_sgt_t *h)
             for (int b = 0; b < 6; b++) par[(mu - 1) * 6 + b] = fpar[b] * ((mu & 1) ? -1 : 1);
         set_mirror(h, par);
     }
-    rc = alloc_b16(h, S, 36, 5, R);
+    rc = alloc_b16(h, S, 36, 5, R, -1);
     if (rc) return rc;
     HIPCHK(h, h->d_jobs.ensure(sizeof(SynthJob) * h->jobs.size()));
     HIPCHK(h, hipMemcpy(h->d_jobs.p, h->jobs.data(), sizeof(SynthJob) * h->jobs.size(), hipMemcpyHostToDevice));
@@ -958,14 +966,18 @@ static int psv_create(dsm_sgt_t *h)
             }
         set_mirror(h, par);
     }
-    rc = alloc_b16(h, S, PSV_NQ, PSV_NQS, R);
+    // P-SV batches write every degree up to maxlmax that a tile reads:
+    // coefficient rows, l = 0 rows, cut-off tails, zero rows below the cut
+    rc = alloc_b16(h, S, PSV_NQ, PSV_NQS, R, s.maxlmax + 1);
     if (rc) return rc;
     HIPCHK(h, h->d_jobs.ensure(sizeof(SynthJob) * h->jobs.size()));
     HIPCHK(h, hipMemcpy(h->d_jobs.p, h->jobs.data(), sizeof(SynthJob) * h->jobs.size(), hipMemcpyHostToDevice));
 
     TimedLaunch t;
     ev_begin(h, t, 1);
-    launch_legendre_psv(h->d_theta.as<double>(), s.theta_n, h->theta_pad, s.maxlmax, LC, h->d_tab.as<double>(),
+    HIPCHK(h, h->d_plm.ensure(sizeof(double) * 4 * (size_t)LC * h->theta_pad));
+    launch_legendre_psv(h->d_theta.as<double>(), s.theta_n, h->theta_pad, s.maxlmax, LC, h->d_plm.as<double>(),
+                        h->d_tab.as<double>(),
                         h->stream);
     HIPCHK(h, hipGetLastError());
     rc = make_mixed_tables(h, PSV_NPLANES);

@@ -45,7 +45,7 @@ void launch_psv_l0(const PsvL0Args &a, hipStream_t s);
 void launch_legendre_sh(const double *theta_rad, int theta_n, int theta_pad, int lmax, int lcap,
                         double *tab, hipStream_t s);
 // P-SV functions F0..F9 of |m| = 0, 1, 2 (legendre_lane.h): tab[plane][l][theta_pad]
-void launch_legendre_psv(const double *theta_rad, int theta_n, int theta_pad, int lmax, int lcap,
+void launch_legendre_psv(const double *theta_rad, int theta_n, int theta_pad, int lmax, int lcap, double *P,
                          double *tab, hipStream_t s);
 
 // ---- zero the l tail of the coefficient rows ------------------------------
@@ -103,6 +103,8 @@ void launch_zero_tail_b16(unsigned short *Eb, int nq, int ns, int nslot, int r_e
                           hipStream_t s);
 void launch_split_rows(const double2 *E, int nslot, int nq, int r_eff, int lcap, int ns, unsigned short *Eb,
                        hipStream_t s);
+// zero the degrees l in [lfrom, lcap) of nrows bf16 rows
+void launch_zero_pad_b16(unsigned short *Eb, size_t nrows, int lcap, int lfrom, hipStream_t s);
 void launch_split_table(const double *tab, int nplanes, int lcap, int theta_pad, int nh, int nhp, int ns,
                         unsigned short *Tb, hipStream_t s);
 size_t synth_b16_grid(int nslot, int r_eff, int nhp, int njobs);

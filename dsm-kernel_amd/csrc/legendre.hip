@@ -26,17 +26,58 @@ void launch_legendre_sh(const double *theta_rad, int theta_n, int theta_pad, int
                        theta_pad, lmax, lcap, tab);
 }
 
-__global__ void legendre_psv_kernel(const double *theta_rad, int theta_n, int theta_pad, int lmax, int lcap,
-                                    double *tab)
+// P-SV table in two passes: (1) per distance, the recurrences of
+// P_l^m (m = 0..3) over l, the only serial part, into P[m][l][theta];
+// (2) one thread per (l, distance): F0..F9 from those values
+// (legendre_psv_values, the per-degree work of legendre_psv_column).
+__global__ __launch_bounds__(64) void legendre_plm_kernel(const double *theta_rad, int theta_n, int lmax, int lcap,
+                                                          int theta_pad, double *P)
 {
-    legendre_psv_column(theta_rad, theta_n, theta_pad, lmax, lcap, tab, blockIdx.x * blockDim.x + threadIdx.x);
+    const int it = blockIdx.x * 64 + threadIdx.x;
+    if (it >= theta_n) return;
+    const double th = theta_rad[it];
+    if (th == 0.0 || th == DSM_PI) return;  // poles: no recurrence
+    const double x = cos(th);
+    double plm[4][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    const size_t pm = (size_t)lcap * theta_pad;
+    for (int l = 0; l <= lmax; l++) {
+        for (int m = 0; m <= (l < 3 ? l : 3); m++) calplm(l, m, x, plm[m]);
+        for (int m = 0; m < 4; m++) P[m * pm + (size_t)l * theta_pad + it] = plm[m][0];
+    }
 }
 
-void launch_legendre_psv(const double *theta_rad, int theta_n, int theta_pad, int lmax, int lcap, double *tab,
-                         hipStream_t s)
+__global__ __launch_bounds__(256) void legendre_psv_fill_kernel(const double *theta_rad, int theta_n, int theta_pad,
+                                                                int lmax, int lcap, const double *P, double *tab)
 {
-    hipLaunchKernelGGL(legendre_psv_kernel, dim3((theta_pad + 63) / 64), dim3(64), 0, s, theta_rad, theta_n,
-                       theta_pad, lmax, lcap, tab);
+    const int it = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int l = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (it >= theta_pad || l >= lcap) return;
+    const size_t plane = (size_t)lcap * theta_pad;
+    double f[3][10];
+    if (it >= theta_n || l > lmax) {
+        for (int mu = 0; mu < 3; mu++)
+            for (int k = 0; k < 10; k++) f[mu][k] = 0.0;
+    } else {
+        const double th = theta_rad[it];
+        const bool pole = (th == 0.0 || th == DSM_PI);
+        const double x = cos(th), s = sin(th);
+        const double sf = (double)(float)s, cf = (double)(float)(x / s);
+        double pl[4] = {0.0, 0.0, 0.0, 0.0};
+        if (!pole)
+            for (int m = 0; m < 4; m++) pl[m] = P[m * plane + (size_t)l * theta_pad + it];
+        legendre_psv_values(l, th, pole, x, s, sf, cf, pl, f);
+    }
+    legendre_psv_store(f, tab, plane, l, theta_pad, it);
+}
+
+// P: scratch of 4 lcap theta_pad doubles
+void launch_legendre_psv(const double *theta_rad, int theta_n, int theta_pad, int lmax, int lcap, double *P,
+                         double *tab, hipStream_t s)
+{
+    hipLaunchKernelGGL(legendre_plm_kernel, dim3((theta_n + 63) / 64), dim3(64), 0, s, theta_rad, theta_n, lmax, lcap,
+                       theta_pad, P);
+    hipLaunchKernelGGL(legendre_psv_fill_kernel, dim3((theta_pad + 63) / 64, (lcap + 3) / 4), dim3(256), 0, s,
+                       theta_rad, theta_n, theta_pad, lmax, lcap, P, tab);
 }
 
 }  // namespace dsm

@@ -102,6 +102,52 @@ __host__ __device__ inline int psv_plane(int mu, int f)
     return 4 + 10 * (mu - 1) + f;
 }
 
+// F0..F9 of degree l at one distance from pl[m] = P_l^m (m = 0..3; the
+// recurrence's values, zero for m > l), s = sin(theta), x = cos(theta).
+__host__ __device__ inline void legendre_psv_values(int l, double th, bool pole, double x, double s, double sf,
+                                                    double cf, const double pl[4], double f[3][10])
+{
+    for (int mu = 0; mu < 3; mu++)
+        for (int k = 0; k < 10; k++) f[mu][k] = 0.0;
+    if (pole) {
+        const double sign1 = (th == DSM_PI && (l & 1)) ? -1.0 : 1.0;
+        f[0][0] = (double)(float)sqrt((double)(2 * l + 1) / (4.0 * DSM_PI)) * sign1;
+        return;
+    }
+    const double xl2 = (double)l * (double)(l + 1);
+    for (int mu = 0; mu <= (l < 2 ? l : 2); mu++) {
+        double fact = 1.0;
+        if (mu != 0)
+            for (int i = l - mu + 1; i <= l + mu; i++) fact = fact * (double)i;
+        const double c = sqrt((double)(2 * l + 1) / (4.0 * DSM_PI) / fact);
+        const double P = pl[mu];
+        const double Pt = (double)mu * x / s * P + pl[mu + 1];
+        double *F = f[mu];
+        F[0] = c * P;
+        F[1] = c * Pt;
+        F[2] = (-x / s * Pt + (double)mu * (double)mu / (1 - x * x) * P - xl2 * P) * c;
+        F[3] = (-x / (1 - x * x) * P + 1.0 / s * Pt) * c;
+        F[4] = 1.0 / s * c * P;
+        F[5] = P * c / sf;
+        F[6] = Pt * c / sf;
+        F[7] = F[4] * cf;
+        F[8] = 1.0 / s * P * c / sf;
+        F[9] = c * Pt * cf;
+    }
+}
+
+// The 24 planes of degree l at distance column it.
+__host__ __device__ inline void legendre_psv_store(const double f[3][10], double *tab, size_t plane, int l,
+                                                   int theta_pad, int it)
+{
+    for (int k = 0; k < 4; k++) {
+        const int fk = (k == 3) ? 9 : k;
+        tab[(size_t)psv_plane(0, fk) * plane + (size_t)l * theta_pad + it] = f[0][fk];
+    }
+    for (int mu = 1; mu <= 2; mu++)
+        for (int k = 0; k < 10; k++) tab[(size_t)psv_plane(mu, k) * plane + (size_t)l * theta_pad + it] = f[mu][k];
+}
+
 __host__ __device__ inline void legendre_psv_column(const double *theta_rad, int theta_n, int theta_pad, int lmax,
                                                     int lcap, double *tab, int it)
 {
@@ -118,41 +164,12 @@ __host__ __device__ inline void legendre_psv_column(const double *theta_rad, int
     const double sf = (double)(float)s, cf = (double)(float)(x / s);
     double plm[4][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
     for (int l = 0; l <= lmax; l++) {
-        double f[3][10];
-        for (int mu = 0; mu < 3; mu++)
-            for (int k = 0; k < 10; k++) f[mu][k] = 0.0;
-        if (pole) {
-            const double sign1 = (th == DSM_PI && (l & 1)) ? -1.0 : 1.0;
-            f[0][0] = (double)(float)sqrt((double)(2 * l + 1) / (4.0 * DSM_PI)) * sign1;
-        } else {
+        if (!pole)
             for (int m = 0; m <= (l < 3 ? l : 3); m++) calplm(l, m, x, plm[m]);
-            const double xl2 = (double)l * (double)(l + 1);
-            for (int mu = 0; mu <= (l < 2 ? l : 2); mu++) {
-                double fact = 1.0;
-                if (mu != 0)
-                    for (int i = l - mu + 1; i <= l + mu; i++) fact = fact * (double)i;
-                const double c = sqrt((double)(2 * l + 1) / (4.0 * DSM_PI) / fact);
-                const double P = plm[mu][0];
-                const double Pt = (double)mu * x / sin(th) * P + plm[mu + 1][0];
-                double *F = f[mu];
-                F[0] = c * P;
-                F[1] = c * Pt;
-                F[2] = (-x / sin(th) * Pt + (double)mu * (double)mu / (1 - x * x) * P - xl2 * P) * c;
-                F[3] = (-x / (1 - x * x) * P + 1.0 / sin(th) * Pt) * c;
-                F[4] = 1.0 / sin(th) * c * P;
-                F[5] = P * c / sf;
-                F[6] = Pt * c / sf;
-                F[7] = F[4] * cf;
-                F[8] = 1.0 / sin(th) * P * c / sf;
-                F[9] = c * Pt * cf;
-            }
-        }
-        for (int k = 0; k < 4; k++) {
-            const int fk = (k == 3) ? 9 : k;
-            tab[(size_t)psv_plane(0, fk) * plane + (size_t)l * theta_pad + it] = f[0][fk];
-        }
-        for (int mu = 1; mu <= 2; mu++)
-            for (int k = 0; k < 10; k++) tab[(size_t)psv_plane(mu, k) * plane + (size_t)l * theta_pad + it] = f[mu][k];
+        const double pl[4] = {plm[0][0], plm[1][0], plm[2][0], plm[3][0]};
+        double f[3][10];
+        legendre_psv_values(l, th, pole, x, s, sf, cf, pl, f);
+        legendre_psv_store(f, tab, plane, l, theta_pad, it);
     }
     for (int l = lmax + 1; l < lcap; l++)
         for (int b = 0; b < PSV_NPLANES; b++) tab[(size_t)b * plane + (size_t)l * theta_pad + it] = 0.0;

@@ -113,6 +113,18 @@ __global__ void zero_tail_b16_kernel(u16 *Eb, int nq, int ns, int nslot, int R, 
     }
 }
 
+// zero the padding degrees l in [lfrom, lcap) of nrows rows (never written)
+__global__ void zero_pad_b16_kernel(u16 *Eb, size_t nrows, int lcap, int lfrom)
+{
+    const int np = lcap - lfrom;
+    const size_t total = nrows * (size_t)np;
+    for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += (size_t)gridDim.x * blockDim.x) {
+        const size_t row = idx / np;
+        const int l = lfrom + (int)(idx - row * np);
+        Eb[row * lcap + lperm(l, lcap)] = 0;
+    }
+}
+
 // tab [plane][lcap][theta_pad] (FP64) -> Tb [s][plane][nhp][lcap], columns
 // c < nh taken from distance c, the others zero
 __global__ void split_table_kernel(const double *tab, int nplanes, int lcap, int theta_pad, int nh, int nhp, int ns,
@@ -345,6 +357,12 @@ void launch_zero_tail_b16(unsigned short *Eb, int nq, int ns, int nslot, int r_e
                           hipStream_t s)
 {
     hipLaunchKernelGGL(zero_tail_b16_kernel, dim3(4096), dim3(256), 0, s, Eb, nq, ns, nslot, r_eff, lcap, nl, lend);
+}
+
+void launch_zero_pad_b16(unsigned short *Eb, size_t nrows, int lcap, int lfrom, hipStream_t s)
+{
+    if (lfrom >= lcap || nrows == 0) return;
+    hipLaunchKernelGGL(zero_pad_b16_kernel, dim3(2048), dim3(256), 0, s, Eb, nrows, lcap, lfrom);
 }
 
 void launch_split_table(const double *tab, int nplanes, int lcap, int theta_pad, int nh, int nhp, int ns,
