@@ -26,23 +26,24 @@ void launch_legendre_sh(const double *theta_rad, int theta_n, int theta_pad, int
                        theta_pad, lmax, lcap, tab);
 }
 
-// P-SV table in two passes: (1) per distance, the recurrences of
+// P-SV table in two passes: (1) per distance and m, the recurrence of
 // P_l^m (m = 0..3) over l, the only serial part, into P[m][l][theta];
 // (2) one thread per (l, distance): F0..F9 from those values
 // (legendre_psv_values, the per-degree work of legendre_psv_column).
 __global__ __launch_bounds__(64) void legendre_plm_kernel(const double *theta_rad, int theta_n, int lmax, int lcap,
                                                           int theta_pad, double *P)
 {
-    const int it = blockIdx.x * 64 + threadIdx.x;
+    // one lane per (distance, m): the recurrences of different m are independent
+    const int it = blockIdx.x * 64 + threadIdx.x, m = blockIdx.y;
     if (it >= theta_n) return;
     const double th = theta_rad[it];
     if (th == 0.0 || th == DSM_PI) return;  // poles: no recurrence
     const double x = cos(th);
-    double plm[4][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
-    const size_t pm = (size_t)lcap * theta_pad;
+    double plm[3] = {0, 0, 0};
+    double *Pm = P + (size_t)m * lcap * theta_pad + it;
     for (int l = 0; l <= lmax; l++) {
-        for (int m = 0; m <= (l < 3 ? l : 3); m++) calplm(l, m, x, plm[m]);
-        for (int m = 0; m < 4; m++) P[m * pm + (size_t)l * theta_pad + it] = plm[m][0];
+        if (l >= m) calplm(l, m, x, plm);
+        Pm[(size_t)l * theta_pad] = plm[0];
     }
 }
 
@@ -74,8 +75,8 @@ __global__ __launch_bounds__(256) void legendre_psv_fill_kernel(const double *th
 void launch_legendre_psv(const double *theta_rad, int theta_n, int theta_pad, int lmax, int lcap, double *P,
                          double *tab, hipStream_t s)
 {
-    hipLaunchKernelGGL(legendre_plm_kernel, dim3((theta_n + 63) / 64), dim3(64), 0, s, theta_rad, theta_n, lmax, lcap,
-                       theta_pad, P);
+    hipLaunchKernelGGL(legendre_plm_kernel, dim3((theta_n + 63) / 64, 4), dim3(64), 0, s, theta_rad, theta_n, lmax,
+                       lcap, theta_pad, P);
     hipLaunchKernelGGL(legendre_psv_fill_kernel, dim3((theta_pad + 63) / 64, (lcap + 3) / 4), dim3(256), 0, s,
                        theta_rad, theta_n, theta_pad, lmax, lcap, P, tab);
 }
