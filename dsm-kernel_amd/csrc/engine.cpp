@@ -174,7 +174,7 @@ struct dsm_sgt {
     // depth cut of the batched forms (0: off): per-row radius / wave speed,
     // per-slot angular frequency, first solved row per (slot, l block)
     double dcut = 0.0;
-    DevBuf d_rrow, d_vrow, d_omg, d_kstart, d_rlow, d_rneed, d_zs, d_loff, d_lsuf, d_llog;
+    DevBuf d_rrow, d_vrow, d_omg, d_kstart, d_rlow, d_rneed, d_zs, d_loff, d_lsuf, d_llog, d_wsave;
     // fused FP64 family (mixed forms): compact band and its selector bits
     DevBuf d_cband, d_csel;
     std::vector<SynthJob> jobs;
@@ -915,6 +915,7 @@ static int psv_create(dsm_sgt_t *h)
     HIPCHK(h, h->d_loff.ensure(sizeof(long long) * (size_t)S * (psv_solve_blocks(0, lch - 1) + 1) + 16));
     HIPCHK(h, h->d_gexc.ensure(sizeof(double2) * (size_t)S * 16 * LC));
     HIPCHK(h, h->d_ends.ensure(sizeof(double2) * 3 * nthreads));
+    HIPCHK(h, h->d_wsave.ensure(sizeof(double2) * 6 * nthreads));
     // batched forms: depth cut (DSM_SGT_DEPTH_CUT overrides; 0 disables)
     h->dcut = (h->precision == DSM_SGT_FP64) ? 0.0 : 40.0;
     if (const char *e = std::getenv("DSM_SGT_DEPTH_CUT")) h->dcut = std::max(0.0, std::atof(e));
@@ -1155,6 +1156,7 @@ static int psv_run_batch(dsm_sgt_t *h, const PsvPrep &p)
         sa.cband = h->d_cband.as<double2>();
         sa.csel = h->d_csel.as<int>();
         sa.zs = h->d_zs.as<double2>();
+        sa.wsave = h->d_wsave.as<double2>();
         sa.loff = nullptr;
         TimedLaunch t0;
         ev_begin(h, t0, 0);

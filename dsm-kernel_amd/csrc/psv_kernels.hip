@@ -369,6 +369,77 @@ __global__ __launch_bounds__(SV_NT) void psv_factor_kernel(PsvSolveArgs a)
         psv_factor_step<F>(W, k, n, e, crecip_t<F>(W.d0), w2, w3, w4);
         if (!CK) storeL(k, w2, w3, w4);
     }
+    // the window after the last row below the source, for psv_factor_top_kernel
+    // (the factors of the last three rows, which forward() of the first
+    // source rows would read, multiply forward values that are zero there)
+    {
+        const double2 wv[6] = {W.d0, W.e01, W.e02, W.d1, W.e12, W.d2};
+#pragma unroll
+        for (int c = 0; c < 6; c++) a.wsave[(size_t)c * ss + gid] = wv[c];
+    }
+}
+
+// Rows fs .. n of the factorization (the source rows and the end of the
+// band) with the forward values of the source solutions, from the window
+// psv_factor_kernel leaves below the source: a separate kernel so the long
+// sweep below runs with the few registers it needs.
+template <bool F, bool CK>
+__global__ __launch_bounds__(SV_NT) void psv_factor_top_kernel(PsvSolveArgs a)
+{
+    constexpr int CW = F ? 8 : 16;  // double2 per column
+    const int tid = threadIdx.x;
+    const int sl = a.slot_list[blockIdx.y];
+    const int lraw = a.l0 + blockIdx.x * SV_NT + tid;
+    const bool act = lraw <= a.lend && lraw != 0;
+    const int l = act ? lraw : max(1, a.lend);
+    const long gid = ((long)blockIdx.y * gridDim.x + blockIdx.x) * SV_NT + tid;
+    const long ss = a.sstride;
+    const double l2 = (double)l * (double)(l + 1);
+    const double lsq = sqrt(l2);
+    const int n = a.nn - a.row0 + 1;
+    const int fs = a.jtmp - a.row0 + 1;    // local row of the first source row (>= 3)
+    const int klo = a.rowlo - a.row0 + 1;  // lowest local row needed
+    const int nsrc = (l >= 2) ? 4 : 3;
+    // local column j (1..n) is band row row0+j-1
+    const double2 *band0 = (F ? a.cband : a.band) + ((size_t)sl * a.nn + (a.row0 - 1)) * CW;
+    const int *kind0 = (F ? a.csel : a.kind) + (a.row0 - 1);
+    const double2 *gx = a.gexc + (size_t)sl * 16 * a.lcap + l;
+    // first solved local row of this block (depth cut of the batched forms)
+    const int blk = blockIdx.y * gridDim.x + blockIdx.x;
+    const int kst = a.kstart ? a.kstart[blk] : 1;
+    // factor rows k >= kL at Lb[(3 (k - kL) + j) Lst + Lln], forward values
+    // at Zs[(4 (k - fs) + s) ss + gid]
+    double2 *Lb = a.loff ? a.scratch + a.loff[blk] : a.scratch;
+    const int kL = a.loff ? max(klo, kst) : klo;
+    const long Lst = a.loff ? SV_NT : ss, Lln = a.loff ? tid : gid;
+    double2 *Zs = a.zs;
+
+    // checkpointed storage (batched mixed forms): windows of the segments
+    // below the source, then the factors of rows xb = max(fs, kL) .. n
+    const int nseg = (CK && fs > kL) ? (fs - kL + PSV_CKPT - 1) / PSV_CKPT : 0;
+    const int xb = CK ? max(fs, kL) : kL;
+    double2 *Lx = CK ? Lb + (size_t)6 * nseg * SV_NT : Lb;
+    auto storeL = [&](int k, double2 w2, double2 w3, double2 w4) {
+        if (act && k >= klo && k >= xb) {
+            Lx[(3 * (k - xb) + 0) * Lst + Lln] = w2;
+            Lx[(3 * (k - xb) + 1) * Lst + Lln] = w3;
+            Lx[(3 * (k - xb) + 2) * Lst + Lln] = w4;
+        }
+    };
+    PsvWin W;
+    W.d0 = a.wsave[0 * ss + gid];
+    W.e01 = a.wsave[1 * ss + gid];
+    W.e02 = a.wsave[2 * ss + gid];
+    W.d1 = a.wsave[3 * ss + gid];
+    W.e12 = a.wsave[4 * ss + gid];
+    W.d2 = a.wsave[5 * ss + gid];
+    // factors of the rows below the source: forward() multiplies them by the
+    // zero forward values of rows below fs only
+    W.w4m3 = W.w4m2 = W.w4m1 = W.w3m2 = W.w3m1 = W.w2m1 = cz();
+    auto column = [&](int j, double2 e[4]) {
+        if (F) psv_col_fast(band0 + (size_t)(j - 1) * CW, kind0[j - 1], l2, lsq, e);
+        else psv_col_exact(band0 + (size_t)(j - 1) * CW, kind0[j - 1], l2, lsq, e);
+    };
     // phase 2: forward values of the source solutions at row k (their
     // right-hand sides sit at rows fs..fs+3), then the factorization step
     double2 z1[4], z2[4], z3[4];
@@ -704,13 +775,16 @@ void launch_psv_solve(const PsvSolveArgs &a, int nslots, bool fast, hipStream_t 
     if (fast) {
         if (a.ckpt && a.loff && a.kstart) {
             hipLaunchKernelGGL((psv_factor_kernel<true, true>), g1, dim3(SV_NT), 0, s, a);
+            hipLaunchKernelGGL((psv_factor_top_kernel<true, true>), g1, dim3(SV_NT), 0, s, a);
             hipLaunchKernelGGL(psv_back_ckpt_kernel, g2, dim3(64), 0, s, a);
         } else {
             hipLaunchKernelGGL((psv_factor_kernel<true, false>), g1, dim3(SV_NT), 0, s, a);
+            hipLaunchKernelGGL((psv_factor_top_kernel<true, false>), g1, dim3(SV_NT), 0, s, a);
             hipLaunchKernelGGL(psv_back_kernel<true>, g2, dim3(64), 0, s, a);
         }
     } else {
         hipLaunchKernelGGL((psv_factor_kernel<false, false>), g1, dim3(SV_NT), 0, s, a);
+        hipLaunchKernelGGL((psv_factor_top_kernel<false, false>), g1, dim3(SV_NT), 0, s, a);
         hipLaunchKernelGGL(psv_back_kernel<false>, g2, dim3(64), 0, s, a);
     }
 }

@@ -58,6 +58,7 @@ struct PsvSolveArgs {
     const double2 *cband;  // fused family: [slot][nn][4][A, B] (psv_compact_kernel)
     const int *csel;       // [nn]: bit ii set where entry ii scales with sqrt(l(l+1))
     double2 *zs;           // forward values: [k - fs][4][sstride]
+    double2 *wsave;        // [6][sstride]: the factorization window below the source (d0 .. d2 of PsvWin)
     // batched forms: factors of (active slot, l block) blk from scratch + loff[blk],
     // [k - max(klo, kstart)][3][256]; null: [k - klo][3][sstride]
     const long long *loff;
