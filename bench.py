@@ -192,6 +192,11 @@ def main():
         kernel, flops, peak = "synth_f64_kernel<0>", st.synth_flops / launches, FP64_MFMA_PEAK
         bound_note = "algorithmic flops, FP64 MFMA peak"
     achieved = flops / (launch_ms * 1e-3) if launch_ms > 0 else 0.0
+    # HBM bytes per launch of the dominant kernel, P-SV bf16x6 form: rocprofv3
+    # FETCH_SIZE and WRITE_SIZE per dispatch of synth_b16_kernel<3, 0> over
+    # this command's run, scaled by the counters of a plain 1 GiB copy
+    # (x1.991, x0.967); profiles/synth_b16_psv_5s_catalog_mixed.md
+    traffic = 2.378e10 if (args.mode == "psv" and args.form == "bf16x6") else None
     form_txt = {"bf16x6": "batched mixed: bf16x6 synthesis, FP64 solve with the depth cut",
                 "bf16x3": "batched mixed: bf16x3 synthesis, FP64 solve with the depth cut",
                 "fp32": "mixed: FP32 synthesis, FP64 solve with the depth cut", "fp64": "plain FP64"}[args.form]
@@ -218,7 +223,7 @@ def main():
         "config": {"workload": workload, "form": form_txt, "frequencies": p["imax"],
                    "lmax": p["maxlmax"], "band_rows": nn, "radii": r_n, "distances": th_n, "l_cut": cuts},
         "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
-                     "frac": achieved / peak, "traffic": None, "kernel": kernel, "avg_launch_ms": launch_ms,
+                     "frac": achieved / peak, "traffic": traffic, "kernel": kernel, "avg_launch_ms": launch_ms,
                      "flops_per_launch": flops, "note": bound_note},
         "stages_ms": {"solve+coef": st.solve_ms, "legendre": st.legendre_ms, "synth": st.synth_ms,
                       "other": st.other_ms},
