@@ -251,11 +251,11 @@ __global__ __launch_bounds__(SV_NT) void psv_factor_kernel(PsvSolveArgs a)
     __shared__ double2 sband[2][SV_CH * CW];
     __shared__ int skind[2][SV_CH];
     const int tid = threadIdx.x;
-    const int sl = a.slot_list[blockIdx.y];
-    const int lraw = a.l0 + blockIdx.x * SV_NT + tid;
+    const int sl = a.slot_list[blockIdx.x];
+    const int lraw = a.l0 + blockIdx.y * SV_NT + tid;
     const bool act = lraw <= a.lend && lraw != 0;
     const int l = act ? lraw : max(1, a.lend);
-    const long gid = ((long)blockIdx.y * gridDim.x + blockIdx.x) * SV_NT + tid;
+    const long gid = ((long)blockIdx.x * gridDim.y + blockIdx.y) * SV_NT + tid;
     const long ss = a.sstride;
     const double l2 = (double)l * (double)(l + 1);
     const double lsq = sqrt(l2);
@@ -268,7 +268,7 @@ __global__ __launch_bounds__(SV_NT) void psv_factor_kernel(PsvSolveArgs a)
     const int *kind0 = (F ? a.csel : a.kind) + (a.row0 - 1);
     const double2 *gx = a.gexc + (size_t)sl * 16 * a.lcap + l;
     // first solved local row of this block (depth cut of the batched forms)
-    const int blk = blockIdx.y * gridDim.x + blockIdx.x;
+    const int blk = blockIdx.x * gridDim.y + blockIdx.y;
     const int kst = a.kstart ? a.kstart[blk] : 1;
     // factor rows k >= kL at Lb[(3 (k - kL) + j) Lst + Lln], forward values
     // at Zs[(4 (k - fs) + s) ss + gid]
@@ -388,11 +388,11 @@ __global__ __launch_bounds__(SV_NT) void psv_factor_top_kernel(PsvSolveArgs a)
 {
     constexpr int CW = F ? 8 : 16;  // double2 per column
     const int tid = threadIdx.x;
-    const int sl = a.slot_list[blockIdx.y];
-    const int lraw = a.l0 + blockIdx.x * SV_NT + tid;
+    const int sl = a.slot_list[blockIdx.x];
+    const int lraw = a.l0 + blockIdx.y * SV_NT + tid;
     const bool act = lraw <= a.lend && lraw != 0;
     const int l = act ? lraw : max(1, a.lend);
-    const long gid = ((long)blockIdx.y * gridDim.x + blockIdx.x) * SV_NT + tid;
+    const long gid = ((long)blockIdx.x * gridDim.y + blockIdx.y) * SV_NT + tid;
     const long ss = a.sstride;
     const double l2 = (double)l * (double)(l + 1);
     const double lsq = sqrt(l2);
@@ -405,7 +405,7 @@ __global__ __launch_bounds__(SV_NT) void psv_factor_top_kernel(PsvSolveArgs a)
     const int *kind0 = (F ? a.csel : a.kind) + (a.row0 - 1);
     const double2 *gx = a.gexc + (size_t)sl * 16 * a.lcap + l;
     // first solved local row of this block (depth cut of the batched forms)
-    const int blk = blockIdx.y * gridDim.x + blockIdx.x;
+    const int blk = blockIdx.x * gridDim.y + blockIdx.y;
     const int kst = a.kstart ? a.kstart[blk] : 1;
     // factor rows k >= kL at Lb[(3 (k - kL) + j) Lst + Lln], forward values
     // at Zs[(4 (k - fs) + s) ss + gid]
@@ -505,12 +505,12 @@ __global__ __launch_bounds__(SV_NT) void psv_factor_top_kernel(PsvSolveArgs a)
 template <bool F>
 __global__ __launch_bounds__(64) void psv_back_kernel(PsvSolveArgs a)
 {
-    const int sub = blockIdx.x & 3, xb = blockIdx.x >> 2;
-    const int sl = a.slot_list[blockIdx.y];
+    const int sub = blockIdx.y & 3, xb = blockIdx.y >> 2;
+    const int sl = a.slot_list[blockIdx.x];
     const int lraw = a.l0 + xb * SV_NT + sub * 64 + threadIdx.x;
     if (lraw > a.lend || lraw == 0) return;
     const int l = lraw;
-    const long gid = ((long)blockIdx.y * (gridDim.x >> 2) + xb) * SV_NT + sub * 64 + threadIdx.x;
+    const long gid = ((long)blockIdx.x * (gridDim.y >> 2) + xb) * SV_NT + sub * 64 + threadIdx.x;
     const long ss = a.sstride;
     const double l2 = (double)l * (double)(l + 1);
     const double lsq = sqrt(l2);
@@ -520,7 +520,7 @@ __global__ __launch_bounds__(64) void psv_back_kernel(PsvSolveArgs a)
     const int nsrc = (l >= 2) ? 4 : 3;
     const double2 *Zs = a.zs;
     const double2 Dn1 = a.ends[gid], Dn = a.ends[ss + gid], w2n1 = a.ends[2 * ss + gid];
-    const int blk = blockIdx.y * (gridDim.x >> 2) + xb;
+    const int blk = blockIdx.x * (gridDim.y >> 2) + xb;
     const int kst = a.kstart ? a.kstart[blk] : 1;
     const int kbot = max(klo, kst);  // rows below kst are outside the solved system
     const double2 *Lb = a.loff ? a.scratch + a.loff[blk] : a.scratch;
@@ -644,12 +644,12 @@ __global__ __launch_bounds__(64) void psv_back_kernel(PsvSolveArgs a)
 __global__ __launch_bounds__(64) void psv_back_ckpt_kernel(PsvSolveArgs a)
 {
     constexpr bool F = true;
-    const int sub = blockIdx.x & 3, xb_ = blockIdx.x >> 2;
-    const int sl = a.slot_list[blockIdx.y];
+    const int sub = blockIdx.y & 3, xb_ = blockIdx.y >> 2;
+    const int sl = a.slot_list[blockIdx.x];
     const int lraw = a.l0 + xb_ * SV_NT + sub * 64 + threadIdx.x;
     if (lraw > a.lend || lraw == 0) return;
     const int l = lraw;
-    const long gid = ((long)blockIdx.y * (gridDim.x >> 2) + xb_) * SV_NT + sub * 64 + threadIdx.x;
+    const long gid = ((long)blockIdx.x * (gridDim.y >> 2) + xb_) * SV_NT + sub * 64 + threadIdx.x;
     const long ss = a.sstride;
     const double l2 = (double)l * (double)(l + 1);
     const double lsq = sqrt(l2);
@@ -659,7 +659,7 @@ __global__ __launch_bounds__(64) void psv_back_ckpt_kernel(PsvSolveArgs a)
     const int nsrc = (l >= 2) ? 4 : 3;
     const double2 *Zs = a.zs;
     const double2 Dn1 = a.ends[gid], Dn = a.ends[ss + gid], w2n1 = a.ends[2 * ss + gid];
-    const int blk = blockIdx.y * (gridDim.x >> 2) + xb_;
+    const int blk = blockIdx.x * (gridDim.y >> 2) + xb_;
     const int kL = max(klo, a.kstart[blk]);  // rows below kL are outside the solved system or not needed
     const int nseg = fs > kL ? (fs - kL + PSV_CKPT - 1) / PSV_CKPT : 0;
     const int xb = max(fs, kL);
@@ -771,7 +771,9 @@ int psv_solve_blocks(int l0, int lend) { return (lend - l0 + SV_NT) / SV_NT; }
 void launch_psv_solve(const PsvSolveArgs &a, int nslots, bool fast, hipStream_t s)
 {
     const int nb = psv_solve_blocks(a.l0, a.lend);
-    const dim3 g1(nb, nslots), g2(4 * nb, nslots);
+    // slots fastest: every slot's lowest (longest, full-depth) block of l is
+    // dispatched first
+    const dim3 g1(nslots, nb), g2(nslots, 4 * nb);
     if (fast) {
         if (a.ckpt && a.loff && a.kstart) {
             hipLaunchKernelGGL((psv_factor_kernel<true, true>), g1, dim3(SV_NT), 0, s, a);
@@ -789,7 +791,13 @@ void launch_psv_solve(const PsvSolveArgs &a, int nslots, bool fast, hipStream_t 
     }
 }
 
-__global__ __launch_bounds__(64) void psv_coef_kernel(PsvCoefArgs a) { psv_coef_lane(a, blockIdx.x, threadIdx.x); }
+// blocks of 64 l in l-major order (the low l, with every receiver's rows, first)
+__global__ __launch_bounds__(64) void psv_coef_kernel(PsvCoefArgs a)
+{
+    const int ns = gridDim.x / a.nlb;
+    const int b = (blockIdx.x % ns) * a.nlb + blockIdx.x / ns;
+    psv_coef_lane(a, b, threadIdx.x);
+}
 
 void launch_psv_coef(const PsvCoefArgs &a, int nblocks, hipStream_t s)
 {
