@@ -1080,7 +1080,9 @@ static int psv_run_batch(dsm_sgt_t *h, const PsvPrep &p)
     for (int k = 0; k < S; k++) active[k] = k;
     HIPCHK(h, hipMemcpyAsync(h->d_slots.p, active.data(), sizeof(int) * S, hipMemcpyHostToDevice, h->stream));
     const PsvSrcConst sc{s.ecC0, s.ecF0, s.ecL0, s.r0, s.ra[s.mtmp - 1], s.ra[s.mtmp]};
-    // l = 0 of every slot
+    // l = 0 of every slot (bf16 forms: after the coefficient kernel, whose
+    // first block stages zeros at l = 0)
+    PsvL0Args l0args{};
     {
         PsvL0Args za{};
         za.band = h->d_band.as<double2>();
@@ -1111,11 +1113,14 @@ static int psv_run_batch(dsm_sgt_t *h, const PsvPrep &p)
             za.eb = B16Rows{h->d_eb.as<unsigned short>(), h->ns, S, R, LC};
             za.esb = B16Rows{h->d_esb.as<unsigned short>(), h->ns, S, 1, LC};
         }
-        TimedLaunch t0;
-        ev_begin(h, t0, 0);
-        launch_psv_l0(za, h->stream);
-        HIPCHK(h, hipGetLastError());
-        ev_end(h, t0);
+        l0args = za;
+        if (!h->ns) {
+            TimedLaunch t0;
+            ev_begin(h, t0, 0);
+            launch_psv_l0(za, h->stream);
+            HIPCHK(h, hipGetLastError());
+            ev_end(h, t0);
+        }
     }
     const int n = nn - s.row0 + 1, klo = g.rowlo - s.row0 + 1, fs = s.jtmp - s.row0 + 1;
     std::vector<double2> ampuv((size_t)S * LC * 2);
@@ -1241,6 +1246,10 @@ static int psv_run_batch(dsm_sgt_t *h, const PsvPrep &p)
         }
         launch_psv_coef(ca, nblocks, h->stream);
         HIPCHK(h, hipGetLastError());
+        if (h->ns && l0 == 0) {
+            launch_psv_l0(l0args, h->stream);
+            HIPCHK(h, hipGetLastError());
+        }
         if (dev_cut) {
             PsvCutArgs cu{h->d_amp.as<double2>(), h->d_lsuf.as<int>(), S, LC, s.maxlmax, s.ratl, h->d_nl.as<int>(),
                           h->d_llog.as<int>()};

@@ -791,6 +791,100 @@ void launch_psv_solve(const PsvSolveArgs &a, int nslots, bool fast, hipStream_t 
     }
 }
 
+// Coefficient rows of the bf16 forms, one 64-lane wave per block of 64
+// degrees (l0 a multiple of 64).  A row's NS terms x (re, im) are staged in
+// LDS, the wave's 32 even and 32 odd degrees being contiguous in the two
+// halves of the permuted row, and written by 16 NS lanes as 16-byte
+// chunks: one store instruction per row where each lane would issue 2 NS
+// 2-byte stores.  Lanes past lend or at l = 0 stage zeros (l = 0 is
+// written afterwards by psv_l0_kernel; degrees above lmax are padding).
+namespace {
+// lane part of a staged row's store address: term s, re|im, half (even or
+// odd degrees) and 16-byte chunk of lane t (t < 16 NS), for receiver r
+__device__ __forceinline__ unsigned short *wave_row_base(const B16Rows &d, int r, int slot, int lw0, int t)
+{
+    const int sg = (t >> 2) % (4 * d.ns), c = t & 3;
+    const int pl = sg >> 1, hh = sg & 1, s = pl >> 1, reim = pl & 1;
+    const size_t M = (size_t)d.nslot * d.r_n * 2, m = ((size_t)r * d.nslot + slot) * 2 + reim;
+    return d.Eb + ((size_t)s * M + m) * d.lcap + (hh ? (d.lcap >> 1) : 0) + (lw0 >> 1) + 8 * c;
+}
+// stage row q's terms of lane t's value, then the wave writes the row
+// (lanes t < 16 NS, 16 bytes each at base + q NS M lcap)
+__device__ __forceinline__ void wave_put(const B16Rows &d, unsigned short (*st)[32], unsigned short *base, size_t qstride,
+                                         int q, double2 v, int t)
+{
+    unsigned short tr[3], ti[3];
+    bf16_terms(v.x, d.ns, tr);
+    bf16_terms(v.y, d.ns, ti);
+    const int h = t & 1, pos = t >> 1;
+    for (int s = 0; s < d.ns; s++) {
+        st[(s * 2 + 0) * 2 + h][pos] = tr[s];
+        st[(s * 2 + 1) * 2 + h][pos] = ti[s];
+    }
+    if (t < 16 * d.ns) {
+        const uint4 val = *(const uint4 *)&st[t >> 2][8 * (t & 3)];
+        *(uint4 *)(base + (size_t)q * qstride) = val;
+    }
+}
+}  // namespace
+
+__global__ __launch_bounds__(64) void psv_coef_b16_kernel(PsvCoefArgs a)
+{
+    __shared__ __attribute__((aligned(16))) unsigned short st[12][32];
+    const int nsl = gridDim.x / a.nlb;
+    const int b = (blockIdx.x % nsl) * a.nlb + blockIdx.x / nsl;  // l-major, as psv_coef_kernel
+    const int t = threadIdx.x;
+    const int sl = a.slot_list[b / a.nlb];
+    const int lw0 = a.l0 + (b % a.nlb) * 64;
+    const int lraw = lw0 + t;
+    const bool live = lraw <= a.lend && lraw != 0;
+    const int l = live ? lraw : max(1, min(lraw, a.lend));  // dead lanes read a valid degree, stage zeros
+    const int R = a.r_n, LC = a.lcap;
+    const double il = 1.0 / sqrt((double)l * (double)(l + 1));
+    const bool mu1 = true, mu2 = (l >= 2);
+    const unsigned char *rneed = a.rneed ? a.rneed + (size_t)((lw0 - a.l0) >> 8) * R : nullptr;
+    const double2 *X = a.xn + (size_t)sl * a.nneed * PSV_NSOL * LC + l;
+    auto xv = [&](int p, int s) { return X[(size_t)(PSV_NSOL * p + s) * LC]; };
+    if (live) {  // amplitude rows of the top receiver (calamp on imt 1, m = 0)
+        a.ampuv[((size_t)sl * LC + l) * 2 + 0] = xv(a.amp_i0, 0);
+        a.ampuv[((size_t)sl * LC + l) * 2 + 1] = xv(a.amp_i1, 0);
+    }
+    {
+        double2 us[PSV_NSOL], vs[PSV_NSOL], es[PSV_NQS];
+        for (int s = 0; s < PSV_NSOL; s++) {
+            us[s] = xv(a.syn_i0, s);
+            vs[s] = xv(a.syn_i1, s);
+        }
+        psv_synn_rows(us, vs, il, mu1, mu2, es);
+        unsigned short *base = wave_row_base(a.esb, 0, sl, lw0, t);
+        const size_t qs = (size_t)a.esb.ns * a.esb.nslot * 2 * a.esb.lcap;
+        for (int q = 0; q < PSV_NQS; q++) wave_put(a.esb, st, base, qs, q, live ? es[q] : cz(), t);
+    }
+    const size_t qstride = (size_t)a.eb.ns * a.eb.nslot * a.eb.r_n * 2 * a.eb.lcap;
+    const int kst = a.kstart ? a.kstart[b / a.nlb * a.nblk + ((lw0 - a.l0) >> 8)] : 1;
+    for (int r = 0; r < R; r++) {
+        if (rneed && !rneed[r]) continue;  // below the solved rows of every slot the tile holds
+        unsigned short *base = wave_row_base(a.eb, r, sl, lw0, t);
+        if (a.kstart && kst > a.rlow[r]) {  // below this block's solved rows: zero rows
+            for (int q = 0; q < PSV_NQ; q++) wave_put(a.eb, st, base, qstride, q, cz(), t);
+            continue;
+        }
+        const int *rr = a.rec_row + 6 * r;
+        const double *w0 = a.w0 + 3 * r, *w1 = a.w1 + 3 * r;
+        PsvVals z[PSV_NSOL];
+        for (int s = 0; s < PSV_NSOL; s++) {
+            const double2 g0 = xv(rr[0], s), g1 = xv(rr[1], s), g2 = xv(rr[2], s), g3 = xv(rr[3], s),
+                          g4 = xv(rr[4], s), g5 = xv(rr[5], s);
+            z[s].U = cadd(cadd(cscale(g0, w0[0]), cscale(g2, w0[1])), cscale(g4, w0[2]));
+            z[s].V = cadd(cadd(cscale(g1, w0[0]), cscale(g3, w0[1])), cscale(g5, w0[2]));
+            z[s].Ud = cadd(cadd(cscale(g0, w1[0]), cscale(g2, w1[1])), cscale(g4, w1[2]));
+            z[s].Vd = cadd(cadd(cscale(g1, w1[0]), cscale(g3, w1[1])), cscale(g5, w1[2]));
+        }
+        psv_rows(z, a.fluid[r] != 0, a.fluidK[(size_t)sl * R + r], il, a.irf[r], mu1, mu2,
+                 [&](int q, double2 v) { wave_put(a.eb, st, base, qstride, q, live ? v : cz(), t); });
+    }
+}
+
 // blocks of 64 l in l-major order (the low l, with every receiver's rows, first)
 __global__ __launch_bounds__(64) void psv_coef_kernel(PsvCoefArgs a)
 {
@@ -801,7 +895,10 @@ __global__ __launch_bounds__(64) void psv_coef_kernel(PsvCoefArgs a)
 
 void launch_psv_coef(const PsvCoefArgs &a, int nblocks, hipStream_t s)
 {
-    hipLaunchKernelGGL(psv_coef_kernel, dim3(nblocks), dim3(64), 0, s, a);
+    if (a.eb.Eb && a.esb.Eb && (a.l0 % 64) == 0 && a.eb.ns <= 3)
+        hipLaunchKernelGGL(psv_coef_b16_kernel, dim3(nblocks), dim3(64), 0, s, a);
+    else
+        hipLaunchKernelGGL(psv_coef_kernel, dim3(nblocks), dim3(64), 0, s, a);
 }
 
 // l = 0: one 64-lane workgroup per frequency, the phases of psv_lane.h
