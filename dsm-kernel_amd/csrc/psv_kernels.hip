@@ -24,9 +24,12 @@ __global__ __launch_bounds__(64) void psv_excite_kernel(PsvSolveArgs a)
         for (int i = 0; i < 4; i++) o[(size_t)(4 * s + i) * a.lcap] = g[s][i];
 }
 
+// One wave per (slot, block of 256 l): the lanes take 64 rows at a time
+// going down from the source, form their rows' increments of D, and a
+// wave prefix sum finds the first row where D passes dcut.
 __global__ __launch_bounds__(64) void psv_depth_kernel(PsvDepthArgs a)
 {
-    const int t = blockIdx.x * 64 + threadIdx.x;
+    const int t = blockIdx.x, lane = threadIdx.x;
     if (t >= a.nslots * a.nblk) return;
     const int k = t / a.nblk, b = t - k * a.nblk;
     const int sl = a.slot_list[k];
@@ -34,21 +37,32 @@ __global__ __launch_bounds__(64) void psv_depth_kernel(PsvDepthArgs a)
     const double al = (double)max(1, a.l0 + 256 * b) + 0.5;
     // walk down from the source row; keep at least three rows below it
     int kst = a.row0;
-    double D = 0.0;
-    for (int j = a.jtmp - 1; j >= a.row0; j--) {  // rows j (1-based), j+1
-        const double rm = 0.5 * (a.rrow[j - 1] + a.rrow[j]);
-        const double vm = 0.5 * (a.vrow[j - 1] + a.vrow[j]);
-        const double dr = a.rrow[j] - a.rrow[j - 1];
-        const double q = al / rm, p = om / vm;
-        const double k2 = q * q - p * p;
-        if (k2 > 0.0 && dr > 0.0) D += fmin(sqrt(k2) * dr, a.dcap);
-        if (D > a.dcut) {
-            kst = j;
+    double carry = 0.0;
+    for (int j0 = a.jtmp - 1; j0 >= a.row0; j0 -= 64) {
+        const int j = j0 - lane;  // rows j (1-based), j+1
+        double c = 0.0;
+        if (j >= a.row0) {
+            const double rm = 0.5 * (a.rrow[j - 1] + a.rrow[j]);
+            const double vm = 0.5 * (a.vrow[j - 1] + a.vrow[j]);
+            const double dr = a.rrow[j] - a.rrow[j - 1];
+            const double q = al / rm, p = om / vm;
+            const double k2 = q * q - p * p;
+            if (k2 > 0.0 && dr > 0.0) c = fmin(sqrt(k2) * dr, a.dcap);
+        }
+        double ps = c;  // inclusive prefix sum, lane 0 the row nearest the source
+        for (int d = 1; d < 64; d <<= 1) {
+            const double o = __shfl_up(ps, d, 64);
+            if (lane >= d) ps += o;
+        }
+        const unsigned long long over = __ballot(j >= a.row0 && carry + ps > a.dcut);
+        if (over) {
+            kst = j0 - (__ffsll((long long)over) - 1);
             break;
         }
+        carry += __shfl(ps, 63, 64);
     }
     kst = min(kst, a.jtmp - 3);
-    a.kstart[t] = max(1, kst - a.row0 + 1);
+    if (lane == 0) a.kstart[t] = max(1, kst - a.row0 + 1);
 }
 
 __global__ __launch_bounds__(64) void psv_need_kernel(PsvNeedArgs a)
@@ -110,20 +124,28 @@ void launch_psv_loff(const int *kstart, int nblocks, int n, int klo, int fs, int
 // host form is calamp in engine.cpp) the entries m != 0 have amplitude 0:
 // they never raise the running maximum (which starts at -1 and so is
 // max(0, amplitudes so far)), their ratio is 0, so they count as small
-// exactly when 0 < ratl and l >= lsuf.  The lanes form their degree's amplitude,
-// the running maximum through it (wave prefix maximum) and the m = 0 test;
-// lane 0 replays the integer count from the 64-bit mask of those tests.
+// exactly when 0 < ratl and l >= lsuf.  The lanes form their degree's
+// amplitude, the running maximum through it (wave prefix maximum) and the
+// m = 0 test f0.  A degree then maps the count x before its entries to
+//   x + 2 side + 1   if f0 (which implies l >= lsuf),
+//   side             if l >= lsuf (and 0 < ratl) but not f0,
+//   0                otherwise          (side: entries m < 0, 2 or 1 at l = 1),
+// maps of the form "set c" or "add k", which compose associatively: a wave
+// scan gives the count before every degree, and the cut-off is the first
+// degree that starts with a count above 20.
 __global__ __launch_bounds__(64) void psv_cut_kernel(PsvCutArgs a)
 {
     const int k = blockIdx.x, tx = threadIdx.x;
     const int lsuf = a.lsuf[k];
-    int ismall = 0, nl = a.maxlmax + 1, llog = a.maxlmax;
     const bool zero_small = 0.0 < a.ratl;  // a ratio of 0 counts as small
+    int nl = a.maxlmax + 1, llog = a.maxlmax;
     double carry = 0.0;  // running maximum over the degrees of earlier chunks (and the m = -1 entry of l = 1)
+    int xin = 0;         // count before the chunk
     for (int l0 = 1; l0 <= a.maxlmax; l0 += 64) {
         const int l = l0 + tx;
+        const bool in = l <= a.maxlmax;
         double a0 = 0.0;
-        if (l <= a.maxlmax) {
+        if (in) {
             const double2 u = a.amp[((size_t)k * a.lcap + l) * 2], v = a.amp[((size_t)k * a.lcap + l) * 2 + 1];
             const double a1 = hypot(u.x, u.y), a2 = hypot(v.x, v.y);
             a0 = sqrt(a1 * a1 + a2 * a2);
@@ -135,26 +157,32 @@ __global__ __launch_bounds__(64) void psv_cut_kernel(PsvCutArgs a)
         }
         const double mx = fmax(carry, pm);
         const double ratio = (a0 != 0.0 && mx != 0.0) ? a0 / mx : 0.0;
-        const unsigned long long small0 = __ballot(l <= a.maxlmax && ratio < a.ratl && l >= lsuf);
-        carry = fmax(carry, __shfl(pm, 63, 64));
-        int stop = 0;
-        if (tx == 0) {
-            for (int t = 0; t < 64 && l0 + t <= a.maxlmax; t++) {
-                const int ll = l0 + t;
-                if (ismall > 20) {
-                    llog = min(llog, ll);
-                    nl = ll;
-                    stop = 1;
-                    break;
-                }
-                const bool s = zero_small && ll >= lsuf, f0 = (small0 >> t) & 1ull;
-                const int side = (ll >= 2) ? 2 : 1;  // entries m < 0 (and m > 0)
-                ismall = s ? ismall + side : 0;
-                ismall = f0 ? ismall + 1 : 0;
-                ismall = s ? ismall + side : 0;
+        const bool f0 = in && ratio < a.ratl && l >= lsuf;
+        const bool sm = in && zero_small && l >= lsuf;
+        const int side = (l >= 2) ? 2 : 1;
+        // this degree's map; lanes past maxlmax: identity (add 0)
+        bool set = in && !f0;
+        int val = f0 ? 2 * side + 1 : (sm ? side : 0);
+        for (int d = 1; d < 64; d <<= 1) {  // inclusive scan of "earlier, then this"
+            const bool os = __shfl_up(set, d, 64);
+            const int ov = __shfl_up(val, d, 64);
+            if (tx >= d) {
+                val = set ? val : ov + val;
+                set = set || os;
             }
         }
-        if (__shfl(stop, 0, 64)) break;
+        const int after = set ? val : xin + val;  // count after this degree's entries
+        const int before0 = __shfl_up(after, 1, 64);
+        const int before = (tx == 0) ? xin : before0;
+        const unsigned long long stop = __ballot(in && before > 20);
+        if (stop) {
+            const int ll = l0 + (__ffsll((long long)stop) - 1);
+            llog = min(llog, ll);
+            nl = ll;
+            break;
+        }
+        carry = fmax(carry, __shfl(pm, 63, 64));
+        xin = __shfl(after, 63, 64);
     }
     if (tx == 0) {
         a.nl[k] = nl;
@@ -190,8 +218,7 @@ void launch_psv_nlr(const int *kstart, const int *rlow, const int *nl, int nslot
 
 void launch_psv_depth(const PsvDepthArgs &a, hipStream_t s)
 {
-    const int n = a.nslots * a.nblk;
-    hipLaunchKernelGGL(psv_depth_kernel, dim3((n + 63) / 64), dim3(64), 0, s, a);
+    hipLaunchKernelGGL(psv_depth_kernel, dim3(a.nslots * a.nblk), dim3(64), 0, s, a);
 }
 
 // ---------------------------------------------------------------------------
