@@ -108,6 +108,25 @@ def catalog_errors(got_all, ref, freqs):
     return worst
 
 
+# HBM bytes of the solve stage (excitation, band factorization and
+# substitutions, coefficient rows, l = 0 systems, cut-off, depth cut) over
+# the whole 5 s P-SV catalog: rocprofv3 FETCH_SIZE + WRITE_SIZE summed over
+# the stage's dispatches, scaled by a plain copy's counters (x1.9906,
+# x0.9664); profiles/band_solve_psv_5s_catalog.md
+SOLVE_STAGE_BYTES = {("psv", "bf16x6"): 1.9994e11, ("psv", "fp64"): 1.6449e12}
+
+
+def band_solve(mode, form, st, full=True):
+    """HBM throughput of the solve stage: its rocprof-measured bytes over
+    the stage's live device time (HIP events on the engine stream)."""
+    out = {"systems": st.systems, "stage_ms": st.solve_ms, "peak": HBM_PEAK / 1e9, "unit": "GB/s"}
+    b = SOLVE_STAGE_BYTES.get((mode, form)) if full else None
+    if b is not None and st.solve_ms > 0:
+        out.update(hbm_bytes=b, achieved=b / (st.solve_ms * 1e-3) / 1e9,
+                   frac=b / (st.solve_ms * 1e-3) / HBM_PEAK)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -119,6 +138,10 @@ def main():
     ap.add_argument("--no-fp64", action="store_true", help="skip the plain FP64 reference run")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline sample")
     ap.add_argument("--imax", type=int, default=None, help="(testing) fewer frequencies")
+    ap.add_argument("--no-tune", action="store_true", help="default launch plan instead of the autotuned one")
+    ap.add_argument("--tune-cache", default=os.path.join(os.environ.get("XDG_CACHE_HOME") or
+                                                         os.path.expanduser("~/.cache"), "dsm_sgt_tune.txt"),
+                    help="autotuner cache file (plans keyed by input, form and device)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only; the catalog's frequencies are independent "
@@ -133,11 +156,22 @@ def main():
     mode = dsm_sgt.PSV if args.mode == "psv" else dsm_sgt.SH
     prec = getattr(dsm_sgt, FORMS[args.form][0])
 
+    # launch plan: the library's autotuner, once per input shape and form
+    # (cached; not part of the timed steps)
+    plan, tune = None, None
+    if not args.no_tune:
+        os.makedirs(os.path.dirname(os.path.abspath(args.tune_cache)), exist_ok=True)
+        t0 = time.perf_counter()
+        plan, probe_ms = dsm_sgt.autotune(inp, mode, prec, cache_path=args.tune_cache)
+        tune = {"plan": plan.as_dict(), "probe_ms": probe_ms, "tune_s": time.perf_counter() - t0,
+                "cached": probe_ms == 0.0}
+        print("autotune: %s" % tune, file=sys.stderr, flush=True)
+
     detail = []
 
     def step(pr):
         t0 = time.perf_counter()
-        eng = dsm_sgt.Engine(inp, mode=mode, precision=pr)
+        eng = dsm_sgt.Engine(inp, mode=mode, precision=pr, tuning=plan if pr == prec else None)
         t1 = time.perf_counter()
         eng.run(1, p["imax"])  # returns with the device synchronised
         st = eng.stats()
@@ -175,6 +209,7 @@ def main():
         errs = catalog_errors(got_all, ref, freqs)
         fp64 = {"wall_s": fp64_s, "speedup": fp64_s / (ms / 1e3),
                 "stages_ms": {"solve+coef": fst.solve_ms, "synth": fst.synth_ms, "other": fst.other_ms},
+                "band_solve": band_solve(args.mode, "fp64", fst, not args.imax),
                 "max_rel_l2_vs_fp64": errs, "error_frequencies": freqs, "bar": 1e-6}
         ref.close()
 
@@ -196,7 +231,7 @@ def main():
     # FETCH_SIZE and WRITE_SIZE per dispatch of synth_b16_kernel<3, 0> over
     # this command's run, scaled by the counters of a plain 1 GiB copy
     # (x1.991, x0.967); profiles/synth_b16_psv_5s_catalog_mixed.md
-    traffic = 2.378e10 if (args.mode == "psv" and args.form == "bf16x6") else None
+    traffic = 2.351e10 if (args.mode == "psv" and args.form == "bf16x6") else None
     form_txt = {"bf16x6": "batched mixed: bf16x6 synthesis, FP64 solve with the depth cut",
                 "bf16x3": "batched mixed: bf16x3 synthesis, FP64 solve with the depth cut",
                 "fp32": "mixed: FP32 synthesis, FP64 solve with the depth cut", "fp64": "plain FP64"}[args.form]
@@ -228,8 +263,10 @@ def main():
         "stages_ms": {"solve+coef": st.solve_ms, "legendre": st.legendre_ms, "synth": st.synth_ms,
                       "other": st.other_ms},
         "steps_detail": steps_detail,
-        "solve": {"systems": st.systems, "bytes_design": st.solve_bytes},
+        "band_solve": band_solve(args.mode, args.form, st, not args.imax),
     }
+    if tune is not None:
+        out["autotune"] = tune
     if fp64 is not None:
         out["plain_fp64"] = fp64
     if not args.no_cpu:

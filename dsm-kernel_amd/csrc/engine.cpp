@@ -153,6 +153,10 @@ struct dsm_sgt {
     std::vector<double> radii, theta_deg;
     // geometry of the device arrays
     int theta_pad = 0, lcap = 0, nneed = 0, nslot = 0, lchunk = 0;
+    // launch plan: the caller's request (0 = default) and the tile grouping
+    // and graph switch in force (xg 0: default grouping, -1: no tile list)
+    dsm_sgt_tuning_t tune{};
+    int xg = 0, graph = 0;
     int ipos0 = 0, npos = 0;  // catalog covers frequencies [ipos0, ipos0+npos)
     std::vector<int> need, rec_slot;
     std::vector<int> llog;    // per position: the reference's logged llog-1
@@ -177,6 +181,9 @@ struct dsm_sgt {
     DevBuf d_rrow, d_vrow, d_omg, d_kstart, d_rlow, d_rneed, d_zs, d_loff, d_lsuf, d_llog, d_wsave;
     // fused FP64 family (mixed forms): compact band and its selector bits
     DevBuf d_cband, d_csel;
+    // graph mode: the executable graphs of the current batch (destroyed once
+    // the batch has synchronised)
+    std::vector<hipGraphExec_t> gexec;
     std::vector<SynthJob> jobs;
     int njobs_main = 0, njobs_syn = 0, njobs_acc = 0;  // launch order: main, accumulate, SYNN
     PsvPrep prep[2];
@@ -348,7 +355,7 @@ static int b16_tile_order(dsm_sgt_t *h, const std::vector<int> &nlr, int S, int 
     // Chunks are dealt to the least loaded XCD; slots past an XCD's queue
     // hold -1 (no tile).
     constexpr int NXCD = 8;
-    const int XG = std::max(1, 32 / nx);
+    const int XG = h->xg > 0 ? h->xg : std::max(1, 32 / nx);
     auto order = [&](int j0, int nj, std::vector<int> &out) {
         std::vector<int> groups;
         std::vector<double> w(nj * ny);
@@ -526,8 +533,8 @@ static int sh_create(dsm_sgt_t *h)
     if (k < 1 || k + 1 > s.nn) return fail(h, DSM_SGT_EINVAL, "source node outside the band");
 
     // geometry of the batches
-    h->nslot = 32;
-    h->lchunk = 512;
+    h->nslot = h->tune.nslot > 0 ? h->tune.nslot : 32;
+    h->lchunk = h->tune.lchunk > 0 ? std::max(64, h->tune.lchunk) : 512;
     h->ipos0 = std::max(0, h->in.imin);
     h->npos = std::max(0, h->in.imax - h->ipos0 + 1);
     h->llog.assign(h->npos, 0);
@@ -834,9 +841,12 @@ static int psv_create(dsm_sgt_t *h)
     // which leaves room for four times more frequencies per batch
     h->nslot = (h->precision == DSM_SGT_FP64) ? 32 : 128;
     if (const char *e = std::getenv("DSM_SGT_NSLOT")) h->nslot = std::max(1, std::atoi(e));
-    // one solve launch covers the whole l range unless told otherwise
+    if (h->tune.nslot > 0) h->nslot = h->tune.nslot;
+    // one solve launch covers the whole l range unless told otherwise (the
+    // batched forms need it: the cut-off is replayed on the device)
     h->lchunk = s.maxlmax + 1;
     if (const char *e = std::getenv("DSM_SGT_LCHUNK")) h->lchunk = std::max(64, std::atoi(e));
+    if (h->tune.lchunk > 0 && h->precision == DSM_SGT_FP64) h->lchunk = std::max(64, h->tune.lchunk);
     h->ipos0 = std::max(0, h->in.imin);
     h->npos = std::max(0, h->in.imax - h->ipos0 + 1);
     h->llog.assign(h->npos, 0);
@@ -1050,6 +1060,75 @@ static int psv_device_band(dsm_sgt_t *h, const std::vector<double2> &scal, int S
     return DSM_SGT_OK;
 }
 
+// Graph mode: the launches between begin_capture and end_capture_launch are
+// recorded on the engine stream (kernels only: no memset or copy nodes) and
+// run as one graph.  The executable graph lives until the batch has
+// synchronised (free_graphs).
+struct CaptureGuard {  // ends a capture left open by an early return
+    hipStream_t s = nullptr;
+    bool on = false;
+    ~CaptureGuard()
+    {
+        if (!on) return;
+        hipGraph_t g = nullptr;
+        (void)hipStreamEndCapture(s, &g);
+        if (g) (void)hipGraphDestroy(g);
+    }
+};
+
+static int begin_capture(dsm_sgt_t *h, CaptureGuard &cg)
+{
+    HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeRelaxed));
+    cg.s = h->stream;
+    cg.on = true;
+    return DSM_SGT_OK;
+}
+
+static int end_capture_launch(dsm_sgt_t *h, CaptureGuard &cg)
+{
+    hipGraph_t g = nullptr;
+    cg.on = false;
+    const hipError_t le = hipGetLastError();
+    HIPCHK(h, hipStreamEndCapture(h->stream, &g));
+    HIPCHK(h, le);
+    hipGraphExec_t x = nullptr;
+    const hipError_t e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    HIPCHK(h, e);
+    h->gexec.push_back(x);
+    HIPCHK(h, hipGraphLaunch(x, h->stream));
+    return DSM_SGT_OK;
+}
+
+static void free_graphs(dsm_sgt_t *h)
+{
+    for (hipGraphExec_t x : h->gexec) (void)hipGraphExecDestroy(x);
+    h->gexec.clear();
+}
+
+// l extent per (receiver, slot) row from the host's view of a batch: the
+// cut-off nl, and the leading blocks of 256 degrees whose solved rows
+// (first solved row kst per block) reach the receiver's stencil.  The
+// tile order is built from it; the kernels use the device's own values.
+static void psv_nlr_estimate(const PsvSetup &s, const std::vector<int> &nl, const std::vector<int> &kst_host,
+                             int kst_nblk, int S, std::vector<int> &nlr)
+{
+    const int R = s.r_n;
+    nlr.assign((size_t)R * S, 0);
+    for (int r = 0; r < R; r++) {
+        const int rlow = s.ksta[r] - s.row0;  // local row of ksta-1
+        for (int k = 0; k < S; k++) {
+            int lr = nl[k];
+            if (!kst_host.empty()) {
+                int nb = 0;
+                while (nb < kst_nblk && kst_host[(size_t)k * kst_nblk + nb] <= rlow) nb++;
+                lr = std::min(lr, 256 * nb);
+            }
+            nlr[(size_t)r * S + k] = lr;
+        }
+    }
+}
+
 static int psv_run_batch(dsm_sgt_t *h, const PsvPrep &p)
 {
     const PsvSetup &s = h->psv;
@@ -1123,6 +1202,13 @@ static int psv_run_batch(dsm_sgt_t *h, const PsvPrep &p)
         }
     }
     const int n = nn - s.row0 + 1, klo = g.rowlo - s.row0 + 1, fs = s.jtmp - s.row0 + 1;
+    // graph mode (BF16 forms with the depth cut, whole l range in one
+    // launch): everything after the factor-storage read-back runs as two
+    // graphs, the solve-to-row-extent sequence and the synthesis pair
+    const bool gmode = h->graph && h->ns && h->dcut > 0.0 && h->d_lsuf.p && h->lchunk > s.maxlmax;
+    CaptureGuard cg;
+    TimedLaunch t_solve;  // events of the current solve launch
+    std::vector<int> nlr_est;
     std::vector<double2> ampuv((size_t)S * LC * 2);
     std::vector<int> kst_host;  // depth cut per (slot, block of 256 l), single-launch case
     int kst_nblk = 0;
@@ -1163,7 +1249,7 @@ static int psv_run_batch(dsm_sgt_t *h, const PsvPrep &p)
         sa.zs = h->d_zs.as<double2>();
         sa.wsave = h->d_wsave.as<double2>();
         sa.loff = nullptr;
-        TimedLaunch t0;
+        TimedLaunch &t0 = t_solve;
         ev_begin(h, t0, 0);
         if (h->precision != DSM_SGT_FP64 && l0 == 0)
             launch_psv_compact(h->d_band.as<double2>(), h->d_kind.as<int>(), nn, h->d_slots.as<int>(),
@@ -1202,6 +1288,17 @@ static int psv_run_batch(dsm_sgt_t *h, const PsvPrep &p)
                 sa.loff = loff;
             } else if (h->precision != DSM_SGT_FP64) {
                 return fail(h, DSM_SGT_EUNSUPPORTED, "the depth cut needs one solve launch per batch (DSM_SGT_LCHUNK)");
+            }
+            if (gmode) {
+                // the synthesis tile order needs only the read-back depth
+                // cut: built and copied before the capture
+                psv_nlr_estimate(s, std::vector<int>(S, s.maxlmax + 1), kst_host, kst_nblk, S, nlr_est);
+                if (h->xg >= 0) {
+                    const int rc = b16_tile_order(h, nlr_est, S, R);
+                    if (rc) return rc;
+                }
+                const int rc = begin_capture(h, cg);
+                if (rc) return rc;
             }
             if (h->ns && l0 == 0 && lend == s.maxlmax) {
                 PsvNeedArgs na{h->d_kstart.as<int>(), h->d_rlow.as<int>(), (int)active.size(), da.nblk, R,
@@ -1256,7 +1353,7 @@ static int psv_run_batch(dsm_sgt_t *h, const PsvPrep &p)
             launch_psv_cut(cu, h->stream);
             HIPCHK(h, hipGetLastError());
         }
-        ev_end(h, t0);
+        if (!gmode) ev_end(h, t0);
         h->stats.solve_launches++;
         const long nsys = (long)active.size() * (lend - std::max(l0, 1) + 1);
         h->stats.systems += nsys;
@@ -1313,9 +1410,10 @@ static int psv_run_batch(dsm_sgt_t *h, const PsvPrep &p)
         }
     }
     if (!dev_cut) HIPCHK(h, hipMemcpyAsync(h->d_nl.p, nl.data(), sizeof(int) * S, hipMemcpyHostToDevice, h->stream));
-    HIPCHK(h, hipMemcpyAsync(h->d_slots.p, all.data(), sizeof(int) * S, hipMemcpyHostToDevice, h->stream));
+    // (one launch over the whole l range: the slot list is already all slots)
+    if (!gmode) HIPCHK(h, hipMemcpyAsync(h->d_slots.p, all.data(), sizeof(int) * S, hipMemcpyHostToDevice, h->stream));
     TimedLaunch tz;
-    ev_begin(h, tz, 3);
+    if (!gmode) ev_begin(h, tz, 3);
     int rc = DSM_SGT_OK;
     if (h->ns) {
         // bf16 rows written by the coefficient kernels: zero the cut-off
@@ -1325,33 +1423,32 @@ static int psv_run_batch(dsm_sgt_t *h, const PsvPrep &p)
                              h->stream);
         launch_zero_tail_b16(h->d_esb.as<unsigned short>(), PSV_NQS, h->ns, S, 1, LC, h->d_nl.as<int>(), s.maxlmax,
                              h->stream);
-        std::vector<int> nlr((size_t)R * S);  // host estimate (the cut-off taken from nl) for the tile order
-        for (int r = 0; r < R; r++) {
-            const int rlow = s.ksta[r] - s.row0;  // local row of ksta-1
-            for (int k = 0; k < S; k++) {
-                int lr = nl[k];
-                if (!kst_host.empty()) {
-                    int nb = 0;
-                    while (nb < kst_nblk && kst_host[(size_t)k * kst_nblk + nb] <= rlow) nb++;
-                    lr = std::min(lr, 256 * nb);
-                }
-                nlr[(size_t)r * S + k] = lr;
-            }
-        }
-        if (h->dcut > 0.0 && !kst_host.empty())
+        if (h->dcut > 0.0 && !kst_host.empty()) {
             launch_psv_nlr(h->d_kstart.as<int>(), h->d_rlow.as<int>(), h->d_nl.as<int>(), S, kst_nblk, R,
                            h->d_nlr.as<int>(), h->stream);
-        else
-            HIPCHK(h, hipMemcpyAsync(h->d_nlr.p, nlr.data(), sizeof(int) * nlr.size(), hipMemcpyHostToDevice,
+        } else {
+            psv_nlr_estimate(s, nl, kst_host, kst_nblk, S, nlr_est);
+            HIPCHK(h, hipMemcpyAsync(h->d_nlr.p, nlr_est.data(), sizeof(int) * nlr_est.size(), hipMemcpyHostToDevice,
                                      h->stream));
-        rc = b16_tile_order(h, nlr, S, R);
-        if (rc) return rc;
+        }
+        if (!gmode) {  // host estimate (the cut-off taken from nl) for the tile order
+            psv_nlr_estimate(s, nl, kst_host, kst_nblk, S, nlr_est);
+            if (h->xg >= 0) rc = b16_tile_order(h, nlr_est, S, R);
+            if (rc) return rc;
+        }
     } else {
         launch_zero_tail(h->d_E.as<double2>(), S, PSV_NQ * R, h->d_slots.as<int>(), h->d_nl.as<int>(), LC, h->stream);
         launch_zero_tail(h->d_Es.as<double2>(), S, PSV_NQS, h->d_slots.as<int>(), h->d_nl.as<int>(), LC, h->stream);
     }
     HIPCHK(h, hipGetLastError());
-    ev_end(h, tz);
+    if (gmode) {  // solve graph: excitation .. row extents
+        rc = end_capture_launch(h, cg);
+        if (rc) return rc;
+        ev_end(h, t_solve);
+    } else {
+        ev_end(h, tz);
+    }
+    const bool tiled = h->ns && h->xg >= 0;
 
     SynthArgs ga{};
     ga.E = h->d_E.as<double2>();
@@ -1368,13 +1465,21 @@ static int psv_run_batch(dsm_sgt_t *h, const PsvPrep &p)
     ga.njobs = h->njobs_main;
     TimedLaunch tg;
     ev_begin(h, tg, 2);
-    launch_synth(h, ga, h->d_eb, h->d_nlr.as<int>(), h->ns ? h->d_tiles.as<int>() : nullptr, h->tiles_main_n);
+    if (gmode) {
+        rc = begin_capture(h, cg);
+        if (rc) return rc;
+    }
+    launch_synth(h, ga, h->d_eb, h->d_nlr.as<int>(), tiled ? h->d_tiles.as<int>() : nullptr, h->tiles_main_n);
     HIPCHK(h, hipGetLastError());
     ga.jobs = h->d_jobs.as<SynthJob>() + h->njobs_main;
     ga.njobs = h->njobs_acc;
-    launch_synth(h, ga, h->d_eb, h->d_nlr.as<int>(), h->ns ? h->d_tiles.as<int>() + h->tiles_main_n : nullptr,
+    launch_synth(h, ga, h->d_eb, h->d_nlr.as<int>(), tiled ? h->d_tiles.as<int>() + h->tiles_main_n : nullptr,
                  h->tiles_acc_n);
     HIPCHK(h, hipGetLastError());
+    if (gmode) {  // synthesis graph: the main and the accumulate launch
+        rc = end_capture_launch(h, cg);
+        if (rc) return rc;
+    }
     ev_end(h, tg);
     ga.E = h->d_Es.as<double2>();
     ga.nq = PSV_NQS;
@@ -1408,6 +1513,7 @@ static int psv_run_batch(dsm_sgt_t *h, const PsvPrep &p)
     // 4 flops per (complex coefficient x real function) per (r,theta)
     h->stats.synth_flops += 4.0 * sumL * s.theta_n * (double)PSV_NQ * R;
     HIPCHK(h, dev_sync(h));
+    free_graphs(h);
     if (h->ns) h->stats.synth_mfma_flops += b16_issued_flops(h, nlr_dev, S, R);
     ev_collect(h);
     return DSM_SGT_OK;
@@ -1421,10 +1527,21 @@ extern "C" {
 int dsm_sgt_create(const dsm_sgt_input_t *in, int32_t mode, int32_t precision, int32_t device,
                    dsm_sgt_t **handle)
 {
+    return dsm_sgt_create_tuned(in, mode, precision, device, nullptr, handle);
+}
+
+int dsm_sgt_create_tuned(const dsm_sgt_input_t *in, int32_t mode, int32_t precision, int32_t device,
+                         const dsm_sgt_tuning_t *tuning, dsm_sgt_t **handle)
+{
     if (!in || !handle) return DSM_SGT_EINVAL;
     *handle = nullptr;
     dsm_sgt_t *h = new (std::nothrow) dsm_sgt_t();
     if (!h) return DSM_SGT_ENOMEM;
+    if (tuning) {
+        h->tune = *tuning;
+        h->xg = tuning->xcd_group < 0 ? -1 : tuning->xcd_group;
+        h->graph = tuning->graph == 1 ? 1 : 0;
+    }
     h->in = *in;
     h->mode = mode;
     h->precision = precision;
@@ -1457,6 +1574,122 @@ int dsm_sgt_create(const dsm_sgt_input_t *in, int32_t mode, int32_t precision, i
     } while (0);
     *handle = h;  // returned even on failure so dsm_sgt_last_error() can report
     return rc;
+}
+
+int dsm_sgt_get_tuning(dsm_sgt_t *h, dsm_sgt_tuning_t *t)
+{
+    if (!h || !t) return DSM_SGT_EINVAL;
+    if (!h->ready) return fail(h, DSM_SGT_ESTATE, "engine not initialised");
+    *t = dsm_sgt_tuning_t{};
+    t->nslot = h->nslot;
+    t->lchunk = h->lchunk;
+    t->xcd_group = h->xg;
+    t->graph = h->graph;
+    return DSM_SGT_OK;
+}
+
+int dsm_sgt_autotune(const dsm_sgt_input_t *in, int32_t mode, int32_t precision, int32_t device,
+                     const char *cache_path, dsm_sgt_tuning_t *best, double *probe_ms)
+{
+    if (!in || !best) return DSM_SGT_EINVAL;
+    if (probe_ms) *probe_ms = 0.0;
+    // cache key: FNV-1a of the input record, mode, precision and device name
+    hipDeviceProp_t prop{};
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) return DSM_SGT_EHIP;
+    uint64_t hv = 1469598103934665603ULL;
+    auto mix = [&hv](const void *p, size_t n) {
+        for (size_t i = 0; i < n; i++) hv = (hv ^ ((const unsigned char *)p)[i]) * 1099511628211ULL;
+    };
+    mix(in, sizeof(*in));
+    mix(&mode, sizeof(mode));
+    mix(&precision, sizeof(precision));
+    mix(prop.gcnArchName, std::strlen(prop.gcnArchName));
+    char key[32];
+    std::snprintf(key, sizeof(key), "%016llx", (unsigned long long)hv);
+    if (cache_path) {
+        if (FILE *f = std::fopen(cache_path, "r")) {
+            char k[64];
+            dsm_sgt_tuning_t t{};
+            double ms = 0.0;
+            bool hit = false;
+            while (std::fscanf(f, "%63s %d %d %d %d %lf", k, &t.nslot, &t.lchunk, &t.xcd_group, &t.graph, &ms) == 6)
+                if (std::strcmp(k, key) == 0) {
+                    *best = t;  // the last entry of a key wins
+                    hit = true;
+                }
+            std::fclose(f);
+            if (hit) return DSM_SGT_OK;
+        }
+    }
+    // candidates: batch size and l chunk need an engine each; the tile
+    // grouping and the graph switch are changed on a live engine.  Largest
+    // batch first: the smaller engines reuse its released device blocks.
+    const bool psv = mode == DSM_SGT_PSV, bf16 = precision == DSM_SGT_BF16X3 || precision == DSM_SGT_BF16X6;
+    std::vector<int> nslots, lchunks = {0}, xgs = {0}, graphs = {0};
+    if (psv && precision != DSM_SGT_FP64) {
+        nslots = {128, 96, 64};
+        if (bf16) {
+            xgs = {0, 2, 8, -1};
+            graphs = {0, 1};
+        }
+    } else if (psv) {
+        nslots = {48, 32, 16};
+    } else {
+        nslots = {64, 32, 16};
+        lchunks = {1024, 512, 256};
+    }
+    const int ilo = std::max(1, std::max(in->imin, in->imax - nslots[0] + 1)), ihi = in->imax;
+    if (ihi < ilo) return DSM_SGT_EINVAL;
+    double best_ms = -1.0;
+    int last_rc = DSM_SGT_OK;
+    for (int ns : nslots)
+        for (int lc : lchunks) {
+            dsm_sgt_tuning_t t{};
+            t.nslot = ns;
+            t.lchunk = lc;
+            dsm_sgt_t *h = nullptr;
+            int rc = dsm_sgt_create_tuned(in, mode, precision, device, &t, &h);
+            if (rc) {  // e.g. a batch that does not fit in device memory
+                last_rc = rc;
+                dsm_sgt_destroy(h);
+                continue;
+            }
+            for (int xg : xgs)
+                for (int gr : graphs) {
+                    h->xg = xg;
+                    h->graph = gr;
+                    double ms = -1.0;
+                    for (int rep = 0; rep < 2 && !rc; rep++) {
+                        const auto t0 = std::chrono::steady_clock::now();
+                        rc = dsm_sgt_run(h, ilo, ihi);
+                        const double dt = ms_since(t0);
+                        if (!rc && (ms < 0.0 || dt < ms)) ms = dt;
+                    }
+                    if (rc) {
+                        last_rc = rc;
+                        break;
+                    }
+                    if (best_ms < 0.0 || ms < best_ms) {
+                        best_ms = ms;
+                        *best = dsm_sgt_tuning_t{};
+                        best->nslot = ns;
+                        best->lchunk = lc;
+                        best->xcd_group = xg;
+                        best->graph = gr;
+                    }
+                }
+            dsm_sgt_destroy(h);
+        }
+    if (best_ms < 0.0) return last_rc ? last_rc : DSM_SGT_EHIP;
+    if (probe_ms) *probe_ms = best_ms;
+    if (cache_path) {
+        if (FILE *f = std::fopen(cache_path, "a")) {
+            std::fprintf(f, "%s %d %d %d %d %.3f\n", key, best->nslot, best->lchunk, best->xcd_group, best->graph,
+                         best_ms);
+            std::fclose(f);
+        }
+    }
+    return DSM_SGT_OK;
 }
 
 int dsm_sgt_get_info(dsm_sgt_t *h, dsm_sgt_info_t *info)
@@ -1705,6 +1938,7 @@ void dsm_sgt_destroy(dsm_sgt_t *h)
     if (h->stream) {
         (void)hipSetDevice(h->device);
         (void)hipStreamSynchronize(h->stream);
+        free_graphs(h);
         ev_collect(h);
         (void)hipStreamDestroy(h->stream);
     }

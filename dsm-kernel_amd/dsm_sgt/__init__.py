@@ -37,6 +37,15 @@ class Stats(C.Structure):
                 ("prep_wait_ms", C.c_double), ("device_wait_ms", C.c_double)]
 
 
+class Tuning(C.Structure):
+    """Launch plan (dsm_sgt_tuning_t): 0 in a field takes the default."""
+    _fields_ = [("nslot", C.c_int32), ("lchunk", C.c_int32), ("xcd_group", C.c_int32), ("graph", C.c_int32),
+                ("reserved", C.c_int32 * 4)]
+
+    def as_dict(self):
+        return {"nslot": self.nslot, "lchunk": self.lchunk, "xcd_group": self.xcd_group, "graph": self.graph}
+
+
 class DsmSgtError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__(f"{ERRORS.get(code, code)}: {msg}")
@@ -48,7 +57,8 @@ _lib = None
 # every entry point declared in include/dsm_sgt.h
 EXPORTS = ("dsm_sgt_create", "dsm_sgt_get_info", "dsm_sgt_get_grid", "dsm_sgt_run", "dsm_sgt_fetch",
            "dsm_sgt_fetch_fp64", "dsm_sgt_lcut", "dsm_sgt_write_catalog", "dsm_sgt_get_stats",
-           "dsm_sgt_reset_stats", "dsm_sgt_clear", "dsm_sgt_last_error", "dsm_sgt_destroy", "dsm_sgt_psv_band")
+           "dsm_sgt_reset_stats", "dsm_sgt_clear", "dsm_sgt_last_error", "dsm_sgt_destroy", "dsm_sgt_psv_band",
+           "dsm_sgt_create_tuned", "dsm_sgt_get_tuning", "dsm_sgt_autotune")
 
 
 def lib():
@@ -60,6 +70,11 @@ def lib():
         L = C.CDLL(LIB_PATH)
         L.dsm_sgt_create.argtypes = [C.POINTER(DsmSgtInput), C.c_int32, C.c_int32, C.c_int32,
                                      C.POINTER(C.c_void_p)]
+        L.dsm_sgt_create_tuned.argtypes = [C.POINTER(DsmSgtInput), C.c_int32, C.c_int32, C.c_int32,
+                                           C.POINTER(Tuning), C.POINTER(C.c_void_p)]
+        L.dsm_sgt_get_tuning.argtypes = [C.c_void_p, C.POINTER(Tuning)]
+        L.dsm_sgt_autotune.argtypes = [C.POINTER(DsmSgtInput), C.c_int32, C.c_int32, C.c_int32, C.c_char_p,
+                                       C.POINTER(Tuning), C.POINTER(C.c_double)]
         L.dsm_sgt_get_info.argtypes = [C.c_void_p, C.POINTER(Info)]
         L.dsm_sgt_get_grid.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         L.dsm_sgt_run.argtypes = [C.c_void_p, C.c_int32, C.c_int32]
@@ -84,10 +99,13 @@ class Engine:
     omega-independent set-up; `run(i0, i1)` computes frequencies; `fetch*`
     return catalog records as numpy arrays shaped (theta_n, num)."""
 
-    def __init__(self, inp, mode=SH, precision=FP64, device=0):
+    def __init__(self, inp, mode=SH, precision=FP64, device=0, tuning=None):
         L = lib()
         self._h = C.c_void_p()
-        rc = L.dsm_sgt_create(C.byref(inp), mode, precision, device, C.byref(self._h))
+        if tuning is not None and not isinstance(tuning, Tuning):
+            tuning = Tuning(**tuning)
+        rc = L.dsm_sgt_create_tuned(C.byref(inp), mode, precision, device,
+                                    C.byref(tuning) if tuning is not None else None, C.byref(self._h))
         if rc:
             msg = L.dsm_sgt_last_error(self._h).decode() if self._h else ""
             if self._h:
@@ -136,6 +154,11 @@ class Engine:
         self._check(lib().dsm_sgt_psv_band(self._h, i, out.ctypes.data))
         return out
 
+    def tuning(self):
+        t = Tuning()
+        self._check(lib().dsm_sgt_get_tuning(self._h, C.byref(t)))
+        return t
+
     def lcut(self, i):
         v = C.c_int32()
         self._check(lib().dsm_sgt_lcut(self._h, i, C.byref(v)))
@@ -171,3 +194,14 @@ class Engine:
 
     def __exit__(self, *a):
         self.close()
+
+
+def autotune(inp, mode=PSV, precision=BF16X6, device=0, cache_path=None):
+    """Run the library's autotuner (dsm_sgt_autotune): returns (Tuning,
+    probe milliseconds; 0.0 when the plan came from the cache file)."""
+    t, ms = Tuning(), C.c_double()
+    rc = lib().dsm_sgt_autotune(C.byref(inp), mode, precision, device,
+                                cache_path.encode() if cache_path else None, C.byref(t), C.byref(ms))
+    if rc:
+        raise DsmSgtError(rc, "autotune failed")
+    return t, ms.value

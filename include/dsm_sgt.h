@@ -106,6 +106,38 @@ typedef struct dsm_sgt_stats {
 int dsm_sgt_create(const dsm_sgt_input_t *in, int32_t mode, int32_t precision, int32_t device,
                    dsm_sgt_t **handle);
 
+/* Launch plan of the batched pipeline.  No counterpart in the reference,
+ * whose MPI driver gives one frequency at a time to each rank
+ * (SGTpsv-MPI.f90:550).  A field left 0 takes the built-in default. */
+typedef struct dsm_sgt_tuning {
+    int32_t nslot;      /* frequencies per batch: a solve launch factors nslot x (l range) systems */
+    int32_t lchunk;     /* degrees per solve launch between cut-off checks (SH, FP64 P-SV);
+                           the batched P-SV forms always take the whole l range */
+    int32_t xcd_group;  /* BF16 synthesis: tile groups dealt to one XCD at a time; -1: no tile list
+                           (the kernel's own XCD-sliced order) */
+    int32_t graph;      /* 1: each P-SV batch's launches after the factor-storage read-back are
+                           captured into one HIP graph and launched as one; 0: eager launches */
+    int32_t reserved[4];
+} dsm_sgt_tuning_t;
+
+/* dsm_sgt_create with a launch plan (NULL: the defaults). */
+int dsm_sgt_create_tuned(const dsm_sgt_input_t *in, int32_t mode, int32_t precision, int32_t device,
+                         const dsm_sgt_tuning_t *tuning, dsm_sgt_t **handle);
+/* The launch plan an engine runs with (defaults filled in). */
+int dsm_sgt_get_tuning(dsm_sgt_t *h, dsm_sgt_tuning_t *tuning);
+/* Autotuner: times the candidate launch plans for this input, mode and
+ * precision on a probe of the most expensive frequencies (the top
+ * frequencies of [imin, imax], as many as the largest candidate batch),
+ * each plan run twice, the faster run counted, and returns the fastest
+ * plan in `best` and its probe time in `probe_ms`.  Candidates: the batch
+ * size, the l chunk (SH, FP64 P-SV) and the synthesis tile grouping (BF16
+ * forms).  With `cache_path` (may be NULL) the result is looked up in and
+ * appended to that text file, keyed by a hash of the input, mode,
+ * precision and device name; a cached plan returns with probe_ms = 0.
+ * The probe engines' catalogs are discarded. */
+int dsm_sgt_autotune(const dsm_sgt_input_t *in, int32_t mode, int32_t precision, int32_t device,
+                     const char *cache_path, dsm_sgt_tuning_t *best, double *probe_ms);
+
 int dsm_sgt_get_info(dsm_sgt_t *h, dsm_sgt_info_t *info);
 /* Receiver radii (r_n values) and distances in degrees (theta_n values). */
 int dsm_sgt_get_grid(dsm_sgt_t *h, double *radii, double *theta_deg);

@@ -1,0 +1,118 @@
+"""GPU: launch plans (dsm_sgt_tuning_t) change how the batched pipeline is
+launched, never what it computes.
+
+* graph mode (each batch's launches after the factor-storage read-back run
+  as two HIP graphs) gives the catalog of the eager launches bit for bit,
+  run after run, with eager work in between;
+* the synthesis tile grouping (per-XCD chunks of 2 or 8, or no tile list)
+  and the batch size give the same catalog bit for bit: each tile sums the
+  same products in the same order, and rows past a frequency's cut-off are
+  exact zeros;
+* the autotuner returns a plan among its candidates, writes it to the cache
+  file and reads it back without probing.
+
+All through the C-ABI (dsm_sgt_create_tuned, dsm_sgt_get_tuning,
+dsm_sgt_autotune).
+"""
+import numpy as np
+import pytest
+
+import dsm_inputs as di
+
+pytestmark = pytest.mark.gpu
+
+# 40 frequencies (a short first batch of 16, then batches of the plan's
+# size), the depth cut active at the low frequencies, a symmetric distance
+# grid (the bf16 forms' mirror), receivers from 3716 km to the surface
+GRID = dict(tlen=1638.4, imax=40, rmin_=3716., rmax_=6371., rdelta_=265.5, thetamin=0., thetamax=180.,
+            thetadelta=2.0, maxlmax=1500)
+FREQS = (1, 2, 17, 33, 40)
+
+
+def _catalog(eng, freqs=FREQS):
+    return {i: eng.fetch_all_fp64(i) for i in freqs}
+
+
+def _same(a, b):
+    for i in a:
+        for x, y in zip(a[i], b[i]):
+            assert np.array_equal(x, y), f"frequency {i} differs: max {np.abs(x - y).max():.3e}"
+
+
+def _input():
+    import dsm_sgt
+    return dsm_sgt.DsmSgtInput.from_buffer_copy(di.synthetic_input(**GRID))
+
+
+def test_graph_mode_equals_eager():
+    import dsm_sgt
+    inp = _input()
+    with dsm_sgt.Engine(inp, mode=dsm_sgt.PSV, precision=dsm_sgt.BF16X6) as eager:
+        eager.run(1, GRID["imax"])
+        want = _catalog(eager)
+        lc = [eager.lcut(i) for i in FREQS]
+    with dsm_sgt.Engine(inp, mode=dsm_sgt.PSV, precision=dsm_sgt.BF16X6, tuning={"graph": 1, "nslot": 16}) as g:
+        assert g.tuning().graph == 1 and g.tuning().nslot == 16
+        for rep in range(3):
+            g.clear()
+            g.run(1, GRID["imax"])
+            _same(_catalog(g), want)
+            assert [g.lcut(i) for i in FREQS] == lc
+            # eager work of the process between the graph runs
+            with dsm_sgt.Engine(inp, mode=dsm_sgt.PSV, precision=dsm_sgt.BF16X6, tuning={"nslot": 16}) as e2:
+                e2.run(GRID["imax"] - 3, GRID["imax"])
+                _same(_catalog(e2, (GRID["imax"],)), {GRID["imax"]: want[GRID["imax"]]})
+        print(f"graph mode: {len(FREQS)} frequencies bit-identical to the eager launches over 3 runs")
+
+
+@pytest.mark.parametrize("plan", [{"xcd_group": -1}, {"xcd_group": 2}, {"xcd_group": 8}, {"nslot": 24},
+                                  {"nslot": 24, "xcd_group": -1, "graph": 1}])
+def test_plans_give_the_same_catalog(plan):
+    import dsm_sgt
+    inp = _input()
+    with dsm_sgt.Engine(inp, mode=dsm_sgt.PSV, precision=dsm_sgt.BF16X6) as ref:
+        ref.run(1, GRID["imax"])
+        want = _catalog(ref)
+    with dsm_sgt.Engine(inp, mode=dsm_sgt.PSV, precision=dsm_sgt.BF16X6, tuning=plan) as e:
+        t = e.tuning().as_dict()
+        for k, v in plan.items():
+            assert t[k] == v, (k, t)
+        e.run(1, GRID["imax"])
+        _same(_catalog(e), want)
+
+
+def test_fp64_and_sh_plans_give_the_same_catalog():
+    import dsm_sgt
+    inp = _input()
+    for mode, prec, plan in ((dsm_sgt.PSV, dsm_sgt.FP64, {"nslot": 8}),
+                             (dsm_sgt.SH, dsm_sgt.FP64, {"nslot": 8, "lchunk": 256}),
+                             (dsm_sgt.SH, dsm_sgt.BF16X6, {"nslot": 8, "lchunk": 1024})):
+        with dsm_sgt.Engine(inp, mode=mode, precision=prec) as ref:
+            ref.run(1, 20)
+            want = _catalog(ref, (1, 9, 20))
+        with dsm_sgt.Engine(inp, mode=mode, precision=prec, tuning=plan) as e:
+            assert e.tuning().nslot == plan["nslot"]
+            e.run(1, 20)
+            _same(_catalog(e, (1, 9, 20)), want)
+
+
+def test_autotune_and_cache(tmp_path):
+    import dsm_sgt
+    inp = _input()
+    cache = str(tmp_path / "tune.txt")
+    plan, ms = dsm_sgt.autotune(inp, dsm_sgt.PSV, dsm_sgt.BF16X6, cache_path=cache)
+    assert ms > 0.0
+    assert plan.nslot in (64, 96, 128) and plan.xcd_group in (0, 2, 8, -1) and plan.graph in (0, 1)
+    again, ms2 = dsm_sgt.autotune(inp, dsm_sgt.PSV, dsm_sgt.BF16X6, cache_path=cache)
+    assert ms2 == 0.0 and again.as_dict() == plan.as_dict()
+    lines = open(cache).read().split("\n")
+    assert len([x for x in lines if x.strip()]) == 1
+    # another precision is another key
+    p64, ms3 = dsm_sgt.autotune(inp, dsm_sgt.PSV, dsm_sgt.FP64, cache_path=cache)
+    assert ms3 > 0.0 and p64.nslot in (16, 32, 48)
+    with dsm_sgt.Engine(inp, mode=dsm_sgt.PSV, precision=dsm_sgt.BF16X6, tuning=plan) as e, \
+            dsm_sgt.Engine(inp, mode=dsm_sgt.PSV, precision=dsm_sgt.BF16X6) as ref:
+        e.run(1, GRID["imax"])
+        ref.run(1, GRID["imax"])
+        _same(_catalog(e), _catalog(ref))
+    print(f"autotune: plan {plan.as_dict()} in {ms:.1f} ms probe; FP64 plan {p64.as_dict()}")
