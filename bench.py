@@ -89,6 +89,101 @@ def cpu_baseline(mode, p, r_n, lsample=1000):
                        f"x {p['imax']} frequencies")
 
 
+# Multi-depth 2.5 s workload: P-SV and SH catalogs for 16 source depths
+# drawn uniformly in 0..700 km, 60 radii x 3601 distances, lmax 16000,
+# i = 1..1311 (tlen 3276.8 s).  The catalog (~1 TB in complex64) does not fit
+# in device memory: each engine holds a window of C5_WINDOW frequencies, and
+# every run replaces it.  64 spot frequencies (4 per depth) are compared
+# with the plain FP64 form of the same depth.
+C5 = dict(tlen=3276.8, imax=1311, rmin_=3716.0, rmax_=6371.0, rdelta_=45.0, thetamin=0.0, thetamax=180.0,
+          thetadelta=0.05, maxlmax=16000, re=1e-1, r0=6371.0)
+C5_SEED = 5
+C5_WINDOW = 128
+
+
+def run_c5(args):
+    """Time the batched mixed form (args.form) over the multi-depth 2.5 s
+    workload; one JSON line.  Per (depth, mode): engine create + runs of
+    C5_WINDOW frequencies with the catalog resident in HBM after each run
+    (records fetched only for the spot checks, outside the timed region)."""
+    import numpy as np
+    import dsm_sgt
+    from parity import component_errors
+    rng = np.random.default_rng(C5_SEED)
+    depths = rng.uniform(0.0, 700.0, 16)
+    spots = rng.choice(np.arange(1, C5["imax"] + 1), 64, replace=False)
+    prec = getattr(dsm_sgt, FORMS[args.form][0])
+    imax = args.imax or C5["imax"]
+    spots = [int(x) for x in spots if x <= imax] if args.imax else [int(x) for x in spots]
+    modes = (("psv", dsm_sgt.PSV), ("sh", dsm_sgt.SH))
+    plans, tune_s = {}, {}
+    for name, mode in modes:  # tuned once per mode, on the deepest source (the largest buffers)
+        p = dict(C5, imax=imax, r0=6371.0 - float(depths.max()))
+        t0 = time.perf_counter()
+        plan, _ = (dsm_sgt.autotune(make_input(p), mode, prec, cache_path=args.tune_cache) if not args.no_tune
+                   else (dsm_sgt.Tuning(), 0.0))
+        plan.catalog_window = C5_WINDOW
+        plans[name], tune_s[name] = plan, time.perf_counter() - t0
+        print(f"c5: {name} plan {plan.as_dict()} ({tune_s[name]:.1f} s)", file=sys.stderr, flush=True)
+    total = {"psv": 0.0, "sh": 0.0}
+    worst = {"psv": {"tsgt": 0.0, "rsgt": 0.0, "synn": 0.0}, "sh": {"tsgt": 0.0, "rsgt": 0.0, "synn": 0.0}}
+    per_depth, bytes64 = [], 0.0
+    for d, depth in enumerate(depths):
+        p = dict(C5, imax=imax, r0=6371.0 - float(depth))
+        inp = make_input(p)
+        my_spots = spots[4 * d:4 * d + 4]
+        row = {"depth_km": float(depth)}
+        for name, mode in modes:
+            t0 = time.perf_counter()
+            eng = dsm_sgt.Engine(inp, mode=mode, precision=prec, tuning=plans[name])
+            t_run = time.perf_counter() - t0
+            info = eng.info
+            spot_ref = dsm_sgt.Engine(inp, mode=mode, precision=dsm_sgt.FP64,
+                                      tuning={"nslot": 4, "catalog_window": 4}) if my_spots else None
+            lcut_max = 0
+            for i0 in range(1, imax + 1, C5_WINDOW):
+                i1 = min(imax, i0 + C5_WINDOW - 1)
+                t1 = time.perf_counter()
+                eng.run(i0, i1)
+                t_run += time.perf_counter() - t1
+                lcut_max = max(lcut_max, max(eng.lcut(i) for i in (i0, i1)))
+                inside = [i for i in my_spots if i0 <= i <= i1]
+                if inside:
+                    for i in inside:  # one at a time (the reference engine's window)
+                        spot_ref.run(i, i)
+                        got, want = eng.fetch_all_fp64(i), spot_ref.fetch_all_fp64(i)
+                        for k, kind in enumerate(("tsgt", "rsgt", "synn")):
+                            worst[name][kind] = max(worst[name][kind],
+                                                    float(component_errors(got[k], want[k]).max()))
+            bytes64 += 8.0 * imax * info.theta_n * (info.r_n * (info.num_tsgt + info.num_rsgt) + info.num_synn)
+            eng.close()
+            if spot_ref is not None:
+                spot_ref.close()
+            total[name] += t_run
+            row[name + "_s"] = t_run
+            row[name + "_lcut_max"] = lcut_max
+        per_depth.append(row)
+        print(f"c5: depth {d + 1}/16 ({depth:.1f} km): P-SV {row['psv_s']:.2f} s, SH {row['sh_s']:.2f} s; "
+              f"worst spot errors so far {worst}", file=sys.stderr, flush=True)
+    value = total["psv"] + total["sh"]
+    out = {
+        "metric": "SGT catalog wall time, multi-depth 2.5 s P-SV + SH (batched mixed form)",
+        "value": value, "unit": "s", "n_gpus": 1, "steps": 1, "warmup": 0, "ms_per_step": 1e3 * value,
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": FORMS[args.form][1],
+        "data": "synthetic",
+        "config": {"workload": f"SGTpsv + SGTsh 2.5 s catalogs for 16 source depths (uniform 0..700 km, seed "
+                               f"{C5_SEED}): 4-zone sphere, tlen 3276.8 s, i=1..{imax}, re 0.1, lmax 16000, "
+                               f"60 radii (3716..6371 km) x 3601 distances; catalog window {C5_WINDOW} "
+                               f"frequencies", "depths_km": [float(x) for x in depths]},
+        "seconds": total, "catalog_bytes_complex64": bytes64,
+        "spot_check": {"frequencies": spots, "per_depth": 4, "against": "plain FP64 form (GPU)",
+                       "max_rel_l2": worst, "bar": 1e-6},
+        "plans": {k: v.as_dict() for k, v in plans.items()}, "tune_s": tune_s, "per_depth": per_depth,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 FORMS = {"bf16x6": ("BF16X6", "bf16x6+f64"), "bf16x3": ("BF16X3", "bf16x3+f64"), "fp32": ("FP32", "f32+f64"),
          "fp64": ("FP64", "f64")}
 BF16_PEAK = 2.5e15
@@ -142,6 +237,9 @@ def main():
     ap.add_argument("--tune-cache", default=os.path.join(os.environ.get("XDG_CACHE_HOME") or
                                                          os.path.expanduser("~/.cache"), "dsm_sgt_tune.txt"),
                     help="autotuner cache file (plans keyed by input, form and device)")
+    ap.add_argument("--workload", choices=("catalog", "c5"), default="catalog",
+                    help="catalog: the 5 s P-SV (or --mode sh 10 s SH) catalog; c5: 16 source depths at 2.5 s, "
+                         "P-SV + SH")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only; the catalog's frequencies are independent "
@@ -149,6 +247,8 @@ def main():
               file=sys.stderr)
         return 2
     import dsm_sgt
+    if args.workload == "c5":
+        return run_c5(args)
     p = dict(WORKLOADS[args.mode])
     if args.imax:
         p["imax"] = args.imax

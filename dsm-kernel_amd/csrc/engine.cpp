@@ -157,10 +157,14 @@ struct dsm_sgt {
     // and graph switch in force (xg 0: default grouping, -1: no tile list)
     dsm_sgt_tuning_t tune{};
     int xg = 0, graph = 0;
-    int ipos0 = 0, npos = 0;  // catalog covers frequencies [ipos0, ipos0+npos)
+    int ipos0 = 0, npos = 0;  // the run covers frequencies [ipos0, ipos0+npos)
+    // device catalog: ncat frequencies from cat0 (ncat = npos and cat0 =
+    // ipos0 unless a catalog window is set)
+    int ncat = 0, cat0 = 0;
     std::vector<int> need, rec_slot;
     std::vector<int> llog;    // per position: the reference's logged llog-1
-    std::vector<char> done;   // per position: computed
+    std::vector<char> done;   // per position: catalog resident on the device
+    std::vector<char> logged; // per position: l cut-off known
     // device buffers
     DevBuf d_theta, d_tab, d_tab32, d_need, d_recslot, d_w0, d_w1, d_rinv, d_src;
     // BF16 forms: split operands, distance symmetry
@@ -539,6 +543,9 @@ static int sh_create(dsm_sgt_t *h)
     h->npos = std::max(0, h->in.imax - h->ipos0 + 1);
     h->llog.assign(h->npos, 0);
     h->done.assign(h->npos, 0);
+    h->logged.assign(h->npos, 0);
+    h->ncat = h->tune.catalog_window > 0 ? std::min(h->tune.catalog_window, std::max(1, h->npos)) : h->npos;
+    h->cat0 = h->ipos0;
 
     std::vector<double> th(s.theta_n);
     for (int i = 0; i < s.theta_n; i++) th[i] = s.theta_deg[i] / 180.0 * PI;
@@ -590,7 +597,7 @@ static int sh_create(dsm_sgt_t *h)
     HIPCHK(h, h->d_E.ensure(sizeof(double2) * (size_t)S * 36 * R * h->lcap));
     HIPCHK(h, h->d_Es.ensure(sizeof(double2) * (size_t)S * 5 * h->lcap));
     HIPCHK(h, h->d_amp.ensure(sizeof(double) * (size_t)S * h->lcap));
-    const size_t np = (size_t)std::max(1, h->npos);
+    const size_t np = (size_t)std::max(1, h->ncat);
     HIPCHK(h, h->d_cat_t.ensure(sizeof(double2) * np * R * s.theta_n * 10));
     HIPCHK(h, h->d_cat_r.ensure(sizeof(double2) * np * R * s.theta_n * 5));
     HIPCHK(h, h->d_cat_s.ensure(sizeof(double2) * np * s.theta_n * 5));
@@ -658,7 +665,7 @@ static int sh_run_batch(dsm_sgt_t *h, const std::vector<int> &freqs)
         om[2 * k + 1] = make_double2(o.coef_spn.real(), o.coef_spn.imag());
         cut[k].lsuf = o.lsuf;
         cut[k].llog = s.maxlmax;
-        slotout[k] = freqs[k] - h->ipos0;
+        slotout[k] = freqs[k] - h->cat0;  // catalog position
     }
     HIPCHK(h, hipMemcpyAsync(h->d_band.p, band.data(), sizeof(double2) * band.size(), hipMemcpyHostToDevice, h->stream));
     HIPCHK(h, hipMemcpyAsync(h->d_om.p, om.data(), sizeof(double2) * om.size(), hipMemcpyHostToDevice, h->stream));
@@ -752,8 +759,9 @@ static int sh_run_batch(dsm_sgt_t *h, const std::vector<int> &freqs)
     for (int k = 0; k < S; k++) {
         nl[k] = cut[k].nl;
         all[k] = k;
-        h->llog[slotout[k]] = cut[k].llog - 1;
-        h->done[slotout[k]] = 1;
+        h->llog[freqs[k] - h->ipos0] = cut[k].llog - 1;
+        h->done[freqs[k] - h->ipos0] = 1;
+        h->logged[freqs[k] - h->ipos0] = 1;
     }
     HIPCHK(h, hipMemcpyAsync(h->d_nl.p, nl.data(), sizeof(int) * S, hipMemcpyHostToDevice, h->stream));
     HIPCHK(h, hipMemcpyAsync(h->d_slots.p, all.data(), sizeof(int) * S, hipMemcpyHostToDevice, h->stream));
@@ -851,6 +859,9 @@ static int psv_create(dsm_sgt_t *h)
     h->npos = std::max(0, h->in.imax - h->ipos0 + 1);
     h->llog.assign(h->npos, 0);
     h->done.assign(h->npos, 0);
+    h->logged.assign(h->npos, 0);
+    h->ncat = h->tune.catalog_window > 0 ? std::min(h->tune.catalog_window, std::max(1, h->npos)) : h->npos;
+    h->cat0 = h->ipos0;
 
     std::vector<double> th(s.theta_n);
     for (int i = 0; i < s.theta_n; i++) th[i] = s.theta_deg[i] / 180.0 * PI;
@@ -961,7 +972,7 @@ static int psv_create(dsm_sgt_t *h)
         HIPCHK(h, h->d_Es.ensure(sizeof(double2) * (size_t)S * PSV_NQS * LC));
     }
     HIPCHK(h, h->d_amp.ensure(sizeof(double2) * 2 * (size_t)S * LC));
-    const size_t np = (size_t)std::max(1, h->npos);
+    const size_t np = (size_t)std::max(1, h->ncat);
     HIPCHK(h, h->d_cat_t.ensure(sizeof(double2) * np * R * s.theta_n * 20));
     HIPCHK(h, h->d_cat_r.ensure(sizeof(double2) * np * R * s.theta_n * 10));
     HIPCHK(h, h->d_cat_s.ensure(sizeof(double2) * np * s.theta_n * 10));
@@ -1143,7 +1154,7 @@ static int psv_run_batch(dsm_sgt_t *h, const PsvPrep &p)
     for (int k = 0; k < S; k++) {
         cut[k].lsuf = p.lsuf[k];
         cut[k].llog = s.maxlmax;
-        slotout[k] = freqs[k] - h->ipos0;
+        slotout[k] = freqs[k] - h->cat0;  // catalog position
     }
     {
         const int rc = psv_device_band(h, p.scal, S, 0);
@@ -1405,8 +1416,9 @@ static int psv_run_batch(dsm_sgt_t *h, const PsvPrep &p)
         nl[k] = dev_cut ? s.maxlmax + 1 : cut[k].nl;  // device cut-off: known after the batch
         all[k] = k;
         if (!dev_cut) {
-            h->llog[slotout[k]] = cut[k].llog - 1;
-            h->done[slotout[k]] = 1;
+            h->llog[freqs[k] - h->ipos0] = cut[k].llog - 1;
+            h->done[freqs[k] - h->ipos0] = 1;
+            h->logged[freqs[k] - h->ipos0] = 1;
         }
     }
     if (!dev_cut) HIPCHK(h, hipMemcpyAsync(h->d_nl.p, nl.data(), sizeof(int) * S, hipMemcpyHostToDevice, h->stream));
@@ -1504,8 +1516,9 @@ static int psv_run_batch(dsm_sgt_t *h, const PsvPrep &p)
         HIPCHK(h, hipMemcpyAsync(llog.data(), h->d_llog.p, sizeof(int) * S, hipMemcpyDeviceToHost, h->stream));
         HIPCHK(h, dev_sync(h));
         for (int k = 0; k < S; k++) {
-            h->llog[slotout[k]] = llog[k] - 1;
-            h->done[slotout[k]] = 1;
+            h->llog[freqs[k] - h->ipos0] = llog[k] - 1;
+            h->done[freqs[k] - h->ipos0] = 1;
+            h->logged[freqs[k] - h->ipos0] = 1;
         }
     }
     double sumL = 0.0;
@@ -1585,6 +1598,7 @@ int dsm_sgt_get_tuning(dsm_sgt_t *h, dsm_sgt_tuning_t *t)
     t->lchunk = h->lchunk;
     t->xcd_group = h->xg;
     t->graph = h->graph;
+    t->catalog_window = h->ncat < h->npos ? h->ncat : 0;
     return DSM_SGT_OK;
 }
 
@@ -1647,6 +1661,7 @@ int dsm_sgt_autotune(const dsm_sgt_input_t *in, int32_t mode, int32_t precision,
             dsm_sgt_tuning_t t{};
             t.nslot = ns;
             t.lchunk = lc;
+            t.catalog_window = ihi - ilo + 1;  // the probe's frequencies only
             dsm_sgt_t *h = nullptr;
             int rc = dsm_sgt_create_tuned(in, mode, precision, device, &t, &h);
             if (rc) {  // e.g. a batch that does not fit in device memory
@@ -1724,6 +1739,14 @@ int dsm_sgt_run(dsm_sgt_t *h, int32_t i_begin, int32_t i_end)
     if (i_begin < h->ipos0 || i_end >= h->ipos0 + h->npos || i_begin > i_end)
         return fail(h, DSM_SGT_EINVAL, "frequency range outside [imin, imax] of the input");
     if (hipSetDevice(h->device) != hipSuccess) return fail(h, DSM_SGT_EHIP, "hipSetDevice failed");
+    if (h->ncat < h->npos) {
+        // catalog window: this run's frequencies replace the resident ones
+        const int first = std::max(i_begin, 1);
+        if (first <= i_end && i_end - first + 1 > h->ncat)
+            return fail(h, DSM_SGT_EINVAL, "more frequencies than the catalog window holds");
+        std::fill(h->done.begin(), h->done.end(), 0);
+        h->cat0 = first;
+    }
     // zig-zag order of the reference's rank distribution is irrelevant on one
     // device; batch consecutive frequencies (similar l cut-off)
     std::vector<int> all;
@@ -1769,9 +1792,9 @@ int dsm_sgt_run(dsm_sgt_t *h, int32_t i_begin, int32_t i_end)
 static int fetch_common(dsm_sgt_t *h, int32_t i, int32_t kind, int32_t ir, std::vector<double2> &buf, int &num)
 {
     if (!h) return DSM_SGT_EINVAL;
-    const int pos = i - h->ipos0;
-    if (pos < 0 || pos >= h->npos || !h->done[pos])
-        return fail(h, DSM_SGT_ESTATE, "frequency " + std::to_string(i) + " not computed");
+    if (i - h->ipos0 < 0 || i - h->ipos0 >= h->npos || !h->done[i - h->ipos0])
+        return fail(h, DSM_SGT_ESTATE, "frequency " + std::to_string(i) + " not computed (or not in the catalog window)");
+    const int pos = i - h->cat0;  // device catalog position
     const int R = h->r_n, T = h->theta_n;
     // device catalogs: [pos][comp][r][theta] (r only for TSGT, RSGT)
     const double2 *src;
@@ -1846,7 +1869,7 @@ int dsm_sgt_lcut(dsm_sgt_t *h, int32_t i, int32_t *llog_minus_1)
 {
     if (!h || !llog_minus_1) return DSM_SGT_EINVAL;
     const int pos = i - h->ipos0;
-    if (pos < 0 || pos >= h->npos || !h->done[pos])
+    if (pos < 0 || pos >= h->npos || !h->logged[pos])
         return fail(h, DSM_SGT_ESTATE, "frequency " + std::to_string(i) + " not computed");
     *llog_minus_1 = h->llog[pos];
     return DSM_SGT_OK;
@@ -1927,6 +1950,7 @@ int dsm_sgt_clear(dsm_sgt_t *h)
 {
     if (!h) return DSM_SGT_EINVAL;
     std::fill(h->done.begin(), h->done.end(), 0);
+    std::fill(h->logged.begin(), h->logged.end(), 0);
     return DSM_SGT_OK;
 }
 

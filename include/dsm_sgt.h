@@ -117,7 +117,10 @@ typedef struct dsm_sgt_tuning {
                            (the kernel's own XCD-sliced order) */
     int32_t graph;      /* 1: each P-SV batch's launches after the factor-storage read-back are
                            captured into one HIP graph and launched as one; 0: eager launches */
-    int32_t reserved[4];
+    int32_t catalog_window; /* > 0: the device catalog holds this many frequencies, the frequencies of
+                               the last dsm_sgt_run only (at most this many), so that catalogs larger
+                               than device memory are produced run by run; 0: all of [imin, imax] */
+    int32_t reserved[3];
 } dsm_sgt_tuning_t;
 
 /* dsm_sgt_create with a launch plan (NULL: the defaults). */

@@ -116,3 +116,31 @@ def test_autotune_and_cache(tmp_path):
         ref.run(1, GRID["imax"])
         _same(_catalog(e), _catalog(ref))
     print(f"autotune: plan {plan.as_dict()} in {ms:.1f} ms probe; FP64 plan {p64.as_dict()}")
+
+
+@pytest.mark.parametrize("mode_prec", ["PSV/BF16X6", "PSV/FP64", "SH/BF16X6"])
+def test_catalog_window(mode_prec):
+    """A catalog window of 12 frequencies: each run replaces the resident
+    catalog; records equal the full catalog's bit for bit; frequencies of
+    an earlier run are no longer fetchable but keep their l cut-off; a run
+    longer than the window is refused."""
+    import dsm_sgt
+    mode, prec = (getattr(dsm_sgt, x) for x in mode_prec.split("/"))
+    inp = _input()
+    with dsm_sgt.Engine(inp, mode=mode, precision=prec) as full:
+        full.run(1, 24)
+        want = _catalog(full, (1, 12, 13, 24))
+        lc = {i: full.lcut(i) for i in (1, 12, 13, 24)}
+    with dsm_sgt.Engine(inp, mode=mode, precision=prec, tuning={"catalog_window": 12, "nslot": 8}) as w:
+        assert w.tuning().catalog_window == 12
+        with pytest.raises(dsm_sgt.DsmSgtError) as e:
+            w.run(1, 13)
+        assert e.value.code == 1  # EINVAL
+        w.run(0, 12)  # i = 0 is skipped, so 12 frequencies
+        _same(_catalog(w, (1, 12)), {i: want[i] for i in (1, 12)})
+        w.run(13, 24)
+        _same(_catalog(w, (13, 24)), {i: want[i] for i in (13, 24)})
+        with pytest.raises(dsm_sgt.DsmSgtError) as e:
+            w.fetch(1, dsm_sgt.TSGT, 0)
+        assert e.value.code == 2  # ESTATE: replaced by the second run
+        assert {i: w.lcut(i) for i in lc} == lc
