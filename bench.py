@@ -120,15 +120,19 @@ def run_c5(args):
     for name, mode in modes:  # tuned once per mode, on the deepest source (the largest buffers)
         p = dict(C5, imax=imax, r0=6371.0 - float(depths.max()))
         t0 = time.perf_counter()
-        plan, _ = (dsm_sgt.autotune(make_input(p), mode, prec, cache_path=args.tune_cache) if not args.no_tune
-                   else (dsm_sgt.Tuning(), 0.0))
+        if args.plan:
+            plan = dsm_sgt.Tuning(**json.loads(args.plan).get(name, {}))
+        elif args.no_tune:
+            plan = dsm_sgt.Tuning()
+        else:
+            plan, _ = dsm_sgt.autotune(make_input(p), mode, prec, cache_path=args.tune_cache)
         plan.catalog_window = C5_WINDOW
         plans[name], tune_s[name] = plan, time.perf_counter() - t0
         print(f"c5: {name} plan {plan.as_dict()} ({tune_s[name]:.1f} s)", file=sys.stderr, flush=True)
     total = {"psv": 0.0, "sh": 0.0}
     worst = {"psv": {"tsgt": 0.0, "rsgt": 0.0, "synn": 0.0}, "sh": {"tsgt": 0.0, "rsgt": 0.0, "synn": 0.0}}
-    per_depth, bytes64 = [], 0.0
-    for d, depth in enumerate(depths):
+    per_depth, bytes64, checked = [], 0.0, []
+    for d, depth in enumerate(depths[:args.c5_depths]):
         p = dict(C5, imax=imax, r0=6371.0 - float(depth))
         inp = make_input(p)
         my_spots = spots[4 * d:4 * d + 4]
@@ -149,6 +153,7 @@ def run_c5(args):
                 lcut_max = max(lcut_max, max(eng.lcut(i) for i in (i0, i1)))
                 inside = [i for i in my_spots if i0 <= i <= i1]
                 if inside:
+                    checked.extend(inside)
                     for i in inside:  # one at a time (the reference engine's window)
                         spot_ref.run(i, i)
                         got, want = eng.fetch_all_fp64(i), spot_ref.fetch_all_fp64(i)
@@ -163,7 +168,7 @@ def run_c5(args):
             row[name + "_s"] = t_run
             row[name + "_lcut_max"] = lcut_max
         per_depth.append(row)
-        print(f"c5: depth {d + 1}/16 ({depth:.1f} km): P-SV {row['psv_s']:.2f} s, SH {row['sh_s']:.2f} s; "
+        print(f"c5: depth {d + 1}/{args.c5_depths} ({depth:.1f} km): P-SV {row['psv_s']:.2f} s, SH {row['sh_s']:.2f} s; "
               f"worst spot errors so far {worst}", file=sys.stderr, flush=True)
     value = total["psv"] + total["sh"]
     out = {
@@ -174,9 +179,9 @@ def run_c5(args):
         "config": {"workload": f"SGTpsv + SGTsh 2.5 s catalogs for 16 source depths (uniform 0..700 km, seed "
                                f"{C5_SEED}): 4-zone sphere, tlen 3276.8 s, i=1..{imax}, re 0.1, lmax 16000, "
                                f"60 radii (3716..6371 km) x 3601 distances; catalog window {C5_WINDOW} "
-                               f"frequencies", "depths_km": [float(x) for x in depths]},
+                               f"frequencies", "depths_km": [float(x) for x in depths[:args.c5_depths]]},
         "seconds": total, "catalog_bytes_complex64": bytes64,
-        "spot_check": {"frequencies": spots, "per_depth": 4, "against": "plain FP64 form (GPU)",
+        "spot_check": {"frequencies": sorted(checked), "per_depth": 4, "against": "plain FP64 form (GPU)",
                        "max_rel_l2": worst, "bar": 1e-6},
         "plans": {k: v.as_dict() for k, v in plans.items()}, "tune_s": tune_s, "per_depth": per_depth,
     }
@@ -237,6 +242,9 @@ def main():
     ap.add_argument("--tune-cache", default=os.path.join(os.environ.get("XDG_CACHE_HOME") or
                                                          os.path.expanduser("~/.cache"), "dsm_sgt_tune.txt"),
                     help="autotuner cache file (plans keyed by input, form and device)")
+    ap.add_argument("--plan", default=None,
+                    help='launch plan as JSON (e.g. \'{"nslot": 64, "xcd_group": 2}\') instead of the autotuner')
+    ap.add_argument("--c5-depths", type=int, default=16, help="(profiling) first N of the 16 source depths")
     ap.add_argument("--workload", choices=("catalog", "c5"), default="catalog",
                     help="catalog: the 5 s P-SV (or --mode sh 10 s SH) catalog; c5: 16 source depths at 2.5 s, "
                          "P-SV + SH")
@@ -259,7 +267,9 @@ def main():
     # launch plan: the library's autotuner, once per input shape and form
     # (cached; not part of the timed steps)
     plan, tune = None, None
-    if not args.no_tune:
+    if args.plan:
+        plan = dsm_sgt.Tuning(**json.loads(args.plan))
+    elif not args.no_tune:
         os.makedirs(os.path.dirname(os.path.abspath(args.tune_cache)), exist_ok=True)
         t0 = time.perf_counter()
         plan, probe_ms = dsm_sgt.autotune(inp, mode, prec, cache_path=args.tune_cache)

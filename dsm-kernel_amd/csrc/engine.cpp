@@ -42,9 +42,13 @@ constexpr double PI = 3.1415926535897932;
 // Device blocks released by destroyed engines, kept for the next engine of
 // the process: an engine holds ~100 GB, and freeing it and allocating it
 // again has stalled engine creation for seconds.  Per device; a released
-// block is reused for a request of at least half its size; all blocks of
-// a device are freed when an allocation fails.  Never torn down (the
-// process exit returns the memory).
+// block is reused for a request of at least half its size.  The blocks a
+// new engine did not take are freed once it is built, and all blocks of a
+// device are freed before an allocation that would leave less than
+// kHbmReserve free; an allocation that would still leave less fails
+// (ENOMEM).  The runtime allocates kernel scratch (register spills) at
+// launch from the same memory, and a queue whose scratch allocation fails
+// aborts.  Never torn down (the process exit returns the memory).
 class BufCache {
     std::mutex m_;
     std::multimap<std::pair<int, size_t>, void *> free_;
@@ -84,6 +88,8 @@ BufCache &buf_cache()
     return *c;
 }
 
+constexpr size_t kHbmReserve = (size_t)8 << 30;
+
 struct DevBuf {
     void *p = nullptr;
     size_t n = 0;
@@ -103,6 +109,13 @@ struct DevBuf {
         if ((p = buf_cache().take(dev, bytes, got))) {
             n = got;
             return hipSuccess;
+        }
+        size_t fr = 0, tot = 0;
+        if (hipMemGetInfo(&fr, &tot) == hipSuccess && fr < bytes + kHbmReserve) {
+            buf_cache().flush(dev);
+            // still short of the reserve: refuse here (ENOMEM) rather than
+            // let a later kernel launch abort the queue
+            if (hipMemGetInfo(&fr, &tot) == hipSuccess && fr < bytes + kHbmReserve) return hipErrorOutOfMemory;
         }
         hipError_t e = hipMalloc(&p, bytes);
         if (e == hipErrorOutOfMemory) {
@@ -134,6 +147,13 @@ struct PsvPrep {
     std::vector<double2> fk;
     std::vector<PsvSrc> src;
     std::vector<double> omg;
+    std::vector<int> lsuf;
+};
+
+struct ShPrep {
+    std::vector<int> freqs;
+    std::vector<double2> band;  // [slot][a0 re/im pair, a2 pair][nn]
+    std::vector<double2> om;    // [slot][omega^2, coef at the source]
     std::vector<int> lsuf;
 };
 
@@ -188,9 +208,13 @@ struct dsm_sgt {
     // graph mode: the executable graphs of the current batch (destroyed once
     // the batch has synchronised)
     std::vector<hipGraphExec_t> gexec;
+    // l bound of the batched P-SV forms' solve launch: the previous batch's
+    // longest cut-off and highest frequency (0: none yet in this run)
+    int lb_nl = 0, lb_f = 0;
     std::vector<SynthJob> jobs;
     int njobs_main = 0, njobs_syn = 0, njobs_acc = 0;  // launch order: main, accumulate, SYNN
     PsvPrep prep[2];
+    ShPrep shprep[2];
     // timing
     bool timing = true;
     std::vector<TimedLaunch> pending;
@@ -645,25 +669,41 @@ static inline void calamp(CutState &c, double amp, int l, double ratl, bool sh)
     else c.ismall = 0;
 }
 
-static int sh_run_batch(dsm_sgt_t *h, const std::vector<int> &freqs)
+// Host part of one SH batch: the band pieces a0, a2 and the scalars of
+// every frequency (SGTsh-MPI.f90:392-420), prepared on a host thread while
+// the previous batch runs on the device.
+static void sh_prepare(const ShSetup &s, const std::vector<int> &freqs, ShPrep &p)
 {
-    const ShSetup &s = h->sh;
-    const int S = (int)freqs.size(), nn = s.nn, R = s.r_n, LC = h->lcap;
-    std::vector<double2> band(4 * (size_t)nn * S), om(2 * S);
-    std::vector<CutState> cut(S);
-    std::vector<int> slotout(S);
+    const int S = (int)freqs.size(), nn = s.nn;
+    p.freqs = freqs;
+    p.band.resize(4 * (size_t)nn * S);
+    p.om.resize(2 * (size_t)S);
+    p.lsuf.resize(S);
     for (int k = 0; k < S; k++) {
         ShOmega o;
         sh_omega_setup(s, freqs[k], o);
         for (int j = 0; j < nn; j++) {
-            band[(size_t)(k * 4 + 0) * nn + j] = make_double2(o.a0[2 * j].real(), o.a0[2 * j].imag());
-            band[(size_t)(k * 4 + 1) * nn + j] = make_double2(o.a0[2 * j + 1].real(), o.a0[2 * j + 1].imag());
-            band[(size_t)(k * 4 + 2) * nn + j] = make_double2(o.a2[2 * j].real(), o.a2[2 * j].imag());
-            band[(size_t)(k * 4 + 3) * nn + j] = make_double2(o.a2[2 * j + 1].real(), o.a2[2 * j + 1].imag());
+            p.band[(size_t)(k * 4 + 0) * nn + j] = make_double2(o.a0[2 * j].real(), o.a0[2 * j].imag());
+            p.band[(size_t)(k * 4 + 1) * nn + j] = make_double2(o.a0[2 * j + 1].real(), o.a0[2 * j + 1].imag());
+            p.band[(size_t)(k * 4 + 2) * nn + j] = make_double2(o.a2[2 * j].real(), o.a2[2 * j].imag());
+            p.band[(size_t)(k * 4 + 3) * nn + j] = make_double2(o.a2[2 * j + 1].real(), o.a2[2 * j + 1].imag());
         }
-        om[2 * k] = make_double2(o.comega2.real(), o.comega2.imag());
-        om[2 * k + 1] = make_double2(o.coef_spn.real(), o.coef_spn.imag());
-        cut[k].lsuf = o.lsuf;
+        p.om[2 * k] = make_double2(o.comega2.real(), o.comega2.imag());
+        p.om[2 * k + 1] = make_double2(o.coef_spn.real(), o.coef_spn.imag());
+        p.lsuf[k] = o.lsuf;
+    }
+}
+
+static int sh_run_batch(dsm_sgt_t *h, const ShPrep &p)
+{
+    const ShSetup &s = h->sh;
+    const std::vector<int> &freqs = p.freqs;
+    const std::vector<double2> &band = p.band, &om = p.om;
+    const int S = (int)freqs.size(), nn = s.nn, R = s.r_n, LC = h->lcap;
+    std::vector<CutState> cut(S);
+    std::vector<int> slotout(S);
+    for (int k = 0; k < S; k++) {
+        cut[k].lsuf = p.lsuf[k];
         cut[k].llog = s.maxlmax;
         slotout[k] = freqs[k] - h->cat0;  // catalog position
     }
@@ -1140,7 +1180,11 @@ static void psv_nlr_estimate(const PsvSetup &s, const std::vector<int> &nl, cons
     }
 }
 
-static int psv_run_batch(dsm_sgt_t *h, const PsvPrep &p)
+// psv_run_batch's answer when a batch's cut-off was not reached within
+// its l bound: run the batch again with the full l range
+constexpr int PSV_RETRY_FULL = -100;
+
+static int psv_run_batch(dsm_sgt_t *h, const PsvPrep &p, bool full_range)
 {
     const PsvSetup &s = h->psv;
     const PsvGeom &g = h->pg;
@@ -1217,6 +1261,21 @@ static int psv_run_batch(dsm_sgt_t *h, const PsvPrep &p)
     // launch): everything after the factor-storage read-back runs as two
     // graphs, the solve-to-row-extent sequence and the synthesis pair
     const bool gmode = h->graph && h->ns && h->dcut > 0.0 && h->d_lsuf.p && h->lchunk > s.maxlmax;
+    // Batched forms (one solve launch, cut-off replayed on the device): the
+    // l range of the launch is bounded by the previous batch's longest
+    // cut-off, scaled to this batch's highest frequency, plus 10% and 128
+    // degrees.  A slot whose cut-off is not reached within the bound sends
+    // the batch back for the full range (the same systems, the same bits):
+    // deep sources stop at a small fraction of lmax.
+    // (DSM_SGT_LBOUND sets the 1.1 factor; 0 turns the bound off)
+    int lmaxb = s.maxlmax;
+    double lbf = 1.1;
+    if (const char *e = std::getenv("DSM_SGT_LBOUND")) lbf = std::atof(e);
+    if (!full_range && lbf > 0.0 && h->precision != DSM_SGT_FP64 && h->d_lsuf.p && h->lchunk > s.maxlmax &&
+        h->lb_nl > 0 && h->lb_f > 0) {
+        const double g = lbf * h->lb_nl * ((double)freqs.back() / h->lb_f) + 128.0 * (lbf >= 1.0);
+        if (g < s.maxlmax) lmaxb = std::min(s.maxlmax, ((int)g + 64) / 64 * 64 - 1);
+    }
     CaptureGuard cg;
     TimedLaunch t_solve;  // events of the current solve launch
     std::vector<int> nlr_est;
@@ -1226,11 +1285,11 @@ static int psv_run_batch(dsm_sgt_t *h, const PsvPrep &p)
     int l0 = 0;
     bool dev_cut = false;
     while (!active.empty()) {
-        const int lend = std::min(l0 + h->lchunk - 1, s.maxlmax);
+        const int lend = std::min(l0 + h->lchunk - 1, lmaxb);
         const int nlb = (lend - l0 + 64) / 64;
         const int nblocks = (int)active.size() * nlb;
         // batched forms in one launch: the cut-off is replayed on the device
-        dev_cut = h->precision != DSM_SGT_FP64 && h->d_lsuf.p && l0 == 0 && lend == s.maxlmax;
+        dev_cut = h->precision != DSM_SGT_FP64 && h->d_lsuf.p && l0 == 0 && lend == lmaxb;
         HIPCHK(h, hipMemcpyAsync(h->d_slots.p, active.data(), sizeof(int) * active.size(), hipMemcpyHostToDevice,
                                  h->stream));
         PsvSolveArgs sa{};
@@ -1280,7 +1339,7 @@ static int psv_run_batch(dsm_sgt_t *h, const PsvPrep &p)
             da.dcap = 0.35;
             da.kstart = h->d_kstart.as<int>();
             launch_psv_depth(da, h->stream);
-            if (l0 == 0 && lend == s.maxlmax) {
+            if (l0 == 0 && lend == lmaxb) {
                 // factor storage of the depth-cut systems: offsets per block, total
                 const int nbt = (int)active.size() * da.nblk;
                 long long *loff = h->d_loff.as<long long>();
@@ -1311,7 +1370,7 @@ static int psv_run_batch(dsm_sgt_t *h, const PsvPrep &p)
                 const int rc = begin_capture(h, cg);
                 if (rc) return rc;
             }
-            if (h->ns && l0 == 0 && lend == s.maxlmax) {
+            if (h->ns && l0 == 0 && lend == lmaxb) {
                 PsvNeedArgs na{h->d_kstart.as<int>(), h->d_rlow.as<int>(), (int)active.size(), da.nblk, R,
                                h->d_rneed.as<unsigned char>()};
                 launch_psv_need(na, h->stream);
@@ -1350,7 +1409,7 @@ static int psv_run_batch(dsm_sgt_t *h, const PsvPrep &p)
         if (h->ns) {
             ca.eb = B16Rows{h->d_eb.as<unsigned short>(), h->ns, S, R, LC};
             ca.esb = B16Rows{h->d_esb.as<unsigned short>(), h->ns, S, 1, LC};
-            if (h->dcut > 0.0 && l0 == 0 && lend == s.maxlmax) ca.rneed = h->d_rneed.as<unsigned char>();
+            if (h->dcut > 0.0 && l0 == 0 && lend == lmaxb) ca.rneed = h->d_rneed.as<unsigned char>();
         }
         launch_psv_coef(ca, nblocks, h->stream);
         HIPCHK(h, hipGetLastError());
@@ -1359,7 +1418,7 @@ static int psv_run_batch(dsm_sgt_t *h, const PsvPrep &p)
             HIPCHK(h, hipGetLastError());
         }
         if (dev_cut) {
-            PsvCutArgs cu{h->d_amp.as<double2>(), h->d_lsuf.as<int>(), S, LC, s.maxlmax, s.ratl, h->d_nl.as<int>(),
+            PsvCutArgs cu{h->d_amp.as<double2>(), h->d_lsuf.as<int>(), S, LC, lmaxb, s.ratl, h->d_nl.as<int>(),
                           h->d_llog.as<int>()};
             launch_psv_cut(cu, h->stream);
             HIPCHK(h, hipGetLastError());
@@ -1443,11 +1502,6 @@ static int psv_run_batch(dsm_sgt_t *h, const PsvPrep &p)
             HIPCHK(h, hipMemcpyAsync(h->d_nlr.p, nlr_est.data(), sizeof(int) * nlr_est.size(), hipMemcpyHostToDevice,
                                      h->stream));
         }
-        if (!gmode) {  // host estimate (the cut-off taken from nl) for the tile order
-            psv_nlr_estimate(s, nl, kst_host, kst_nblk, S, nlr_est);
-            if (h->xg >= 0) rc = b16_tile_order(h, nlr_est, S, R);
-            if (rc) return rc;
-        }
     } else {
         launch_zero_tail(h->d_E.as<double2>(), S, PSV_NQ * R, h->d_slots.as<int>(), h->d_nl.as<int>(), LC, h->stream);
         launch_zero_tail(h->d_Es.as<double2>(), S, PSV_NQS, h->d_slots.as<int>(), h->d_nl.as<int>(), LC, h->stream);
@@ -1459,6 +1513,22 @@ static int psv_run_batch(dsm_sgt_t *h, const PsvPrep &p)
         ev_end(h, t_solve);
     } else {
         ev_end(h, tz);
+    }
+    if (dev_cut && lmaxb < s.maxlmax) {
+        // the bound holds if every slot's cut-off fell within it
+        HIPCHK(h, hipMemcpyAsync(nl.data(), h->d_nl.p, sizeof(int) * S, hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(h, dev_sync(h));
+        for (int k = 0; k < S; k++)
+            if (nl[k] > lmaxb) {
+                free_graphs(h);
+                ev_collect(h);
+                return PSV_RETRY_FULL;
+            }
+    }
+    if (h->ns && !gmode) {  // host estimate of the l extents for the tile order
+        psv_nlr_estimate(s, nl, kst_host, kst_nblk, S, nlr_est);
+        if (h->xg >= 0) rc = b16_tile_order(h, nlr_est, S, R);
+        if (rc) return rc;
     }
     const bool tiled = h->ns && h->xg >= 0;
 
@@ -1520,6 +1590,8 @@ static int psv_run_batch(dsm_sgt_t *h, const PsvPrep &p)
             h->done[freqs[k] - h->ipos0] = 1;
             h->logged[freqs[k] - h->ipos0] = 1;
         }
+        h->lb_nl = *std::max_element(nl.begin(), nl.end());
+        h->lb_f = freqs.back();
     }
     double sumL = 0.0;
     for (int k = 0; k < S; k++) sumL += nl[k];
@@ -1584,6 +1656,8 @@ int dsm_sgt_create_tuned(const dsm_sgt_input_t *in, int32_t mode, int32_t precis
             break;
         }
         rc = (mode == DSM_SGT_SH) ? sh_create(h) : psv_create(h);
+        // cached blocks this engine did not take go back to the device
+        buf_cache().flush(device);
     } while (0);
     *handle = h;  // returned even on failure so dsm_sgt_last_error() can report
     return rc;
@@ -1739,6 +1813,7 @@ int dsm_sgt_run(dsm_sgt_t *h, int32_t i_begin, int32_t i_end)
     if (i_begin < h->ipos0 || i_end >= h->ipos0 + h->npos || i_begin > i_end)
         return fail(h, DSM_SGT_EINVAL, "frequency range outside [imin, imax] of the input");
     if (hipSetDevice(h->device) != hipSuccess) return fail(h, DSM_SGT_EHIP, "hipSetDevice failed");
+    h->lb_nl = h->lb_f = 0;
     if (h->ncat < h->npos) {
         // catalog window: this run's frequencies replace the resident ones
         const int first = std::max(i_begin, 1);
@@ -1763,16 +1838,25 @@ int dsm_sgt_run(dsm_sgt_t *h, int32_t i_begin, int32_t i_end)
             b += len;
         }
     }
+    // the host set-up of batch b+1 runs while batch b is on the device
+    const int nb = (int)batches.size();
+    if (nb == 0) return DSM_SGT_OK;
     if (h->mode == DSM_SGT_SH) {
-        for (auto &batch : batches) {
-            int rc = sh_run_batch(h, batch);
-            if (rc) return rc;
+        auto prepare = [h, &batches](int b) { sh_prepare(h->sh, batches[b], h->shprep[b & 1]); };
+        std::future<void> next = std::async(std::launch::async, prepare, 0);
+        for (int b = 0; b < nb; b++) {
+            const auto tw = std::chrono::steady_clock::now();
+            next.get();
+            h->stats.prep_wait_ms += ms_since(tw);
+            if (b + 1 < nb) next = std::async(std::launch::async, prepare, b + 1);
+            const int rc = sh_run_batch(h, h->shprep[b & 1]);
+            if (rc) {
+                if (b + 1 < nb) next.wait();
+                return rc;
+            }
         }
         return DSM_SGT_OK;
     }
-    // P-SV: the host set-up of batch b+1 runs while batch b is on the device
-    const int nb = (int)batches.size();
-    if (nb == 0) return DSM_SGT_OK;
     auto prepare = [h, &batches](int b) { return psv_prepare(h->psv, batches[b], h->prep[b & 1]); };
     std::future<void> next = std::async(std::launch::async, prepare, 0);
     for (int b = 0; b < nb; b++) {
@@ -1780,7 +1864,11 @@ int dsm_sgt_run(dsm_sgt_t *h, int32_t i_begin, int32_t i_end)
         next.get();
         h->stats.prep_wait_ms += ms_since(tw);
         if (b + 1 < nb) next = std::async(std::launch::async, prepare, b + 1);
-        int rc = psv_run_batch(h, h->prep[b & 1]);
+        int rc = psv_run_batch(h, h->prep[b & 1], false);
+        if (rc == PSV_RETRY_FULL) {
+            h->stats.solve_retries++;
+            rc = psv_run_batch(h, h->prep[b & 1], true);
+        }
         if (rc) {
             if (b + 1 < nb) next.wait();
             return rc;

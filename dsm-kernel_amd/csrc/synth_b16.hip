@@ -101,15 +101,27 @@ __global__ void split_rows_kernel(const double2 *E, int nslot, int nq, int R, in
     }
 }
 
-// zero the degrees l in [nl(slot), lend] of every row (the l cut-off):
-// one row per thread, nothing to do for slots that run to lend
-__global__ void zero_tail_b16_kernel(u16 *Eb, int nq, int ns, int nslot, int R, int lcap, const int *nl, int lend)
+// zero the degrees l in [nl(slot), min(lend, longest nl + 128)] of every
+// row (the l cut-off).  A synthesis tile reads its rows' degrees in chunks
+// of 32 even and 32 odd ones up to its longest series, so never past the
+// batch's longest cut-off + 128; rows from earlier batches may hold other
+// values there.  Rows in turn per workgroup, lanes along l (the even and
+// the odd degrees of a row are two contiguous runs).
+__global__ __launch_bounds__(256) void zero_tail_b16_kernel(u16 *Eb, int nq, int ns, int nslot, int R, int lcap,
+                                                            const int *nl, int lend)
 {
+    __shared__ int smax;
+    if (threadIdx.x == 0) smax = 0;
+    __syncthreads();
+    for (int i = threadIdx.x; i < nslot; i += blockDim.x) atomicMax(&smax, nl[i]);
+    __syncthreads();
+    const int lz = min(lend, smax + 128);
     const size_t rows = (size_t)nq * ns * nslot * R * 2;
     const size_t M = (size_t)nslot * R * 2;
-    for (size_t row = (size_t)blockIdx.x * blockDim.x + threadIdx.x; row < rows; row += (size_t)gridDim.x * blockDim.x) {
+    for (size_t row = blockIdx.x; row < rows; row += gridDim.x) {
         const int sl = (int)(((row % M) >> 1) % nslot);
-        for (int l = nl[sl]; l <= lend; l++) Eb[row * lcap + lperm(l, lcap)] = 0;
+        u16 *p = Eb + row * lcap;
+        for (int l = nl[sl] + (int)threadIdx.x; l <= lz; l += blockDim.x) p[lperm(l, lcap)] = 0;
     }
 }
 
@@ -356,7 +368,7 @@ void launch_split_rows(const double2 *E, int nslot, int nq, int r_eff, int lcap,
 void launch_zero_tail_b16(unsigned short *Eb, int nq, int ns, int nslot, int r_eff, int lcap, const int *nl, int lend,
                           hipStream_t s)
 {
-    hipLaunchKernelGGL(zero_tail_b16_kernel, dim3(4096), dim3(256), 0, s, Eb, nq, ns, nslot, r_eff, lcap, nl, lend);
+    hipLaunchKernelGGL(zero_tail_b16_kernel, dim3(8192), dim3(256), 0, s, Eb, nq, ns, nslot, r_eff, lcap, nl, lend);
 }
 
 void launch_zero_pad_b16(unsigned short *Eb, size_t nrows, int lcap, int lfrom, hipStream_t s)

@@ -34,7 +34,7 @@ class Stats(C.Structure):
                 ("other_ms", C.c_double), ("solve_launches", C.c_int64), ("synth_launches", C.c_int64),
                 ("systems", C.c_int64), ("solve_bytes", C.c_double), ("synth_flops", C.c_double),
                 ("synth_bytes", C.c_double), ("synth_mfma_flops", C.c_double),
-                ("prep_wait_ms", C.c_double), ("device_wait_ms", C.c_double)]
+                ("prep_wait_ms", C.c_double), ("device_wait_ms", C.c_double), ("solve_retries", C.c_int64)]
 
 
 class Tuning(C.Structure):

@@ -19,6 +19,12 @@ module dsm_sgt_api
      real(c_double) :: thetamin, thetamax, thetadelta
   end type dsm_sgt_input
 
+  ! dsm_sgt_tuning_t (launch plan; 0 = default)
+  type, bind(c) :: dsm_sgt_tuning
+     integer(c_int32_t) :: nslot = 0, lchunk = 0, xcd_group = 0, graph = 0, catalog_window = 0
+     integer(c_int32_t) :: reserved(3) = 0
+  end type dsm_sgt_tuning
+
   interface
      integer(c_int32_t) function dsm_sgt_create(inp, mode, precision, device, handle) bind(c)
        import :: c_int32_t, c_ptr, dsm_sgt_input
@@ -26,6 +32,21 @@ module dsm_sgt_api
        integer(c_int32_t), value :: mode, precision, device
        type(c_ptr), intent(out) :: handle
      end function dsm_sgt_create
+     integer(c_int32_t) function dsm_sgt_create_tuned(inp, mode, precision, device, plan, handle) bind(c)
+       import :: c_int32_t, c_ptr, dsm_sgt_input, dsm_sgt_tuning
+       type(dsm_sgt_input), intent(in) :: inp
+       integer(c_int32_t), value :: mode, precision, device
+       type(dsm_sgt_tuning), intent(in) :: plan
+       type(c_ptr), intent(out) :: handle
+     end function dsm_sgt_create_tuned
+     integer(c_int32_t) function dsm_sgt_autotune(inp, mode, precision, device, cache, plan, probe_ms) bind(c)
+       import :: c_int32_t, c_char, c_double, dsm_sgt_input, dsm_sgt_tuning
+       type(dsm_sgt_input), intent(in) :: inp
+       integer(c_int32_t), value :: mode, precision, device
+       character(kind=c_char), dimension(*), intent(in) :: cache
+       type(dsm_sgt_tuning), intent(out) :: plan
+       real(c_double), intent(out) :: probe_ms
+     end function dsm_sgt_autotune
      integer(c_int32_t) function dsm_sgt_run(handle, i0, i1) bind(c)
        import :: c_int32_t, c_ptr
        type(c_ptr), value :: handle

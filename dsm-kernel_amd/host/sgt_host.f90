@@ -47,7 +47,11 @@ contains
     integer :: n, nc, nm, ios, i, j, k, iz, sw(4), pos
     real(c_double) :: vals(28)
     type(c_ptr) :: h
-    integer(c_int32_t) :: rc, llog
+    integer(c_int32_t) :: rc, llog, prec, i0, i1, win
+    type(dsm_sgt_tuning) :: plan
+    real(c_double) :: probe
+    character(len=256) :: envv
+    integer :: est
 
     call read_records(5, recs, n)
     if (n < 10) call fail('parameter file: expected 10 records')
@@ -122,20 +126,52 @@ contains
     enddo
     list1 = trim(outdir)//'/log/list'//trim(kind)//'.'//trim(modelname)//'.'//tag
 
-    rc = dsm_sgt_create(inp, mode, DSM_SGT_FP64, 0_c_int32_t, h)
-    if (rc /= 0) call fail(dsm_sgt_message(h))
-    rc = dsm_sgt_run(h, inp%imin, inp%imax)
+    ! form: DSM_SGT_PRECISION = fp64 (default, the parity form) | fp32 |
+    ! bf16x3 | bf16x6 (the batched mixed form)
+    prec = DSM_SGT_FP64
+    call get_environment_variable('DSM_SGT_PRECISION', envv, status=est)
+    if (est == 0) then
+       select case (trim(envv))
+       case ('fp64'); prec = DSM_SGT_FP64
+       case ('fp32'); prec = DSM_SGT_FP32
+       case ('bf16x3'); prec = DSM_SGT_BF16X3
+       case ('bf16x6'); prec = DSM_SGT_BF16X6
+       case default; call fail('DSM_SGT_PRECISION: fp64, fp32, bf16x3 or bf16x6')
+       end select
+    endif
+    ! launch plan: autotuned when DSM_SGT_TUNE_CACHE names a cache file;
+    ! the device catalog holds DSM_SGT_WINDOW frequencies (default 128),
+    ! written out window by window
+    call get_environment_variable('DSM_SGT_TUNE_CACHE', envv, status=est)
+    if (est == 0 .and. len_trim(envv) > 0) then
+       rc = dsm_sgt_autotune(inp, mode, prec, 0_c_int32_t, trim(envv)//c_null_char, plan, probe)
+       if (rc /= 0) call fail('autotune failed')
+    endif
+    win = 128
+    call get_environment_variable('DSM_SGT_WINDOW', envv, status=est)
+    if (est == 0) read(envv, *) win
+    if (win < 1) call fail('DSM_SGT_WINDOW must be positive')
+    plan%catalog_window = win
+    rc = dsm_sgt_create_tuned(inp, mode, prec, 0_c_int32_t, plan, h)
     if (rc /= 0) call fail(dsm_sgt_message(h))
 
     open(24, file=list1, status='unknown', form='formatted')
     write(24, *)
-    do i = inp%imin, inp%imax
-       if (i == 0) cycle
-       rc = dsm_sgt_lcut(h, i, llog)
+    i0 = inp%imin
+    do while (i0 <= inp%imax)
+       ! win computed frequencies (i = 0 is skipped)
+       i1 = min(inp%imax, max(i0, 1_c_int32_t) + win - 1)
+       rc = dsm_sgt_run(h, i0, i1)
        if (rc /= 0) call fail(dsm_sgt_message(h))
-       write(24, *) i, dble(i) / inp%tlen, llog
-       rc = dsm_sgt_write_catalog(h, i, trim(outdir)//c_null_char, trim(modelname)//c_null_char)
-       if (rc /= 0) call fail(dsm_sgt_message(h))
+       do i = i0, i1
+          if (i == 0) cycle
+          rc = dsm_sgt_lcut(h, i, llog)
+          if (rc /= 0) call fail(dsm_sgt_message(h))
+          write(24, *) i, dble(i) / inp%tlen, llog
+          rc = dsm_sgt_write_catalog(h, i, trim(outdir)//c_null_char, trim(modelname)//c_null_char)
+          if (rc /= 0) call fail(dsm_sgt_message(h))
+       enddo
+       i0 = i1 + 1
     enddo
     close(24)
     call dsm_sgt_destroy(h)

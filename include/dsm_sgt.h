@@ -98,6 +98,8 @@ typedef struct dsm_sgt_stats {
     double synth_mfma_flops;  /* MFMA flops the synthesis kernels issued (bf16 forms: all cross products) */
     double prep_wait_ms;      /* host time the run waited for a batch's host set-up */
     double device_wait_ms;    /* host time the run blocked on the device inside batches */
+    int64_t solve_retries;    /* batched P-SV forms: batches run again with the full l range because a
+                                 cut-off fell past the batch's l bound */
 } dsm_sgt_stats_t;
 
 /* Create an engine on HIP device `device` for `mode` and `precision`.

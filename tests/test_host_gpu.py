@@ -87,3 +87,35 @@ def test_dropin_program_writes_the_catalog(tmp_path, prog, mode):
     rows = [ln.split() for ln in lst if ln.strip()]
     assert [int(rw[0]) for rw in rows] == list(range(1, imax + 1))
     assert int(rows[-1][2]) == oracle(inp, imax)[3]
+
+
+def test_dropin_program_mixed_form_in_windows(tmp_path, monkeypatch):
+    """The drop-in program with the batched mixed form
+    (DSM_SGT_PRECISION=bf16x6) and a catalog window of 3 frequencies
+    (DSM_SGT_WINDOW=3, so the 8 frequencies are written window by window):
+    the same files, records within the mixed form's small-grid bound of
+    the oracle, the same l cut-offs in the list file."""
+    monkeypatch.setenv("DSM_SGT_PRECISION", "bf16x6")
+    monkeypatch.setenv("DSM_SGT_WINDOW", "3")
+    imax = 8
+    inp = _run(tmp_path, "sgtpsv_gpu", "5421.d0 6371.d0 475.d0", imax)
+    out = tmp_path / "out"
+    info = ol.psv_info(inp)
+    radii = [inp.rmin_ + k * inp.rdelta_ for k in range(int((inp.rmax_ - inp.rmin_) / inp.rdelta_) + 1)]
+    T = info.theta_n
+    r0 = f"{int(6371.0e3):07d}"
+    lst = (out / "log" / f"list.m4.0000000.{imax:07d}.0000000").read_text().split("\n")
+    rows = [ln.split() for ln in lst if ln.strip()]
+    assert [int(rw[0]) for rw in rows] == list(range(1, imax + 1))
+    worst = 0.0
+    for i in (1, 3, 4, 8):
+        t, r, s, llog = ol.psv_omega(inp, i)
+        assert int(rows[i - 1][2]) == llog
+        gt = np.stack([np.fromfile(out / "TSGT" / f"m4.{r0}.{int(rr * 1e3):07d}.{i:07d}.TSGT_PSV",
+                                   np.complex64).reshape(T, 20) for rr in radii], axis=1)
+        gs = np.fromfile(out / "RSGT" / f"m4.{r0}.{i:07d}.SYNN_PSV", np.complex64).reshape(T, 10)
+        for x, y in ((gt, t), (gs, s)):
+            worst = max(worst, component_errors(x.astype(np.complex128), y.astype(np.complex64).astype(np.complex128),
+                                                floor=1e-5).max())
+    assert worst <= 8e-6, worst
+    print(f"drop-in bf16x6 in windows of 3: worst component error {worst:.2e}")

@@ -144,3 +144,50 @@ def test_catalog_window(mode_prec):
             w.fetch(1, dsm_sgt.TSGT, 0)
         assert e.value.code == 2  # ESTATE: replaced by the second run
         assert {i: w.lcut(i) for i in lc} == lc
+
+
+def _env_engine(inp, env, **kw):
+    import os
+    import dsm_sgt
+    old = os.environ.get("DSM_SGT_LBOUND")
+    os.environ["DSM_SGT_LBOUND"] = env
+    try:
+        return dsm_sgt.Engine(inp, **kw)
+    finally:
+        if old is None:
+            os.environ.pop("DSM_SGT_LBOUND", None)
+        else:
+            os.environ["DSM_SGT_LBOUND"] = old
+
+
+@pytest.mark.parametrize("factor", ["1.1", "0.05"])
+def test_l_bound_deep_source(factor):
+    """Deep source (600 km): the batched form bounds each batch's l range by
+    the previous batch's cut-off (factor 1.1, no retry expected), or by a
+    deliberately short bound (factor 0.05: batches are sent back for the
+    full range).  Either way the catalog and the cut-offs equal those of
+    full-range launches bit for bit, and the run is reported in
+    solve_retries."""
+    import os
+    import dsm_sgt
+    g = dict(GRID, r0=6371.0 - 600.0, imax=48)
+    inp = dsm_sgt.DsmSgtInput.from_buffer_copy(di.synthetic_input(**g))
+    freqs = (1, 8, 9, 30, 48)
+    with _env_engine(inp, "0", mode=dsm_sgt.PSV, precision=dsm_sgt.BF16X6, tuning={"nslot": 8}) as full:
+        full.run(1, g["imax"])
+        want = _catalog(full, freqs)
+        lc = [full.lcut(i) for i in freqs]
+        assert full.stats().solve_retries == 0
+    assert max(lc) < g["maxlmax"] // 2, lc  # the deep source's cut-off is far below lmax
+    os.environ["DSM_SGT_LBOUND"] = factor
+    try:
+        with dsm_sgt.Engine(inp, mode=dsm_sgt.PSV, precision=dsm_sgt.BF16X6, tuning={"nslot": 8}) as b:
+            b.run(1, g["imax"])
+            _same(_catalog(b, freqs), want)
+            assert [b.lcut(i) for i in freqs] == lc
+            retries = b.stats().solve_retries
+    finally:
+        os.environ.pop("DSM_SGT_LBOUND", None)
+    if factor == "0.05":
+        assert retries > 0
+    print(f"l bound x{factor}: cut-offs {lc}, {retries} batch(es) sent back for the full range")
