@@ -42,11 +42,10 @@ constexpr double PI = 3.1415926535897932;
 // Device blocks released by destroyed engines, kept for the next engine of
 // the process: an engine holds ~100 GB, and freeing it and allocating it
 // again has stalled engine creation for seconds.  Per device; a released
-// block is reused for a request of at least half its size.  The blocks a
-// new engine did not take are freed once it is built, and all blocks of a
-// device are freed before an allocation that would leave less than
-// kHbmReserve free; an allocation that would still leave less fails
-// (ENOMEM).  The runtime allocates kernel scratch (register spills) at
+// block is reused for a request of at least half its size.  All blocks of
+// a device are freed when a newly built engine leaves less than
+// kHbmReserve free, and before an allocation that would leave less; an
+// allocation that would still leave less fails (ENOMEM).  The runtime allocates kernel scratch (register spills) at
 // launch from the same memory, and a queue whose scratch allocation fails
 // aborts.  Never torn down (the process exit returns the memory).
 class BufCache {
@@ -160,6 +159,10 @@ struct ShPrep {
 struct dsm_sgt {
     int mode = 0, precision = 0, device = 0;
     hipStream_t stream = nullptr;
+    // BF16 P-SV forms: the l = 0 systems run on a second stream beside the
+    // (omega, l >= 1) solves (default priority, like the engine stream)
+    hipStream_t stream2 = nullptr;
+    hipEvent_t ev_l0in = nullptr, ev_l0out = nullptr;
     std::string err;
     dsm_sgt_input_t in{};
     ShSetup sh;
@@ -205,6 +208,9 @@ struct dsm_sgt {
     DevBuf d_rrow, d_vrow, d_omg, d_kstart, d_rlow, d_rneed, d_zs, d_loff, d_lsuf, d_llog, d_wsave;
     // fused FP64 family (mixed forms): compact band and its selector bits
     DevBuf d_cband, d_csel;
+    // l = 0 rows of the BF16 forms, written beside the solves and copied in
+    // after the coefficient kernel: [nq][ns][m] (one degree), slot list
+    DevBuf d_l0row, d_l0rows, d_l0slots;
     // graph mode: the executable graphs of the current batch (destroyed once
     // the batch has synchronised)
     std::vector<hipGraphExec_t> gexec;
@@ -313,11 +319,14 @@ static void launch_synth(dsm_sgt_t *h, const SynthArgs &ga, const DevBuf &Eb, co
     }
 }
 
-// bf16 split of the coefficient rows E [slot][nq][r_eff][lcap] into Eb
+// bf16 split of the coefficient rows E [slot][nq][r_eff][lcap] into Eb:
+// degrees below each slot's cut-off d_nl, zeros up to what the synthesis
+// reads
 static int split_rows(dsm_sgt_t *h, const DevBuf &E, int nslot, int nq, int r_eff, DevBuf &Eb)
 {
     if (!h->ns) return DSM_SGT_OK;
-    launch_split_rows(E.as<double2>(), nslot, nq, r_eff, h->lcap, h->ns, Eb.as<unsigned short>(), h->stream);
+    launch_split_rows(E.as<double2>(), nslot, nq, r_eff, h->lcap, h->ns, Eb.as<unsigned short>(), h->d_nl.as<int>(),
+                      h->stream);
     HIPCHK(h, hipGetLastError());
     return DSM_SGT_OK;
 }
@@ -807,8 +816,10 @@ static int sh_run_batch(dsm_sgt_t *h, const ShPrep &p)
     HIPCHK(h, hipMemcpyAsync(h->d_slots.p, all.data(), sizeof(int) * S, hipMemcpyHostToDevice, h->stream));
     TimedLaunch tz;
     ev_begin(h, tz, 3);
-    launch_zero_tail(h->d_E.as<double2>(), S, 36 * R, h->d_slots.as<int>(), h->d_nl.as<int>(), LC, h->stream);
-    launch_zero_tail(h->d_Es.as<double2>(), S, 5, h->d_slots.as<int>(), h->d_nl.as<int>(), LC, h->stream);
+    if (!h->ns) {  // (the bf16 split writes the zeros of the BF16 forms)
+        launch_zero_tail(h->d_E.as<double2>(), S, 36 * R, h->d_slots.as<int>(), h->d_nl.as<int>(), LC, h->stream);
+        launch_zero_tail(h->d_Es.as<double2>(), S, 5, h->d_slots.as<int>(), h->d_nl.as<int>(), LC, h->stream);
+    }
     HIPCHK(h, hipGetLastError());
     int rc = split_rows(h, h->d_E, S, 36, R, h->d_eb);
     if (!rc) rc = split_rows(h, h->d_Es, S, 5, 1, h->d_esb);
@@ -1032,6 +1043,11 @@ static int psv_create(dsm_sgt_t *h)
     // coefficient rows, l = 0 rows, cut-off tails, zero rows below the cut
     rc = alloc_b16(h, S, PSV_NQ, PSV_NQS, R, s.maxlmax + 1);
     if (rc) return rc;
+    if (h->ns) {
+        HIPCHK(h, h->d_l0row.ensure(sizeof(unsigned short) * PSV_NQ * h->ns * (size_t)S * R * 2));
+        HIPCHK(h, h->d_l0rows.ensure(sizeof(unsigned short) * PSV_NQS * h->ns * (size_t)S * 2));
+        HIPCHK(h, h->d_l0slots.ensure(sizeof(int) * S));
+    }
     HIPCHK(h, h->d_jobs.ensure(sizeof(SynthJob) * h->jobs.size()));
     HIPCHK(h, hipMemcpy(h->d_jobs.p, h->jobs.data(), sizeof(SynthJob) * h->jobs.size(), hipMemcpyHostToDevice));
 
@@ -1135,7 +1151,9 @@ static int begin_capture(dsm_sgt_t *h, CaptureGuard &cg)
     return DSM_SGT_OK;
 }
 
-static int end_capture_launch(dsm_sgt_t *h, CaptureGuard &cg)
+// t (may be null): timing events around the graph's launch only, after
+// its instantiation
+static int end_capture_launch(dsm_sgt_t *h, CaptureGuard &cg, TimedLaunch *t, int kind)
 {
     hipGraph_t g = nullptr;
     cg.on = false;
@@ -1147,7 +1165,9 @@ static int end_capture_launch(dsm_sgt_t *h, CaptureGuard &cg)
     (void)hipGraphDestroy(g);
     HIPCHK(h, e);
     h->gexec.push_back(x);
+    if (t) ev_begin(h, *t, kind);
     HIPCHK(h, hipGraphLaunch(x, h->stream));
+    if (t) ev_end(h, *t);
     return DSM_SGT_OK;
 }
 
@@ -1244,8 +1264,21 @@ static int psv_run_batch(dsm_sgt_t *h, const PsvPrep &p, bool full_range)
         za.Es = h->d_Es.as<double2>();
         za.lcap = LC;
         if (h->ns) {
-            za.eb = B16Rows{h->d_eb.as<unsigned short>(), h->ns, S, R, LC};
-            za.esb = B16Rows{h->d_esb.as<unsigned short>(), h->ns, S, 1, LC};
+            // one-degree rows (lcap 1) beside the solves, on the second
+            // stream once the band and the scalars are in place
+            za.eb = B16Rows{h->d_l0row.as<unsigned short>(), h->ns, S, R, 1};
+            za.esb = B16Rows{h->d_l0rows.as<unsigned short>(), h->ns, S, 1, 1};
+            za.slot_list = h->d_l0slots.as<int>();
+            HIPCHK(h, hipMemcpyAsync(h->d_l0slots.p, active.data(), sizeof(int) * S, hipMemcpyHostToDevice,
+                                     h->stream));
+            HIPCHK(h, hipEventRecord(h->ev_l0in, h->stream));
+            HIPCHK(h, hipStreamWaitEvent(h->stream2, h->ev_l0in, 0));
+            // rows the l = 0 system leaves unwritten stay zero (not captured)
+            HIPCHK(h, hipMemsetAsync(h->d_l0row.p, 0, h->d_l0row.n, h->stream2));
+            HIPCHK(h, hipMemsetAsync(h->d_l0rows.p, 0, h->d_l0rows.n, h->stream2));
+            launch_psv_l0(za, h->stream2);
+            HIPCHK(h, hipGetLastError());
+            HIPCHK(h, hipEventRecord(h->ev_l0out, h->stream2));
         }
         l0args = za;
         if (!h->ns) {
@@ -1367,6 +1400,10 @@ static int psv_run_batch(dsm_sgt_t *h, const PsvPrep &p, bool full_range)
                     const int rc = b16_tile_order(h, nlr_est, S, R);
                     if (rc) return rc;
                 }
+                ev_end(h, t0);  // the eager part of the solve stage
+                // (the capture takes no event of another stream: the l = 0
+                // stream is joined before it)
+                HIPCHK(h, hipStreamWaitEvent(h->stream, h->ev_l0out, 0));
                 const int rc = begin_capture(h, cg);
                 if (rc) return rc;
             }
@@ -1413,8 +1450,12 @@ static int psv_run_batch(dsm_sgt_t *h, const PsvPrep &p, bool full_range)
         }
         launch_psv_coef(ca, nblocks, h->stream);
         HIPCHK(h, hipGetLastError());
-        if (h->ns && l0 == 0) {
-            launch_psv_l0(l0args, h->stream);
+        if (h->ns && l0 == 0) {  // the l = 0 rows into degree 0 (the coefficient kernel staged zeros there)
+            if (!gmode) HIPCHK(h, hipStreamWaitEvent(h->stream, h->ev_l0out, 0));
+            launch_b16_put_l0(h->d_l0row.as<unsigned short>(), h->d_eb.as<unsigned short>(),
+                              (size_t)PSV_NQ * h->ns * S * R * 2, LC, h->stream);
+            launch_b16_put_l0(h->d_l0rows.as<unsigned short>(), h->d_esb.as<unsigned short>(),
+                              (size_t)PSV_NQS * h->ns * S * 2, LC, h->stream);
             HIPCHK(h, hipGetLastError());
         }
         if (dev_cut) {
@@ -1508,9 +1549,9 @@ static int psv_run_batch(dsm_sgt_t *h, const PsvPrep &p, bool full_range)
     }
     HIPCHK(h, hipGetLastError());
     if (gmode) {  // solve graph: excitation .. row extents
-        rc = end_capture_launch(h, cg);
+        TimedLaunch tga;
+        rc = end_capture_launch(h, cg, &tga, 0);
         if (rc) return rc;
-        ev_end(h, t_solve);
     } else {
         ev_end(h, tz);
     }
@@ -1546,10 +1587,11 @@ static int psv_run_batch(dsm_sgt_t *h, const PsvPrep &p, bool full_range)
     ga.jobs = h->d_jobs.as<SynthJob>();
     ga.njobs = h->njobs_main;
     TimedLaunch tg;
-    ev_begin(h, tg, 2);
     if (gmode) {
         rc = begin_capture(h, cg);
         if (rc) return rc;
+    } else {
+        ev_begin(h, tg, 2);
     }
     launch_synth(h, ga, h->d_eb, h->d_nlr.as<int>(), tiled ? h->d_tiles.as<int>() : nullptr, h->tiles_main_n);
     HIPCHK(h, hipGetLastError());
@@ -1559,10 +1601,11 @@ static int psv_run_batch(dsm_sgt_t *h, const PsvPrep &p, bool full_range)
                  h->tiles_acc_n);
     HIPCHK(h, hipGetLastError());
     if (gmode) {  // synthesis graph: the main and the accumulate launch
-        rc = end_capture_launch(h, cg);
+        rc = end_capture_launch(h, cg, &tg, 2);
         if (rc) return rc;
+    } else {
+        ev_end(h, tg);
     }
-    ev_end(h, tg);
     ga.E = h->d_Es.as<double2>();
     ga.nq = PSV_NQS;
     ga.r_eff = 1;
@@ -1651,13 +1694,18 @@ int dsm_sgt_create_tuned(const dsm_sgt_input_t *in, int32_t mode, int32_t precis
             rc = fail(h, DSM_SGT_EHIP, "hipSetDevice failed");
             break;
         }
-        if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
+        if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess ||
+            hipStreamCreateWithFlags(&h->stream2, hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&h->ev_l0in, hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&h->ev_l0out, hipEventDisableTiming) != hipSuccess) {
             rc = fail(h, DSM_SGT_EHIP, "hipStreamCreate failed");
             break;
         }
         rc = (mode == DSM_SGT_SH) ? sh_create(h) : psv_create(h);
-        // cached blocks this engine did not take go back to the device
-        buf_cache().flush(device);
+        // the cache gives its blocks back when the engine leaves less than
+        // the reserve free (kernel scratch is allocated at launch)
+        size_t fr = 0, tot = 0;
+        if (hipMemGetInfo(&fr, &tot) == hipSuccess && fr < kHbmReserve) buf_cache().flush(device);
     } while (0);
     *handle = h;  // returned even on failure so dsm_sgt_last_error() can report
     return rc;
@@ -1758,7 +1806,9 @@ int dsm_sgt_autotune(const dsm_sgt_input_t *in, int32_t mode, int32_t precision,
                         last_rc = rc;
                         break;
                     }
-                    if (best_ms < 0.0 || ms < best_ms) {
+                    // another plan must be faster by more than 1% (timing
+                    // noise): the candidates come simplest first
+                    if (best_ms < 0.0 || ms < 0.99 * best_ms) {
                         best_ms = ms;
                         *best = dsm_sgt_tuning_t{};
                         best->nslot = ns;
@@ -2053,6 +2103,14 @@ void dsm_sgt_destroy(dsm_sgt_t *h)
         free_graphs(h);
         ev_collect(h);
         (void)hipStreamDestroy(h->stream);
+    }
+    if (h->stream2) {
+        (void)hipStreamSynchronize(h->stream2);
+        (void)hipStreamDestroy(h->stream2);
+    }
+    if (h->ev_l0in) (void)hipEventDestroy(h->ev_l0in);
+    if (h->ev_l0out) {
+        (void)hipEventDestroy(h->ev_l0out);
     }
     delete h;
 }

@@ -102,7 +102,9 @@ struct SynthB16Args {
 void launch_zero_tail_b16(unsigned short *Eb, int nq, int ns, int nslot, int r_eff, int lcap, const int *nl, int lend,
                           hipStream_t s);
 void launch_split_rows(const double2 *E, int nslot, int nq, int r_eff, int lcap, int ns, unsigned short *Eb,
-                       hipStream_t s);
+                       const int *nl, hipStream_t s);
+// degree 0 of nrows bf16 rows [row][lcap] from one-degree rows one[row]
+void launch_b16_put_l0(const unsigned short *one, unsigned short *Eb, size_t nrows, int lcap, hipStream_t s);
 // zero the degrees l in [lfrom, lcap) of nrows bf16 rows
 void launch_zero_pad_b16(unsigned short *Eb, size_t nrows, int lcap, int lfrom, hipStream_t s);
 void launch_split_table(const double *tab, int nplanes, int lcap, int theta_pad, int nh, int nhp, int ns,

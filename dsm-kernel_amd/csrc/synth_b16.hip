@@ -10,7 +10,8 @@
 //   NS = 3 (BF16X6): the above + x0 y2 + x1 y1 + x2 y0
 // issued small terms first into one FP32 MFMA accumulator per 32-wide K
 // chunk, which is then added into FP64; so the error does not grow with
-// the length of the l series.
+// the length of the l series.  (Two chunks per FP32 accumulator measured
+// no faster and ~3x the error on the 5 s P-SV catalog.)
 //
 // Degree order.  Both operands store the degrees permuted, even l first
 // (l -> l/2) and odd l after (l -> lcap/2 + l/2), so every K chunk holds
@@ -39,6 +40,8 @@
 // the barrier makes every wave's loads visible.  Rows are 64 bytes, their
 // four 16-byte degree segments permuted per row (sw_seg), so the 16-byte
 // fragment reads of 8 and of 16 consecutive rows hit distinct banks.
+#include <climits>
+
 #include "dev_common.h"
 #include "kernels.h"
 
@@ -84,20 +87,32 @@ __device__ __forceinline__ bool xcd_tile_b16(int nx, int ny, int &tx, int &ty, i
 }  // namespace
 
 // E [slot][q][r][lcap] (FP64 complex) -> Eb [q][s][m][lcap] (bf16 terms,
-// degrees permuted), receiver-major rows m = (r nslot + slot) 2 + re|im
-__global__ void split_rows_kernel(const double2 *E, int nslot, int nq, int R, int lcap, int ns, u16 *Eb)
+// degrees permuted), receiver-major rows m = (r nslot + slot) 2 + re|im,
+// for the degrees a synthesis tile reads: l < nl(slot) split, l in
+// [nl(slot), lz) zero, lz = min(lcap, longest nl + 128) (a tile reads its
+// rows' degrees in chunks of 32 even and 32 odd ones up to its longest
+// series; degrees from lz up keep what earlier batches left).  Rows in
+// turn per workgroup, lanes along l.
+__global__ __launch_bounds__(256) void split_rows_kernel(const double2 *E, int nslot, int nq, int R, int lcap, int ns,
+                                                         u16 *Eb, const int *nl)
 {
-    const size_t total = (size_t)nslot * nq * R * lcap;
+    __shared__ int smax;
+    if (threadIdx.x == 0) smax = 0;
+    __syncthreads();
+    for (int i = threadIdx.x; i < nslot; i += blockDim.x) atomicMax(&smax, nl[i]);
+    __syncthreads();
+    const int lz = min(lcap, smax + 128);
+    const size_t rows = (size_t)nslot * nq * R;
     const B16Rows dst{Eb, ns, nslot, R, lcap};
-    for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
-         idx += (size_t)gridDim.x * blockDim.x) {
-        const int l = (int)(idx % lcap);
-        size_t rest = idx / lcap;
-        const int r = (int)(rest % R);
-        rest /= R;
+    for (size_t row = blockIdx.x; row < rows; row += gridDim.x) {
+        const int r = (int)(row % R);
+        const size_t rest = row / R;
         const int q = (int)(rest % nq);
         const int sl = (int)(rest / nq);
-        dst.put(q, r, sl, l, E[idx]);
+        const int n = nl[sl];
+        const double2 *src = E + row * lcap;
+        for (int l = threadIdx.x; l < lz; l += blockDim.x)
+            dst.put(q, r, sl, l, l < n ? src[l] : make_double2(0.0, 0.0));
     }
 }
 
@@ -110,19 +125,32 @@ __global__ void split_rows_kernel(const double2 *E, int nslot, int nq, int R, in
 __global__ __launch_bounds__(256) void zero_tail_b16_kernel(u16 *Eb, int nq, int ns, int nslot, int R, int lcap,
                                                             const int *nl, int lend)
 {
-    __shared__ int smax;
-    if (threadIdx.x == 0) smax = 0;
+    __shared__ int smax, smin;
+    if (threadIdx.x == 0) {
+        smax = 0;
+        smin = INT_MAX;
+    }
     __syncthreads();
-    for (int i = threadIdx.x; i < nslot; i += blockDim.x) atomicMax(&smax, nl[i]);
+    for (int i = threadIdx.x; i < nslot; i += blockDim.x) {
+        atomicMax(&smax, nl[i]);
+        atomicMin(&smin, nl[i]);
+    }
     __syncthreads();
     const int lz = min(lend, smax + 128);
-    const size_t rows = (size_t)nq * ns * nslot * R * 2;
-    const size_t M = (size_t)nslot * R * 2;
+    if (smin > lz) return;  // every slot runs past what the tiles read
+    const size_t rows = (size_t)nq * ns * nslot * R * 2, M = (size_t)nslot * R * 2;
     for (size_t row = blockIdx.x; row < rows; row += gridDim.x) {
         const int sl = (int)(((row % M) >> 1) % nslot);
         u16 *p = Eb + row * lcap;
         for (int l = nl[sl] + (int)threadIdx.x; l <= lz; l += blockDim.x) p[lperm(l, lcap)] = 0;
     }
+}
+
+// degree 0 of nrows bf16 rows from one-degree rows (lperm(0, lcap) = 0)
+__global__ void b16_put_l0_kernel(const u16 *one, u16 *Eb, size_t nrows, int lcap)
+{
+    for (size_t r = (size_t)blockIdx.x * blockDim.x + threadIdx.x; r < nrows; r += (size_t)gridDim.x * blockDim.x)
+        Eb[r * lcap] = one[r];
 }
 
 // zero the padding degrees l in [lfrom, lcap) of nrows rows (never written)
@@ -360,15 +388,20 @@ __global__ __launch_bounds__(512) void synth_b16_kernel(SynthB16Args a, int nx, 
 }
 
 void launch_split_rows(const double2 *E, int nslot, int nq, int r_eff, int lcap, int ns, unsigned short *Eb,
-                       hipStream_t s)
+                       const int *nl, hipStream_t s)
 {
-    hipLaunchKernelGGL(split_rows_kernel, dim3(8192), dim3(256), 0, s, E, nslot, nq, r_eff, lcap, ns, Eb);
+    hipLaunchKernelGGL(split_rows_kernel, dim3(8192), dim3(256), 0, s, E, nslot, nq, r_eff, lcap, ns, Eb, nl);
 }
 
 void launch_zero_tail_b16(unsigned short *Eb, int nq, int ns, int nslot, int r_eff, int lcap, const int *nl, int lend,
                           hipStream_t s)
 {
     hipLaunchKernelGGL(zero_tail_b16_kernel, dim3(8192), dim3(256), 0, s, Eb, nq, ns, nslot, r_eff, lcap, nl, lend);
+}
+
+void launch_b16_put_l0(const unsigned short *one, unsigned short *Eb, size_t nrows, int lcap, hipStream_t s)
+{
+    hipLaunchKernelGGL(b16_put_l0_kernel, dim3(1024), dim3(256), 0, s, one, Eb, nrows, lcap);
 }
 
 void launch_zero_pad_b16(unsigned short *Eb, size_t nrows, int lcap, int lfrom, hipStream_t s)

@@ -1,0 +1,53 @@
+"""GPU parity on the reference's own CPU-sized P-SV configuration (20 s
+period): tlen 1638.4 s, i = 1..82, re 1e-2 (about 2950 band rows), lmax
+2100, 20 receivers 5421..6371 km every 50 km, 361 distances 0..180 degrees
+every 0.5 degree (both poles), surface source, all three record kinds.
+
+The plain FP64 form against the oracle at the lowest frequency (where
+A(omega, l) is worst conditioned; the surface source runs to lmax, about a
+minute of oracle time): bar 1e-9 relative L2 per component and the same
+l cut-off.  The batched mixed
+form (BF16x6) on the same frequencies against the oracle within the
+mixed form's small-grid bound (a receiver at the source depth; see
+test_mixed_gpu.py)."""
+import pytest
+
+import dsm_inputs as di
+import oracle_lib as ol
+from parity import component_errors
+
+pytestmark = pytest.mark.gpu
+
+C1 = dict(tlen=1638.4, imax=82, rmin_=5421., rmax_=6371., rdelta_=50., thetamin=0., thetamax=180.,
+          thetadelta=0.5, maxlmax=2100, re=1e-2)
+FREQS = (1,)
+_ORACLE = {}
+
+
+def _oracle(inp, i):
+    if i not in _ORACLE:
+        _ORACLE[i] = ol.psv_omega(inp, i, fast=True)
+    return _ORACLE[i]
+
+
+@pytest.mark.parametrize("form,tol", [("FP64", 1e-9), ("BF16X6", 8e-6)])
+def test_c1_configuration_vs_oracle(form, tol):
+    import dsm_sgt
+    inp = di.synthetic_input(**C1)
+    eng = dsm_sgt.Engine(dsm_sgt.DsmSgtInput.from_buffer_copy(inp), mode=dsm_sgt.PSV,
+                         precision=getattr(dsm_sgt, form))
+    try:
+        assert eng.info.theta_n == 361 and eng.info.r_n == 20
+        eng.run(1, 8)
+        worst = 0.0
+        for i in FREQS:
+            t, r, s, llog = _oracle(inp, i)
+            assert eng.lcut(i) == llog, (i, eng.lcut(i), llog)
+            got = eng.fetch_all_fp64(i)
+            for name, x, y in zip(("tsgt", "rsgt", "synn"), got, (t, r, s)):
+                e = component_errors(x, y).max()
+                worst = max(worst, e)
+                assert e <= tol, (form, i, name, e)
+        print(f"C1 {form}: {eng.info.nn} band rows, frequencies {FREQS}, worst component error {worst:.2e}")
+    finally:
+        eng.close()
