@@ -337,11 +337,12 @@ def main():
         kernel, flops, peak = "synth_f64_kernel<0>", st.synth_flops / launches, FP64_MFMA_PEAK
         bound_note = "algorithmic flops, FP64 MFMA peak"
     achieved = flops / (launch_ms * 1e-3) if launch_ms > 0 else 0.0
-    # HBM bytes per launch of the dominant kernel, P-SV bf16x6 form: rocprofv3
-    # FETCH_SIZE and WRITE_SIZE per dispatch of synth_b16_kernel<3, 0> over
-    # this command's run, scaled by the counters of a plain 1 GiB copy
-    # (x1.991, x0.967); profiles/synth_b16_psv_5s_catalog_mixed.md
-    traffic = 2.351e10 if (args.mode == "psv" and args.form == "bf16x6") else None
+    # HBM bytes per launch of the dominant kernel, P-SV bf16x6 form with the
+    # tuned plan (128 frequencies per batch, tile groups of 2 per XCD):
+    # rocprofv3 FETCH_SIZE and WRITE_SIZE per dispatch of
+    # synth_b16_kernel<3, 0>, scaled by the counters of a plain 1 GiB copy
+    # (x1.9908, x0.9667); profiles/synth_b16_psv_5s_catalog_mixed.md
+    traffic = 2.847e10 if (args.mode == "psv" and args.form == "bf16x6") else None
     form_txt = {"bf16x6": "batched mixed: bf16x6 synthesis, FP64 solve with the depth cut",
                 "bf16x3": "batched mixed: bf16x3 synthesis, FP64 solve with the depth cut",
                 "fp32": "mixed: FP32 synthesis, FP64 solve with the depth cut", "fp64": "plain FP64"}[args.form]
