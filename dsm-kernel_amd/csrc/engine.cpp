@@ -42,9 +42,10 @@ constexpr double PI = 3.1415926535897932;
 // Device blocks released by destroyed engines, kept for the next engine of
 // the process: an engine holds ~100 GB, and freeing it and allocating it
 // again has stalled engine creation for seconds.  Per device; a released
-// block is reused for a request of at least half its size.  All blocks of
-// a device are freed when a newly built engine leaves less than
-// kHbmReserve free, and before an allocation that would leave less; an
+// block is reused for a request of at least half its size.  All cached
+// blocks of a device are freed when a newly built engine leaves less than
+// kHbmReserve free, and before an allocation that would leave less (freeing
+// them block by block as needed was measured slower: it repeats); an
 // allocation that would still leave less fails (ENOMEM).  The runtime allocates kernel scratch (register spills) at
 // launch from the same memory, and a queue whose scratch allocation fails
 // aborts.  Never torn down (the process exit returns the memory).
@@ -215,7 +216,7 @@ struct dsm_sgt {
     // the batch has synchronised)
     std::vector<hipGraphExec_t> gexec;
     // l bound of the batched P-SV forms' solve launch: the previous batch's
-    // longest cut-off and highest frequency (0: none yet in this run)
+    // longest cut-off and highest frequency (0: none yet)
     int lb_nl = 0, lb_f = 0;
     std::vector<SynthJob> jobs;
     int njobs_main = 0, njobs_syn = 0, njobs_acc = 0;  // launch order: main, accumulate, SYNN
@@ -480,30 +481,52 @@ static int alloc_b16(dsm_sgt_t *h, int nslot, int nq, int nqs, int R, int lpad)
 }
 
 // Synthesis jobs of the SH contraction (table in sh_jobs.h).
+// BF16 forms: a component whose theta functions have both parities under
+// theta -> pi - theta (Y0 with Y1; Y2, Z33 with Z23, Y3) becomes two jobs,
+// one per parity, the second accumulating, so that every job can use the
+// distance symmetry.  Launch order: main (writes), accumulate, SYNN.
 static void sh_synth_jobs(dsm_sgt_t *h)
 {
+    static const int fpar[6] = {1, -1, -1, 1, 1, -1};  // parity of Y0..Y5 without (-1)^(l+mu)
     h->jobs.clear();
     void *outs[3] = {h->d_cat_t.p, h->d_cat_r.p, h->d_cat_s.p};
     const int ncomp[3] = {10, 5, 5};
-    h->njobs_main = 0;
+    std::vector<SynthJob> main, acc, syn;
     for (int k = 0; k < SH_NJOBS; k++) {
         const ShJobDef &d = SH_JOBS[k];
         SynthJob j{};
-        j.q0 = d.q0;
-        j.nb = d.nb;
-        for (int t = 0; t < 4; t++) j.b[t] = (d.mu - 1) * 6 + d.b[t];
         j.comp[0] = d.comp;
         j.comp[1] = -1;
         j.sign[0] = 1.0;
         j.sign[1] = 0.0;
-        j.accumulate = 0;
         j.ncomp = ncomp[d.kind];
         j.out = (double2 *)outs[d.kind];
-        h->jobs.push_back(j);
-        if (d.kind < 2) h->njobs_main++;
+        // split point: the first plane whose parity differs from the first's
+        // (the planes of one parity are contiguous in SH_JOBS)
+        int cut = d.nb;
+        if (h->ns)
+            for (int t = 1; t < d.nb; t++)
+                if (fpar[d.b[t]] != fpar[d.b[0]]) {
+                    cut = t;
+                    break;
+                }
+        for (int part = 0; part < 2; part++) {
+            const int t0 = part ? cut : 0, t1 = part ? d.nb : cut;
+            if (t1 <= t0) continue;
+            SynthJob p = j;
+            p.q0 = d.q0 + t0;
+            p.nb = t1 - t0;
+            for (int t = 0; t < 4; t++) p.b[t] = (t < p.nb) ? (d.mu - 1) * 6 + d.b[t0 + t] : (d.mu - 1) * 6;
+            p.accumulate = part;
+            (d.kind == 2 ? syn : part ? acc : main).push_back(p);
+        }
     }
-    h->njobs_syn = (int)h->jobs.size() - h->njobs_main;
-    h->njobs_acc = 0;
+    h->jobs = main;
+    h->jobs.insert(h->jobs.end(), acc.begin(), acc.end());
+    h->jobs.insert(h->jobs.end(), syn.begin(), syn.end());
+    h->njobs_main = (int)main.size();
+    h->njobs_acc = (int)acc.size();
+    h->njobs_syn = (int)syn.size();
 }
 
 // Synthesis jobs of the P-SV contraction (table in psv_jobs.h), ordered
@@ -849,19 +872,25 @@ static int sh_run_batch(dsm_sgt_t *h, const ShPrep &p)
     ev_begin(h, tg, 2);
     launch_synth(h, ga, h->d_eb, h->d_nlr.as<int>());
     HIPCHK(h, hipGetLastError());
+    if (h->njobs_acc) {  // the second parity of split components
+        ga.jobs = h->d_jobs.as<SynthJob>() + h->njobs_main;
+        ga.njobs = h->njobs_acc;
+        launch_synth(h, ga, h->d_eb, h->d_nlr.as<int>());
+        HIPCHK(h, hipGetLastError());
+    }
     ev_end(h, tg);
     // SYNN rows (surface node, no receiver dimension)
     ga.E = h->d_Es.as<double2>();
     ga.nq = SH_NQS;
     ga.r_eff = 1;
-    ga.jobs = h->d_jobs.as<SynthJob>() + h->njobs_main;
+    ga.jobs = h->d_jobs.as<SynthJob>() + h->njobs_main + h->njobs_acc;
     ga.njobs = h->njobs_syn;
     TimedLaunch tg2;
     ev_begin(h, tg2, 3);
     launch_synth(h, ga, h->d_esb);
     HIPCHK(h, hipGetLastError());
     ev_end(h, tg2);
-    h->stats.synth_launches += 1;
+    h->stats.synth_launches += h->njobs_acc ? 2 : 1;
     double sumL = 0.0;
     for (int k = 0; k < S; k++) sumL += nl[k];
     // 4 flops per (complex coefficient x real basis) per (r,theta); 36 rows
@@ -1774,51 +1803,74 @@ int dsm_sgt_autotune(const dsm_sgt_input_t *in, int32_t mode, int32_t precision,
         nslots = {64, 32, 16};
         lchunks = {1024, 512, 256};
     }
-    const int ilo = std::max(1, std::max(in->imin, in->imax - nslots[0] + 1)), ihi = in->imax;
+    // the probe: the whole frequency range when it holds at most 1024
+    // frequencies (the plan that wins on the top frequencies alone was
+    // measured slower on the whole 5 s catalog); else the top frequencies,
+    // a run's short first batch (16) and two full batches of the largest
+    // candidate
+    const int first = std::max(1, in->imin), ihi = in->imax;
+    const int nprobe = (ihi - first + 1 <= 1024) ? ihi - first + 1 : 16 + 2 * nslots[0];
+    const int ilo = std::max(first, ihi - nprobe + 1);
     if (ihi < ilo) return DSM_SGT_EINVAL;
     double best_ms = -1.0;
     int last_rc = DSM_SGT_OK;
+    // time plan `t` (tile grouping and graph mode set on the live engine):
+    // two runs, the faster counted; a later plan must beat the best by more
+    // than 1% (timing noise), and the candidates come simplest first
+    auto probe = [&](dsm_sgt_t *h, const dsm_sgt_tuning_t &t) -> int {
+        h->xg = t.xcd_group;
+        h->graph = t.graph;
+        double ms = -1.0;
+        for (int rep = 0; rep < 2; rep++) {
+            const auto t0 = std::chrono::steady_clock::now();
+            const int rc = dsm_sgt_run(h, ilo, ihi);
+            if (rc) return rc;
+            const double dt = ms_since(t0);
+            if (ms < 0.0 || dt < ms) ms = dt;
+        }
+        if (best_ms < 0.0 || ms < 0.99 * best_ms) {
+            best_ms = ms;
+            *best = t;
+            best->catalog_window = 0;
+        }
+        return DSM_SGT_OK;
+    };
+    auto engine = [&](const dsm_sgt_tuning_t &t, dsm_sgt_t **h) -> int {
+        dsm_sgt_tuning_t c = t;
+        c.catalog_window = ihi - ilo + 1;  // the probe's frequencies only
+        const int rc = dsm_sgt_create_tuned(in, mode, precision, device, &c, h);
+        if (rc) {  // e.g. a batch that does not fit in device memory
+            dsm_sgt_destroy(*h);
+            *h = nullptr;
+        }
+        return rc;
+    };
+    // stage 1: batch size, l chunk and tile grouping, eager launches
     for (int ns : nslots)
         for (int lc : lchunks) {
             dsm_sgt_tuning_t t{};
             t.nslot = ns;
             t.lchunk = lc;
-            t.catalog_window = ihi - ilo + 1;  // the probe's frequencies only
             dsm_sgt_t *h = nullptr;
-            int rc = dsm_sgt_create_tuned(in, mode, precision, device, &t, &h);
-            if (rc) {  // e.g. a batch that does not fit in device memory
-                last_rc = rc;
-                dsm_sgt_destroy(h);
-                continue;
+            int rc = engine(t, &h);
+            for (int xg : xgs) {
+                if (rc) break;
+                t.xcd_group = xg;
+                rc = probe(h, t);
             }
-            for (int xg : xgs)
-                for (int gr : graphs) {
-                    h->xg = xg;
-                    h->graph = gr;
-                    double ms = -1.0;
-                    for (int rep = 0; rep < 2 && !rc; rep++) {
-                        const auto t0 = std::chrono::steady_clock::now();
-                        rc = dsm_sgt_run(h, ilo, ihi);
-                        const double dt = ms_since(t0);
-                        if (!rc && (ms < 0.0 || dt < ms)) ms = dt;
-                    }
-                    if (rc) {
-                        last_rc = rc;
-                        break;
-                    }
-                    // another plan must be faster by more than 1% (timing
-                    // noise): the candidates come simplest first
-                    if (best_ms < 0.0 || ms < 0.99 * best_ms) {
-                        best_ms = ms;
-                        *best = dsm_sgt_tuning_t{};
-                        best->nslot = ns;
-                        best->lchunk = lc;
-                        best->xcd_group = xg;
-                        best->graph = gr;
-                    }
-                }
+            if (rc) last_rc = rc;
             dsm_sgt_destroy(h);
         }
+    // stage 2: graph mode for the winner
+    if (best_ms >= 0.0 && graphs.size() > 1) {
+        dsm_sgt_tuning_t t = *best;
+        t.graph = 1;
+        dsm_sgt_t *h = nullptr;
+        int rc = engine(t, &h);
+        if (!rc) rc = probe(h, t);
+        if (rc) last_rc = rc;
+        dsm_sgt_destroy(h);
+    }
     if (best_ms < 0.0) return last_rc ? last_rc : DSM_SGT_EHIP;
     if (probe_ms) *probe_ms = best_ms;
     if (cache_path) {
@@ -1863,7 +1915,9 @@ int dsm_sgt_run(dsm_sgt_t *h, int32_t i_begin, int32_t i_end)
     if (i_begin < h->ipos0 || i_end >= h->ipos0 + h->npos || i_begin > i_end)
         return fail(h, DSM_SGT_EINVAL, "frequency range outside [imin, imax] of the input");
     if (hipSetDevice(h->device) != hipSuccess) return fail(h, DSM_SGT_EHIP, "hipSetDevice failed");
-    h->lb_nl = h->lb_f = 0;
+    // (the l bound keeps the last batch's cut-off across runs: a run of the
+    // next catalog window continues from it; a wrong guess only costs a
+    // full-range re-run of the batch)
     if (h->ncat < h->npos) {
         // catalog window: this run's frequencies replace the resident ones
         const int first = std::max(i_begin, 1);

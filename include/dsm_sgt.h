@@ -131,12 +131,12 @@ int dsm_sgt_create_tuned(const dsm_sgt_input_t *in, int32_t mode, int32_t precis
 /* The launch plan an engine runs with (defaults filled in). */
 int dsm_sgt_get_tuning(dsm_sgt_t *h, dsm_sgt_tuning_t *tuning);
 /* Autotuner: times the candidate launch plans for this input, mode and
- * precision on a probe of the most expensive frequencies (the top
- * frequencies of [imin, imax], as many as the largest candidate batch),
- * each plan run twice, the faster run counted, and returns the fastest
- * plan in `best` and its probe time in `probe_ms`.  Candidates: the batch
- * size, the l chunk (SH, FP64 P-SV) and the synthesis tile grouping (BF16
- * forms).  With `cache_path` (may be NULL) the result is looked up in and
+ * precision on a probe run (all of [imin, imax] when it holds at most 1024
+ * frequencies, else its top 16 + twice the largest candidate batch), each
+ * plan run twice, the faster run counted, and returns the fastest plan in
+ * `best` and its probe time in `probe_ms`.  Candidates: the batch size,
+ * the l chunk (SH, FP64 P-SV) and the synthesis tile grouping (BF16
+ * forms); then graph mode for the winner (batched P-SV forms).  With `cache_path` (may be NULL) the result is looked up in and
  * appended to that text file, keyed by a hash of the input, mode,
  * precision and device name; a cached plan returns with probe_ms = 0.
  * The probe engines' catalogs are discarded. */
