@@ -117,7 +117,16 @@ def run_c5(args):
     spots = [int(x) for x in spots if x <= imax] if args.imax else [int(x) for x in spots]
     modes = (("psv", dsm_sgt.PSV), ("sh", dsm_sgt.SH))
     plans, tune_s = {}, {}
-    for name, mode in modes:  # tuned once per mode, on the deepest source (the largest buffers)
+    total = {"psv": 0.0, "sh": 0.0}
+    worst = {"psv": {"tsgt": 0.0, "rsgt": 0.0, "synn": 0.0}, "sh": {"tsgt": 0.0, "rsgt": 0.0, "synn": 0.0}}
+    bytes64, checked = 0.0, []
+    chosen = list(enumerate(depths[:args.c5_depths]))
+    rows = {d: {"depth_km": float(depth)} for d, depth in chosen}
+    # one mode at a time, tuned right before its depths, deepest source first:
+    # the engines of the deepest source need the largest buffers, which the
+    # shallower sources' engines of the same mode then reuse (device blocks
+    # are cached across engines) instead of allocating ~100 GB again
+    for name, mode in modes:
         p = dict(C5, imax=imax, r0=6371.0 - float(depths.max()))
         t0 = time.perf_counter()
         if args.plan:
@@ -129,19 +138,11 @@ def run_c5(args):
         plan.catalog_window = C5_WINDOW
         plans[name], tune_s[name] = plan, time.perf_counter() - t0
         print(f"c5: {name} plan {plan.as_dict()} ({tune_s[name]:.1f} s)", file=sys.stderr, flush=True)
-    total = {"psv": 0.0, "sh": 0.0}
-    worst = {"psv": {"tsgt": 0.0, "rsgt": 0.0, "synn": 0.0}, "sh": {"tsgt": 0.0, "rsgt": 0.0, "synn": 0.0}}
-    per_depth, bytes64, checked = [], 0.0, []
-    # deepest source first: its engines need the largest buffers, which the
-    # shallower sources' engines then reuse (device blocks are cached across
-    # engines) instead of allocating and freeing ~100 GB in between
-    chosen = list(enumerate(depths[:args.c5_depths]))
-    for d, depth in sorted(chosen, key=lambda x: -x[1]):
-        p = dict(C5, imax=imax, r0=6371.0 - float(depth))
-        inp = make_input(p)
-        my_spots = spots[4 * d:4 * d + 4]
-        row = {"depth_km": float(depth)}
-        for name, mode in modes:
+        for d, depth in sorted(chosen, key=lambda x: -x[1]):
+            p = dict(C5, imax=imax, r0=6371.0 - float(depth))
+            inp = make_input(p)
+            my_spots = spots[4 * d:4 * d + 4]
+            row = rows[d]
             t0 = time.perf_counter()
             eng = dsm_sgt.Engine(inp, mode=mode, precision=prec, tuning=plans[name])
             t_run = time.perf_counter() - t0
@@ -176,9 +177,10 @@ def run_c5(args):
             total[name] += t_run
             row[name + "_s"] = t_run
             row[name + "_lcut_max"] = lcut_max
-        per_depth.append(row)
-        print(f"c5: depth {d + 1}/{args.c5_depths} ({depth:.1f} km, {len(per_depth)} done): P-SV {row['psv_s']:.2f} s, SH {row['sh_s']:.2f} s; "
-              f"worst spot errors so far {worst}", file=sys.stderr, flush=True)
+            print(f"c5: {name} depth {d + 1}/{args.c5_depths} ({depth:.1f} km): {t_run:.2f} s "
+                  f"(create {row[name + '_create_s']:.2f} s); worst spot errors so far {worst[name]}",
+                  file=sys.stderr, flush=True)
+    per_depth = [rows[d] for d, _ in chosen]
     value = total["psv"] + total["sh"]
     out = {
         "metric": "SGT catalog wall time, multi-depth 2.5 s P-SV + SH (batched mixed form)",
