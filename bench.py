@@ -314,6 +314,7 @@ def main():
     st = eng.stats()
     cuts = [eng.lcut(i) for i in (1, p["imax"])]
     info = eng.info
+    eng_plan = eng.tuning().as_dict()
     ms = 1e3 * sum(times) / len(times)
     freqs = sorted(set(int(round(x)) for x in [1 + k * (p["imax"] - 1) / 16 for k in range(17)]))
     compare = args.form != "fp64" and not args.no_fp64
@@ -348,12 +349,16 @@ def main():
         kernel, flops, peak = "synth_f64_kernel<0>", st.synth_flops / launches, FP64_MFMA_PEAK
         bound_note = "algorithmic flops, FP64 MFMA peak"
     achieved = flops / (launch_ms * 1e-3) if launch_ms > 0 else 0.0
-    # HBM bytes per launch of the dominant kernel, P-SV bf16x6 form with the
-    # tuned plan (128 frequencies per batch, tile groups of 2 per XCD):
+    # HBM bytes per launch of the dominant kernel, P-SV bf16x6 form, 128
+    # frequencies per batch, by the synthesis tile grouping of the plan:
     # rocprofv3 FETCH_SIZE and WRITE_SIZE per dispatch of
     # synth_b16_kernel<3, 0>, scaled by the counters of a plain 1 GiB copy
-    # (x1.9908, x0.9667); profiles/synth_b16_psv_5s_catalog_mixed.md
-    traffic = 2.847e10 if (args.mode == "psv" and args.form == "bf16x6") else None
+    # (x1.991, x0.967); profiles/synth_b16_psv_5s_catalog_mixed.md
+    traffic = None
+    if args.mode == "psv" and args.form == "bf16x6" and not args.imax:
+        tp = eng_plan if eng_plan is not None else {"nslot": 128, "xcd_group": 0}
+        if tp.get("nslot") in (0, 128):
+            traffic = {0: 2.351e10, 2: 2.847e10}.get(tp.get("xcd_group"))
     form_txt = {"bf16x6": "batched mixed: bf16x6 synthesis, FP64 solve with the depth cut",
                 "bf16x3": "batched mixed: bf16x3 synthesis, FP64 solve with the depth cut",
                 "fp32": "mixed: FP32 synthesis, FP64 solve with the depth cut", "fp64": "plain FP64"}[args.form]
