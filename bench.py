@@ -57,17 +57,13 @@ def make_input(p):
     return mk(zones, conf, params)
 
 
-def cpu_baseline(mode, p, r_n, lsample=1000):
-    """Time the oracle (the C restatement of the reference's driver, one
-    thread) on one frequency of the same radial grid with all receivers,
-    the first `lsample` degrees and two small distance subsets; fit
-    t = L*(a + b*theta_n) (every per-l step costs the same) and extrapolate
-    to the full l range, distances and frequencies.  Every frequency of the
-    surface-source catalog runs the full l range (the GPU run reports the l
-    cut-off of the first and last frequency)."""
+def _oracle_sample(args):
+    """One frequency of the CPU sample (run in a worker process): the
+    oracle on the first `lsample` degrees with 2 and then 12 distances."""
+    mode, p, i, lsample = args
+    sys.path.insert(0, os.path.join(HERE, "tests"))
     import oracle_lib as ol
     import dsm_inputs as di
-    i = p["imax"] // 2
     pts = []
     for nth in (2, 12):
         inp = di.synthetic_input(tlen=p["tlen"], imax=p["imax"], rmin_=p["rmin_"], rmax_=p["rmax_"],
@@ -76,17 +72,41 @@ def cpu_baseline(mode, p, r_n, lsample=1000):
         t0 = time.perf_counter()
         (ol.psv_omega if mode == "psv" else ol.sh_omega)(inp, i, fast=True)
         pts.append((nth, time.perf_counter() - t0))
-    (n1, t1), (n2, t2) = pts
-    b = (t2 - t1) / (n2 - n1)
-    a = max(t1 - b * n1, 0.0)
+    return i, pts
+
+
+def cpu_baseline(mode, p, r_n, lsample=1000):
+    """Time the oracle (the C restatement of the reference's driver) on the
+    host's cores, one frequency per core as the reference gives one
+    frequency to each MPI rank: on each of C frequencies spread over the
+    catalog, the same radial grid with all receivers, the first `lsample`
+    degrees and two small distance subsets.  Per frequency, fit
+    t = L*(a + b*theta_n) (every per-l step costs the same) and extrapolate
+    to the full l range and distances; the catalog's core time is the mean
+    over the sampled frequencies times their number, its wall time on C
+    cores that divided by C (frequencies are independent).  Every
+    frequency of the surface-source catalog runs the full l range (the GPU
+    run reports the l cut-off of the first and last frequency).  Worker
+    processes are started fresh (spawn): they never touch the GPU."""
+    import multiprocessing as mp
+    cores = max(1, min(16, os.cpu_count() or 1))
+    freqs = sorted(set(max(1, int(round(p["imax"] * (k + 0.5) / cores))) for k in range(cores)))
+    t0 = time.perf_counter()
+    with mp.get_context("spawn").Pool(len(freqs)) as pool:
+        res = pool.map(_oracle_sample, [(mode, p, i, lsample) for i in freqs])
+    wall = time.perf_counter() - t0
     nfull = int(round((p["thetamax"] - p["thetamin"]) / p["thetadelta"])) + 1
-    per_omega = (a + b * nfull) * (p["maxlmax"] + 1) / (lsample + 1)
-    total = per_omega * p["imax"]
-    return dict(value=total, unit="s", cores=1, kind="port",
-                sample=f"oracle (gcc -O3, 1 thread) on frequency {i} of the same grid, {r_n} radii, "
-                       f"l=0..{lsample}, {n1} and {n2} distances ({t1:.1f} s, {t2:.1f} s); "
-                       f"t = L*(a + b*theta_n) extrapolated to l=0..{p['maxlmax']}, {nfull} distances "
-                       f"x {p['imax']} frequencies")
+    per_omega = []
+    for i, ((n1, t1), (n2, t2)) in res:
+        b = (t2 - t1) / (n2 - n1)
+        a = max(t1 - b * n1, 0.0)
+        per_omega.append((a + b * nfull) * (p["maxlmax"] + 1) / (lsample + 1))
+    core_s = sum(per_omega) / len(per_omega) * p["imax"]
+    return dict(value=core_s / len(freqs), unit="s", cores=len(freqs), kind="port", core_seconds=core_s,
+                sample=f"oracle (gcc -O3, one frequency per core) on frequencies {freqs} of the same grid, "
+                       f"{r_n} radii, l=0..{lsample}, 2 and 12 distances ({wall:.1f} s wall); per frequency "
+                       f"t = L*(a + b*theta_n) extrapolated to l=0..{p['maxlmax']}, {nfull} distances; mean x "
+                       f"{p['imax']} frequencies / {len(freqs)} cores")
 
 
 # Multi-depth 2.5 s workload: P-SV and SH catalogs for 16 source depths

@@ -224,7 +224,7 @@ __host__ __device__ inline void sh_solve_lane(const ShSolveArgs &a, int b, int t
         if (p < a.nneed && a.need[p] == j) {
             for (int s = 0; s < 10; s++) xo[(size_t)(SH_NSOL * p + s) * a.lcap] = x[s];
             xo[(size_t)(SH_NSOL * p + 10) * a.lcap] = xs;
-            p++;
+            if (++p == a.nneed) break;  // (need is descending: no row below is recorded)
         }
     }
 }
