@@ -79,3 +79,33 @@ def test_input_readers_agree():
     assert a.nzone == 4 and a.imax == 328 and abs(a.thetadelta - 0.1) < 1e-15
     assert a.vsv[1][0] == 0.0 and a.vsv[2][0] == 10.6998604327 and a.qkappa[3] == 57823.0
     assert (a.rsgtswitch, a.tsgtswitch, a.synnswitch, a.psgtswitch) == (1, 1, 1, 0)
+
+
+def test_struct_layouts_match_the_header(tmp_path):
+    """The Python bindings' structures have the C-ABI's sizes and field
+    offsets (dsm_sgt_tuning_t, dsm_sgt_stats_t, dsm_sgt_info_t,
+    dsm_sgt_input_t), checked against the header compiled by gcc."""
+    import shutil
+    import subprocess
+    import dsm_sgt
+    if shutil.which("gcc") is None:
+        pytest.skip("no gcc")
+    src = tmp_path / "layout.c"
+    src.write_text("""#include <stddef.h>
+#include <stdio.h>
+#include "dsm_sgt.h"
+int main(void) {
+  printf("%zu %zu %zu %zu\\n", sizeof(dsm_sgt_tuning_t), sizeof(dsm_sgt_stats_t), sizeof(dsm_sgt_info_t),
+         sizeof(dsm_sgt_input_t));
+  printf("%zu %zu %zu\\n", offsetof(dsm_sgt_tuning_t, catalog_window), offsetof(dsm_sgt_stats_t, solve_retries),
+         offsetof(dsm_sgt_info_t, r0));
+  return 0;
+}
+""")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    sizes, offs = [list(map(int, ln.split())) for ln in subprocess.run([str(exe)], capture_output=True, text=True,
+                                                                        check=True).stdout.split("\n")[:2]]
+    assert sizes == [C.sizeof(dsm_sgt.Tuning), C.sizeof(dsm_sgt.Stats), C.sizeof(dsm_sgt.Info),
+                     C.sizeof(dsm_sgt.DsmSgtInput)]
+    assert offs == [dsm_sgt.Tuning.catalog_window.offset, dsm_sgt.Stats.solve_retries.offset, dsm_sgt.Info.r0.offset]
