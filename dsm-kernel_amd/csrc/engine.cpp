@@ -152,6 +152,7 @@ struct PsvPrep {
 
 struct ShPrep {
     std::vector<int> freqs;
+    std::vector<double> omg;    // [slot] real angular frequency (depth cut)
     std::vector<double2> band;  // [slot][a0 re/im pair, a2 pair][nn]
     std::vector<double2> om;    // [slot][omega^2, coef at the source]
     std::vector<int> lsuf;
@@ -657,6 +658,33 @@ static int sh_create(dsm_sgt_t *h)
     HIPCHK(h, h->d_cat_t.ensure(sizeof(double2) * np * R * s.theta_n * 10));
     HIPCHK(h, h->d_cat_r.ensure(sizeof(double2) * np * R * s.theta_n * 5));
     HIPCHK(h, h->d_cat_s.ensure(sizeof(double2) * np * s.theta_n * 5));
+    // depth cut of the mixed forms (DSM_SGT_DEPTH_CUT overrides; 0 disables):
+    // node radius and smaller shear speed per band row
+    h->dcut = (h->precision == DSM_SGT_FP64) ? 0.0 : 40.0;
+    if (const char *e = std::getenv("DSM_SGT_DEPTH_CUT")) h->dcut = std::max(0.0, std::atof(e));
+    if (h->dcut > 0.0) {
+        std::vector<double> rr(s.nn), vv(s.nn);
+        for (int j = 0; j < s.nn; j++) {
+            const double r = s.ra[j];
+            int iz = 0;
+            while (iz + 1 < s.nzone && r > s.vrmax[iz]) iz++;
+            double x = 1.0, a = 0.0, c = 0.0;
+            for (int q = 0; q < 4; q++) {
+                a += s.vsv[4 * iz + q] * x;
+                c += s.vsh[4 * iz + q] * x;
+                x *= r / s.rmax;
+            }
+            rr[j] = r;
+            vv[j] = std::min(a, c);
+        }
+        const int nlbmax = (h->lchunk + 63) / 64 + 1;
+        HIPCHK(h, h->d_rrow.ensure(sizeof(double) * s.nn));
+        HIPCHK(h, h->d_vrow.ensure(sizeof(double) * s.nn));
+        HIPCHK(h, h->d_omg.ensure(sizeof(double) * S));
+        HIPCHK(h, h->d_kstart.ensure(sizeof(int) * (size_t)S * nlbmax));
+        HIPCHK(h, hipMemcpy(h->d_rrow.p, rr.data(), sizeof(double) * s.nn, hipMemcpyHostToDevice));
+        HIPCHK(h, hipMemcpy(h->d_vrow.p, vv.data(), sizeof(double) * s.nn, hipMemcpyHostToDevice));
+    }
     sh_synth_jobs(h);
     {
         // parity of Y0..Y5 (legendre_lane.h) times (-1)^mu
@@ -711,9 +739,11 @@ static void sh_prepare(const ShSetup &s, const std::vector<int> &freqs, ShPrep &
     p.band.resize(4 * (size_t)nn * S);
     p.om.resize(2 * (size_t)S);
     p.lsuf.resize(S);
+    p.omg.resize(S);
     for (int k = 0; k < S; k++) {
         ShOmega o;
         sh_omega_setup(s, freqs[k], o);
+        p.omg[k] = o.omega;
         for (int j = 0; j < nn; j++) {
             p.band[(size_t)(k * 4 + 0) * nn + j] = make_double2(o.a0[2 * j].real(), o.a0[2 * j].imag());
             p.band[(size_t)(k * 4 + 1) * nn + j] = make_double2(o.a0[2 * j + 1].real(), o.a0[2 * j + 1].imag());
@@ -742,6 +772,8 @@ static int sh_run_batch(dsm_sgt_t *h, const ShPrep &p)
     HIPCHK(h, hipMemcpyAsync(h->d_band.p, band.data(), sizeof(double2) * band.size(), hipMemcpyHostToDevice, h->stream));
     HIPCHK(h, hipMemcpyAsync(h->d_om.p, om.data(), sizeof(double2) * om.size(), hipMemcpyHostToDevice, h->stream));
     HIPCHK(h, hipMemcpyAsync(h->d_slotout.p, slotout.data(), sizeof(int) * S, hipMemcpyHostToDevice, h->stream));
+    if (h->dcut > 0.0)
+        HIPCHK(h, hipMemcpyAsync(h->d_omg.p, p.omg.data(), sizeof(double) * S, hipMemcpyHostToDevice, h->stream));
 
     std::vector<int> active(S);
     for (int k = 0; k < S; k++) active[k] = k;
@@ -771,6 +803,13 @@ static int sh_run_batch(dsm_sgt_t *h, const ShPrep &p)
         sa.src = h->d_src.as<ShSrcConst>();
         TimedLaunch t0;
         ev_begin(h, t0, 0);
+        if (h->dcut > 0.0) {  // mixed forms: first solved row per block of 64 degrees
+            ShDepthArgs da{h->d_rrow.as<double>(), h->d_vrow.as<double>(), h->d_omg.as<double>(),
+                           h->d_slots.as<int>(), (int)active.size(), nlb, l0, s.ns, h->dcut, 0.35,
+                           h->d_kstart.as<int>()};
+            launch_sh_depth(da, h->stream);
+            sa.jstart = h->d_kstart.as<int>();
+        }
         launch_sh_solve(sa, nblocks, h->stream);
         HIPCHK(h, hipGetLastError());
         ShCoefArgs ca{};

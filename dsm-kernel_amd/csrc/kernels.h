@@ -12,6 +12,7 @@ namespace dsm {
 // SH lane bodies and argument structs: sh_lane.h
 void launch_sh_solve(const ShSolveArgs &a, int nblocks, hipStream_t s);
 void launch_sh_coef(const ShCoefArgs &a, int nblocks, hipStream_t s);
+void launch_sh_depth(const ShDepthArgs &a, hipStream_t s);
 
 // P-SV lane bodies and argument structs: psv_lane.h
 // excitation g of every (frequency, l) into a.gexc, then the band solve;

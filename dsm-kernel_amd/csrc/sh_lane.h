@@ -37,6 +37,18 @@ struct ShSolveArgs {
     int lcap;
     const double2 *om;     // [slot][2]: comega2, coef(spn)
     const ShSrcConst *src;
+    const int *jstart;     // mixed forms, depth cut: first row solved per block of the launch (null: row 1)
+};
+// Depth cut of the SH systems (mixed forms): per block of 64 degrees the
+// first row whose decay below the source, integrated from the source down,
+// stays under dcut (sh_depth_kernel).
+struct ShDepthArgs {
+    const double *rrow, *vrow;  // [nn] node radius, smaller shear speed
+    const double *omega;        // [slot] real angular frequency
+    const int *slot_list;
+    int nslots, nlb, l0, k;     // k: first source node (1-based)
+    double dcut, dcap;
+    int *jstart;                // [active slot][nlb]
 };
 constexpr int SH_NSOL = 11;  // 10 sources + surface force
 
@@ -181,13 +193,16 @@ __host__ __device__ inline void sh_solve_lane(const ShSolveArgs &a, int b, int t
     sh_excitation(l, a.om[2 * sl], a.om[2 * sl + 1], *a.src, ex, gam);
 
     // pass 1: factor (dclib.f90:36-61) + forward substitution of the sources
-    // (dclib.f90:93-97), whose right-hand sides sit at rows k, k+1
+    // (dclib.f90:93-97), whose right-hand sides sit at rows k, k+1; with the
+    // depth cut the band starts at row jlo (rows below carry a decay of more
+    // than e^-dcut from the source)
+    const int jlo = a.jstart ? a.jstart[b] : 1;
     double2 drp = cz();
     double2 z[10];
-    for (int j = 1; j <= nn; j++) {
+    for (int j = jlo; j <= nn; j++) {
         const double2 dg = caxpy(a0d[j - 1], l2, a2d[j - 1]);
         double2 Lj, drj;
-        if (j == 1) {
+        if (j == jlo) {
             drj = crecip(dg);
             Lj = cz();
         } else {
@@ -212,7 +227,7 @@ __host__ __device__ inline void sh_solve_lane(const ShSolveArgs &a, int b, int t
     for (int s = 0; s < 10; s++) x[s] = Zs[(size_t)(10 * (nn - k) + s) * ss + gid];
     double2 xs = cmul(gam, drp);  // surface force: z(nn) = gam * dr(nn)
     int p = 0;
-    for (int j = nn; j >= 1; j--) {
+    for (int j = nn; j >= jlo; j--) {
         if (j < nn) {
             const double2 Ln = Ls[(size_t)j * ss + gid];  // L_{j+1}
             for (int s = 0; s < 10; s++) {
@@ -227,6 +242,8 @@ __host__ __device__ inline void sh_solve_lane(const ShSolveArgs &a, int b, int t
             if (++p == a.nneed) break;  // (need is descending: no row below is recorded)
         }
     }
+    for (; p < a.nneed; p++)  // stencil rows below the cut
+        for (int q = 0; q < SH_NSOL; q++) xo[(size_t)(SH_NSOL * p + q) * a.lcap] = cz();
 }
 
 // ---------------------------------------------------------------------------

@@ -191,3 +191,32 @@ def test_l_bound_deep_source(factor):
     if factor == "0.05":
         assert retries > 0
     print(f"l bound x{factor}: cut-offs {lc}, {retries} batch(es) sent back for the full range")
+
+
+@pytest.mark.parametrize("depth_km", [0.0, 600.0])
+def test_sh_depth_cut(depth_km):
+    """SH mixed form with the depth cut (default) against the same form
+    with it off (DSM_SGT_DEPTH_CUT=0): the rows dropped below the source
+    carry less than e^-40 of the solution, so the catalogs agree to
+    round-off and the l cut-offs are the same."""
+    import os
+    import dsm_sgt
+    from parity import component_errors
+    g = dict(GRID, r0=6371.0 - depth_km, imax=24, rmin_=4421.0, rdelta_=195.0)
+    inp = dsm_sgt.DsmSgtInput.from_buffer_copy(di.synthetic_input(**g))
+    freqs = (1, 12, 24)
+    os.environ["DSM_SGT_DEPTH_CUT"] = "0"
+    try:
+        with dsm_sgt.Engine(inp, mode=dsm_sgt.SH, precision=dsm_sgt.BF16X6) as full:
+            full.run(1, g["imax"])
+            want = _catalog(full, freqs)
+            lc = [full.lcut(i) for i in freqs]
+    finally:
+        os.environ.pop("DSM_SGT_DEPTH_CUT", None)
+    with dsm_sgt.Engine(inp, mode=dsm_sgt.SH, precision=dsm_sgt.BF16X6) as cut:
+        cut.run(1, g["imax"])
+        got = _catalog(cut, freqs)
+        assert [cut.lcut(i) for i in freqs] == lc
+    worst = max(float(component_errors(x, y).max()) for i in freqs for x, y in zip(got[i], want[i]))
+    assert worst <= 1e-10, worst
+    print(f"SH depth cut, source {depth_km} km: worst component difference {worst:.1e}")
