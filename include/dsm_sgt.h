@@ -117,8 +117,9 @@ typedef struct dsm_sgt_tuning {
                            the batched P-SV forms always take the whole l range */
     int32_t xcd_group;  /* BF16 synthesis: tile groups dealt to one XCD at a time; -1: no tile list
                            (the kernel's own XCD-sliced order) */
-    int32_t graph;      /* 1: each P-SV batch's launches after the factor-storage read-back are
-                           captured into one HIP graph and launched as one; 0: eager launches */
+    int32_t graph;      /* 1 (batched P-SV forms): each batch's launches after the factor-storage
+                           read-back run as two HIP graphs, the solve sequence and the synthesis
+                           pair; 0: eager launches */
     int32_t catalog_window; /* > 0: the device catalog holds this many frequencies, the frequencies of
                                the last dsm_sgt_run only (at most this many), so that catalogs larger
                                than device memory are produced run by run; 0: all of [imin, imax] */
