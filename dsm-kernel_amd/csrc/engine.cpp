@@ -183,9 +183,12 @@ struct dsm_sgt {
     dsm_sgt_tuning_t tune{};
     int xg = 0, graph = 0;
     int ipos0 = 0, npos = 0;  // the run covers frequencies [ipos0, ipos0+npos)
-    // device catalog: ncat frequencies from cat0 (ncat = npos and cat0 =
-    // ipos0 unless a catalog window is set)
-    int ncat = 0, cat0 = 0;
+    // device catalog: ncat frequency positions; catpos[i - ipos0] is the
+    // position of frequency i (i - ipos0 unless a catalog window is set:
+    // then the frequencies of the last run, in ascending order; -1 when
+    // not resident)
+    int ncat = 0;
+    std::vector<int> catpos;
     std::vector<int> need, rec_slot;
     std::vector<int> llog;    // per position: the reference's logged llog-1
     std::vector<char> done;   // per position: catalog resident on the device
@@ -223,6 +226,19 @@ struct dsm_sgt {
     int njobs_main = 0, njobs_syn = 0, njobs_acc = 0;  // launch order: main, accumulate, SYNN
     PsvPrep prep[2];
     ShPrep shprep[2];
+    // catalog output: the records of a run packed on the device into
+    // complex64 blocks (d_pack, one per frequency of packlist), copied to
+    // the host on stream_io and written by the writer thread, beside the
+    // engine's next run
+    hipStream_t stream_io = nullptr;
+    hipEvent_t ev_pack = nullptr;
+    DevBuf d_pack, d_packpos;
+    std::vector<int> packlist, packpos;
+    std::thread writer;
+    int io_rc = 0;
+    std::string io_err;
+    float2 *pin = nullptr;  // pinned staging of the writer (NPIN blocks)
+    size_t pin_n = 0;
     // timing
     bool timing = true;
     std::vector<TimedLaunch> pending;
@@ -562,6 +578,14 @@ static void psv_synth_jobs(dsm_sgt_t *h)
     h->njobs_syn = counts[2];
 }
 
+// Catalog positions without a window: frequency i at i - ipos0.
+static void init_catpos(dsm_sgt_t *h)
+{
+    h->catpos.assign(h->npos, -1);
+    if (h->ncat >= h->npos)
+        for (int p = 0; p < h->npos; p++) h->catpos[p] = p;
+}
+
 static int sh_create(dsm_sgt_t *h)
 {
     int rc = sh_setup(h->in, h->sh, h->err);
@@ -602,7 +626,7 @@ static int sh_create(dsm_sgt_t *h)
     h->done.assign(h->npos, 0);
     h->logged.assign(h->npos, 0);
     h->ncat = h->tune.catalog_window > 0 ? std::min(h->tune.catalog_window, std::max(1, h->npos)) : h->npos;
-    h->cat0 = h->ipos0;
+    init_catpos(h);
 
     std::vector<double> th(s.theta_n);
     for (int i = 0; i < s.theta_n; i++) th[i] = s.theta_deg[i] / 180.0 * PI;
@@ -767,7 +791,7 @@ static int sh_run_batch(dsm_sgt_t *h, const ShPrep &p)
     for (int k = 0; k < S; k++) {
         cut[k].lsuf = p.lsuf[k];
         cut[k].llog = s.maxlmax;
-        slotout[k] = freqs[k] - h->cat0;  // catalog position
+        slotout[k] = h->catpos[freqs[k] - h->ipos0];  // catalog position
     }
     HIPCHK(h, hipMemcpyAsync(h->d_band.p, band.data(), sizeof(double2) * band.size(), hipMemcpyHostToDevice, h->stream));
     HIPCHK(h, hipMemcpyAsync(h->d_om.p, om.data(), sizeof(double2) * om.size(), hipMemcpyHostToDevice, h->stream));
@@ -980,7 +1004,7 @@ static int psv_create(dsm_sgt_t *h)
     h->done.assign(h->npos, 0);
     h->logged.assign(h->npos, 0);
     h->ncat = h->tune.catalog_window > 0 ? std::min(h->tune.catalog_window, std::max(1, h->npos)) : h->npos;
-    h->cat0 = h->ipos0;
+    init_catpos(h);
 
     std::vector<double> th(s.theta_n);
     for (int i = 0; i < s.theta_n; i++) th[i] = s.theta_deg[i] / 180.0 * PI;
@@ -1050,15 +1074,17 @@ static int psv_create(dsm_sgt_t *h)
     const int lch = std::min(h->lchunk, s.maxlmax + 1);
     const size_t nthreads = (size_t)S * (psv_solve_blocks(0, lch - 1) + 1) * 256;
     HIPCHK(h, h->d_zs.ensure(sizeof(double2) * (size_t)4 * (n - fs + 1) * nthreads));
-    if (h->precision == DSM_SGT_FP64)  // full systems; the batched forms size it per batch
+    // batched forms: depth cut (DSM_SGT_DEPTH_CUT overrides; 0 disables)
+    h->dcut = (h->precision == DSM_SGT_FP64) ? 0.0 : 40.0;
+    if (const char *e = std::getenv("DSM_SGT_DEPTH_CUT")) h->dcut = std::max(0.0, std::atof(e));
+    // factor storage of full systems: the FP64 form, and every form without
+    // the depth cut (the cut systems of the batched forms are sized per batch)
+    if (h->precision == DSM_SGT_FP64 || h->dcut <= 0.0)
         HIPCHK(h, h->d_scratch.ensure(sizeof(double2) * (size_t)3 * (n - klo + 1) * nthreads));
     HIPCHK(h, h->d_loff.ensure(sizeof(long long) * (size_t)S * (psv_solve_blocks(0, lch - 1) + 1) + 16));
     HIPCHK(h, h->d_gexc.ensure(sizeof(double2) * (size_t)S * 16 * LC));
     HIPCHK(h, h->d_ends.ensure(sizeof(double2) * 3 * nthreads));
     HIPCHK(h, h->d_wsave.ensure(sizeof(double2) * 6 * nthreads));
-    // batched forms: depth cut (DSM_SGT_DEPTH_CUT overrides; 0 disables)
-    h->dcut = (h->precision == DSM_SGT_FP64) ? 0.0 : 40.0;
-    if (const char *e = std::getenv("DSM_SGT_DEPTH_CUT")) h->dcut = std::max(0.0, std::atof(e));
     {
         std::vector<double> rr, vv;
         psv_row_profile(s, rr, vv);
@@ -1286,7 +1312,7 @@ static int psv_run_batch(dsm_sgt_t *h, const PsvPrep &p, bool full_range)
     for (int k = 0; k < S; k++) {
         cut[k].lsuf = p.lsuf[k];
         cut[k].llog = s.maxlmax;
-        slotout[k] = freqs[k] - h->cat0;  // catalog position
+        slotout[k] = h->catpos[freqs[k] - h->ipos0];  // catalog position
     }
     {
         const int rc = psv_device_band(h, p.scal, S, 0);
@@ -1481,6 +1507,10 @@ static int psv_run_batch(dsm_sgt_t *h, const PsvPrep &p, bool full_range)
                 launch_psv_need(na, h->stream);
             }
         }
+        // every solve path stores factors (or checkpoints) in the scratch
+        // buffer: refuse the launch rather than let it address null
+        if (!sa.scratch || (sa.ckpt && !sa.loff))
+            return fail(h, DSM_SGT_ESTATE, "band solve launched without factor storage");
         launch_psv_excite(sa, (int)active.size(), h->stream);
         launch_psv_solve(sa, (int)active.size(), h->precision != DSM_SGT_FP64, h->stream);
         HIPCHK(h, hipGetLastError());
@@ -1764,6 +1794,8 @@ int dsm_sgt_create_tuned(const dsm_sgt_input_t *in, int32_t mode, int32_t precis
         }
         if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess ||
             hipStreamCreateWithFlags(&h->stream2, hipStreamNonBlocking) != hipSuccess ||
+            hipStreamCreateWithFlags(&h->stream_io, hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&h->ev_pack, hipEventDisableTiming) != hipSuccess ||
             hipEventCreateWithFlags(&h->ev_l0in, hipEventDisableTiming) != hipSuccess ||
             hipEventCreateWithFlags(&h->ev_l0out, hipEventDisableTiming) != hipSuccess) {
             rc = fail(h, DSM_SGT_EHIP, "hipStreamCreate failed");
@@ -1947,29 +1979,25 @@ int dsm_sgt_get_grid(dsm_sgt_t *h, double *radii, double *theta_deg)
     return DSM_SGT_OK;
 }
 
-int dsm_sgt_run(dsm_sgt_t *h, int32_t i_begin, int32_t i_end)
+// Run the frequencies `all` (ascending, distinct, no 0) in batches of
+// consecutive entries.
+static int run_frequencies(dsm_sgt_t *h, const std::vector<int> &all)
 {
-    if (!h) return DSM_SGT_EINVAL;
-    if (!h->ready) return fail(h, DSM_SGT_ESTATE, "engine not initialised");
-    if (i_begin < h->ipos0 || i_end >= h->ipos0 + h->npos || i_begin > i_end)
-        return fail(h, DSM_SGT_EINVAL, "frequency range outside [imin, imax] of the input");
     if (hipSetDevice(h->device) != hipSuccess) return fail(h, DSM_SGT_EHIP, "hipSetDevice failed");
     // (the l bound keeps the last batch's cut-off across runs: a run of the
     // next catalog window continues from it; a wrong guess only costs a
     // full-range re-run of the batch)
     if (h->ncat < h->npos) {
         // catalog window: this run's frequencies replace the resident ones
-        const int first = std::max(i_begin, 1);
-        if (first <= i_end && i_end - first + 1 > h->ncat)
+        if ((int)all.size() > h->ncat)
             return fail(h, DSM_SGT_EINVAL, "more frequencies than the catalog window holds");
         std::fill(h->done.begin(), h->done.end(), 0);
-        h->cat0 = first;
+        std::fill(h->catpos.begin(), h->catpos.end(), -1);
+        for (size_t k = 0; k < all.size(); k++) h->catpos[all[k] - h->ipos0] = (int)k;
     }
-    // zig-zag order of the reference's rank distribution is irrelevant on one
-    // device; batch consecutive frequencies (similar l cut-off)
-    std::vector<int> all;
-    for (int i = i_begin; i <= i_end; i++)
-        if (i != 0) all.push_back(i);
+    // the reference's rank distribution (SGTpsv-MPI.f90:550) balances
+    // ranks; on one device, batches of consecutive frequencies of the set
+    // share similar l cut-offs
     std::vector<std::vector<int>> batches;
     {
         // a short first batch when batches are large: its host set-up is the
@@ -2020,12 +2048,41 @@ int dsm_sgt_run(dsm_sgt_t *h, int32_t i_begin, int32_t i_end)
     return DSM_SGT_OK;
 }
 
+int dsm_sgt_run(dsm_sgt_t *h, int32_t i_begin, int32_t i_end)
+{
+    if (!h) return DSM_SGT_EINVAL;
+    if (!h->ready) return fail(h, DSM_SGT_ESTATE, "engine not initialised");
+    if (i_begin < h->ipos0 || i_end >= h->ipos0 + h->npos || i_begin > i_end)
+        return fail(h, DSM_SGT_EINVAL, "frequency range outside [imin, imax] of the input");
+    std::vector<int> all;
+    for (int i = i_begin; i <= i_end; i++)
+        if (i != 0) all.push_back(i);
+    return run_frequencies(h, all);
+}
+
+int dsm_sgt_run_list(dsm_sgt_t *h, const int32_t *idx, int32_t n)
+{
+    if (!h) return DSM_SGT_EINVAL;
+    if (!h->ready) return fail(h, DSM_SGT_ESTATE, "engine not initialised");
+    if (n < 0 || (n > 0 && !idx)) return fail(h, DSM_SGT_EINVAL, "bad frequency list");
+    std::vector<int> all;
+    for (int k = 0; k < n; k++) {
+        if (idx[k] < h->ipos0 || idx[k] >= h->ipos0 + h->npos)
+            return fail(h, DSM_SGT_EINVAL, "frequency " + std::to_string(idx[k]) + " outside [imin, imax] of the input");
+        if (idx[k] != 0) all.push_back(idx[k]);
+    }
+    std::sort(all.begin(), all.end());
+    all.erase(std::unique(all.begin(), all.end()), all.end());
+    return run_frequencies(h, all);
+}
+
 static int fetch_common(dsm_sgt_t *h, int32_t i, int32_t kind, int32_t ir, std::vector<double2> &buf, int &num)
 {
     if (!h) return DSM_SGT_EINVAL;
     if (i - h->ipos0 < 0 || i - h->ipos0 >= h->npos || !h->done[i - h->ipos0])
         return fail(h, DSM_SGT_ESTATE, "frequency " + std::to_string(i) + " not computed (or not in the catalog window)");
-    const int pos = i - h->cat0;  // device catalog position
+    const int pos = h->catpos[i - h->ipos0];  // device catalog position
+    if (pos < 0 || pos >= h->ncat) return fail(h, DSM_SGT_ESTATE, "frequency " + std::to_string(i) + " not resident");
     const int R = h->r_n, T = h->theta_n;
     // device catalogs: [pos][comp][r][theta] (r only for TSGT, RSGT)
     const double2 *src;
@@ -2106,6 +2163,85 @@ int dsm_sgt_lcut(dsm_sgt_t *h, int32_t i, int32_t *llog_minus_1)
     return DSM_SGT_OK;
 }
 
+// complex64 values of one frequency's records: TSGT, RSGT, SYNN blocks
+static size_t record_block(const dsm_sgt_t *h)
+{
+    return (size_t)h->theta_n * ((size_t)h->r_n * (h->ncomp[0] + h->ncomp[1]) + h->ncomp[2]);
+}
+
+// The computed frequencies of [i_begin, i_end] (i = 0 skipped); fails if
+// one of them is not resident.
+static int resident_list(dsm_sgt_t *h, int32_t i_begin, int32_t i_end, std::vector<int> &freqs)
+{
+    freqs.clear();
+    if (i_begin > i_end || i_begin < h->ipos0 || i_end >= h->ipos0 + h->npos)
+        return fail(h, DSM_SGT_EINVAL, "frequency range outside [imin, imax] of the input");
+    for (int i = i_begin; i <= i_end; i++) {
+        if (i == 0) continue;
+        const int p = i - h->ipos0;
+        if (!h->done[p] || h->catpos[p] < 0)
+            return fail(h, DSM_SGT_ESTATE, "frequency " + std::to_string(i) + " not computed (or not in the catalog window)");
+        freqs.push_back(i);
+    }
+    return DSM_SGT_OK;
+}
+
+// Join the writer thread and the copies of the last asynchronous output;
+// returns their status.
+static int io_join(dsm_sgt_t *h)
+{
+    if (h->writer.joinable()) h->writer.join();
+    if (h->stream_io) {
+        const hipError_t e = hipStreamSynchronize(h->stream_io);
+        if (e != hipSuccess && !h->io_rc) {
+            h->io_rc = DSM_SGT_EHIP;
+            h->io_err = std::string("catalog copy: ") + hipGetErrorString(e);
+        }
+    }
+    const int rc = h->io_rc;
+    if (rc) h->err = h->io_err;
+    h->io_rc = 0;
+    h->io_err.clear();
+    return rc;
+}
+
+// Records of `freqs` (resident) cast and transposed into d_pack on the
+// engine stream, block k for freqs[k]; ev_pack marks their completion.
+// The caller has joined the previous output (d_pack is free).
+static int pack_catalog(dsm_sgt_t *h, const std::vector<int> &freqs)
+{
+    const size_t blk = record_block(h), n = freqs.size();
+    if (!n) return DSM_SGT_OK;
+    HIPCHK(h, h->d_pack.ensure(sizeof(float2) * blk * n));
+    HIPCHK(h, h->d_packpos.ensure(sizeof(int) * n));
+    h->packlist = freqs;
+    h->packpos.resize(n);
+    for (size_t k = 0; k < n; k++) h->packpos[k] = h->catpos[freqs[k] - h->ipos0];
+    HIPCHK(h, hipMemcpyAsync(h->d_packpos.p, h->packpos.data(), sizeof(int) * n, hipMemcpyHostToDevice, h->stream));
+    const int R = h->r_n, T = h->theta_n;
+    const DevBuf *src[3] = {&h->d_cat_t, &h->d_cat_r, &h->d_cat_s};
+    size_t off = 0;
+    TimedLaunch t;
+    ev_begin(h, t, 3);
+    for (int kind = 0; kind < 3; kind++) {
+        PackArgs a{};
+        a.src = src[kind]->as<double2>();
+        a.ncomp = h->ncomp[kind];
+        a.nrec = kind == DSM_SGT_SYNN ? 1 : R;
+        a.theta_n = T;
+        a.pos_list = h->d_packpos.as<int>();
+        a.dst = h->d_pack.as<float2>();
+        a.dst_stride = blk;
+        a.dst_off = off;
+        launch_pack_records(a, (int)n, h->stream);
+        off += (size_t)a.nrec * T * a.ncomp;
+    }
+    ev_end(h, t);
+    HIPCHK(h, hipGetLastError());
+    HIPCHK(h, hipEventRecord(h->ev_pack, h->stream));
+    return DSM_SGT_OK;
+}
+
 // File names of SGTsh-MPI.f90:570-606 / SGTpsv-MPI.f90:1106-1144:
 // Fortran I7 fields joined by '.', blanks replaced by '0'.
 static std::string i7(long v)
@@ -2119,47 +2255,187 @@ static std::string i7(long v)
     return s;
 }
 
-static int write_record(dsm_sgt_t *h, const std::string &path, int32_t i, int32_t kind, int32_t ir)
+static bool write_file(const std::string &path, const float2 *data, size_t n, std::string &err)
 {
-    const int T = h->theta_n;
-    const int num = h->ncomp[kind];
-    std::vector<float> rec((size_t)2 * num * T);
-    int rc = dsm_sgt_fetch(h, i, kind, ir, rec.data());
-    if (rc) return rc;
     FILE *f = std::fopen(path.c_str(), "wb");
-    if (!f) return fail(h, DSM_SGT_EIO, "cannot open " + path);
-    size_t w = std::fwrite(rec.data(), sizeof(float), rec.size(), f);
-    int c = std::fclose(f);
-    if (w != rec.size() || c != 0) return fail(h, DSM_SGT_EIO, "short write " + path);
+    if (!f) {
+        err = "cannot open " + path;
+        return false;
+    }
+    const size_t w = std::fwrite(data, sizeof(float2), n, f);
+    const int c = std::fclose(f);
+    if (w != n || c != 0) {
+        err = "short write " + path;
+        return false;
+    }
+    return true;
+}
+
+// The files of frequency i from its packed block `blk` (record_block
+// values): TSGT of every receiver, RSGT if rsgtswitch, SYNN if synnswitch
+// (tsgtswitch is not read, as in SGTpsv-MPI.f90:1103-1150).
+static bool write_block_files(const dsm_sgt_t *h, const std::string &od, const std::string &mn, int i,
+                              const float2 *blk, std::string &err)
+{
+    const char *suf = h->mode == DSM_SGT_SH ? "SH" : "PSV";
+    const int R = h->r_n, T = h->theta_n;
+    const size_t nt = (size_t)T * h->ncomp[0], nr = (size_t)T * h->ncomp[1], ns = (size_t)T * h->ncomp[2];
+    for (int ir = 0; ir < R; ir++) {
+        const long rr = (long)(h->radii[ir] * 1.0e3);
+        std::string name = od + "/TSGT/" + mn + "." + i7(h->intir0) + "." + i7(rr) + "." + i7(i) + ".TSGT_" + suf;
+        if (!write_file(name, blk + ir * nt, nt, err)) return false;
+        if (h->in.rsgtswitch) {
+            name = od + "/RSGT/" + mn + "." + i7(rr) + "." + i7(i) + ".RSGT_" + suf;
+            if (!write_file(name, blk + R * nt + ir * nr, nr, err)) return false;
+        }
+    }
+    if (h->in.synnswitch) {
+        const std::string name = od + "/RSGT/" + mn + "." + i7(h->intir0) + "." + i7(i) + ".SYNN_" + suf;
+        if (!write_file(name, blk + R * (nt + nr), ns, err)) return false;
+    }
+    return true;
+}
+
+static void make_dirs(const std::string &od)
+{
+    ::mkdir(od.c_str(), 0755);
+    ::mkdir((od + "/TSGT").c_str(), 0755);
+    ::mkdir((od + "/RSGT").c_str(), 0755);
+    ::mkdir((od + "/log").c_str(), 0755);
+}
+
+constexpr int NPIN = 3;  // pinned staging blocks of the writer
+
+// Writer thread: copies the packed blocks to pinned staging on stream_io
+// (NPIN - 1 ahead) and writes each frequency's files while the next
+// blocks copy.  Touches only d_pack, the staging, stream_io and the
+// handle's read-only geometry; its status goes to io_rc / io_err.
+static void writer_body(dsm_sgt_t *h, std::string od, std::string mn)
+{
+    (void)hipSetDevice(h->device);
+    const size_t blk = record_block(h);
+    const int n = (int)h->packlist.size();
+    hipEvent_t ev[NPIN] = {};
+    for (auto &e : ev) (void)hipEventCreateWithFlags(&e, hipEventDisableTiming);
+    std::string err;
+    int rc = DSM_SGT_OK;
+    auto issue = [&](int k) {
+        const hipError_t e = hipMemcpyAsync(h->pin + (size_t)(k % NPIN) * blk, h->d_pack.as<float2>() + (size_t)k * blk,
+                                            sizeof(float2) * blk, hipMemcpyDeviceToHost, h->stream_io);
+        if (e == hipSuccess) (void)hipEventRecord(ev[k % NPIN], h->stream_io);
+        return e;
+    };
+    for (int k = 0; k < std::min(NPIN - 1, n) && !rc; k++)
+        if (issue(k) != hipSuccess) rc = DSM_SGT_EHIP, err = "catalog copy failed";
+    for (int k = 0; k < n && !rc; k++) {
+        if (k + NPIN - 1 < n && issue(k + NPIN - 1) != hipSuccess) {
+            rc = DSM_SGT_EHIP, err = "catalog copy failed";
+            break;
+        }
+        if (hipEventSynchronize(ev[k % NPIN]) != hipSuccess) {
+            rc = DSM_SGT_EHIP, err = "catalog copy failed";
+            break;
+        }
+        if (!write_block_files(h, od, mn, h->packlist[k], h->pin + (size_t)(k % NPIN) * blk, err)) rc = DSM_SGT_EIO;
+    }
+    (void)hipStreamSynchronize(h->stream_io);
+    for (auto &e : ev) (void)hipEventDestroy(e);
+    h->io_rc = rc;
+    h->io_err = err;
+}
+
+int dsm_sgt_write_catalogs_async(dsm_sgt_t *h, int32_t i_begin, int32_t i_end, const char *output_dir,
+                                 const char *model_name)
+{
+    if (!h || !output_dir || !model_name) return DSM_SGT_EINVAL;
+    if (!h->ready) return fail(h, DSM_SGT_ESTATE, "engine not initialised");
+    int rc = io_join(h);
+    if (rc) return rc;
+    std::vector<int> freqs;
+    rc = resident_list(h, i_begin, i_end, freqs);
+    if (rc || freqs.empty()) return rc;
+    if (hipSetDevice(h->device) != hipSuccess) return fail(h, DSM_SGT_EHIP, "hipSetDevice failed");
+    const size_t blk = record_block(h);
+    if (h->pin_n < NPIN * blk) {
+        if (h->pin) (void)hipHostFree(h->pin);
+        h->pin = nullptr;
+        h->pin_n = 0;
+        HIPCHK(h, hipHostMalloc((void **)&h->pin, sizeof(float2) * NPIN * blk, hipHostMallocDefault));
+        h->pin_n = NPIN * blk;
+    }
+    make_dirs(output_dir);
+    rc = pack_catalog(h, freqs);
+    if (rc) return rc;
+    HIPCHK(h, hipStreamWaitEvent(h->stream_io, h->ev_pack, 0));
+    try {
+        h->writer = std::thread(writer_body, h, std::string(output_dir), std::string(model_name));
+    } catch (...) {
+        return fail(h, DSM_SGT_ENOMEM, "cannot start the catalog writer thread");
+    }
     return DSM_SGT_OK;
+}
+
+int dsm_sgt_copy_catalog_async(dsm_sgt_t *h, int32_t i_begin, int32_t i_end, void *dst, size_t dst_bytes)
+{
+    if (!h || !dst) return DSM_SGT_EINVAL;
+    if (!h->ready) return fail(h, DSM_SGT_ESTATE, "engine not initialised");
+    int rc = io_join(h);
+    if (rc) return rc;
+    std::vector<int> freqs;
+    rc = resident_list(h, i_begin, i_end, freqs);
+    if (rc || freqs.empty()) return rc;
+    const size_t bytes = sizeof(float2) * record_block(h) * freqs.size();
+    if (dst_bytes < bytes)
+        return fail(h, DSM_SGT_EINVAL, "destination holds " + std::to_string(dst_bytes) + " bytes, the records need " +
+                                           std::to_string(bytes));
+    if (hipSetDevice(h->device) != hipSuccess) return fail(h, DSM_SGT_EHIP, "hipSetDevice failed");
+    rc = pack_catalog(h, freqs);
+    if (rc) return rc;
+    HIPCHK(h, hipStreamWaitEvent(h->stream_io, h->ev_pack, 0));
+    HIPCHK(h, hipMemcpyAsync(dst, h->d_pack.p, bytes, hipMemcpyDeviceToHost, h->stream_io));
+    return DSM_SGT_OK;
+}
+
+int dsm_sgt_wait(dsm_sgt_t *h)
+{
+    if (!h) return DSM_SGT_EINVAL;
+    return io_join(h);
+}
+
+int dsm_sgt_host_alloc(size_t bytes, void **ptr)
+{
+    if (!ptr) return DSM_SGT_EINVAL;
+    *ptr = nullptr;
+    if (hipHostMalloc(ptr, bytes ? bytes : 1, hipHostMallocDefault) != hipSuccess) {
+        *ptr = nullptr;
+        return DSM_SGT_ENOMEM;
+    }
+    return DSM_SGT_OK;
+}
+
+void dsm_sgt_host_free(void *ptr)
+{
+    if (ptr) (void)hipHostFree(ptr);
 }
 
 int dsm_sgt_write_catalog(dsm_sgt_t *h, int32_t i, const char *output_dir, const char *model_name)
 {
     if (!h || !output_dir || !model_name) return DSM_SGT_EINVAL;
-    const std::string od(output_dir), mn(model_name);
-    const char *suf = h->mode == DSM_SGT_SH ? "SH" : "PSV";
-    ::mkdir(od.c_str(), 0755);
-    ::mkdir((od + "/TSGT").c_str(), 0755);
-    ::mkdir((od + "/RSGT").c_str(), 0755);
-    ::mkdir((od + "/log").c_str(), 0755);
-    const int R = h->r_n;
-    for (int ir = 0; ir < R; ir++) {
-        const long rr = (long)(h->radii[ir] * 1.0e3);
-        std::string name = od + "/TSGT/" + mn + "." + i7(h->intir0) + "." + i7(rr) + "." + i7(i) + ".TSGT_" + suf;
-        int rc = write_record(h, name, i, DSM_SGT_TSGT, ir);
-        if (rc) return rc;
-        if (h->in.rsgtswitch) {
-            name = od + "/RSGT/" + mn + "." + i7(rr) + "." + i7(i) + ".RSGT_" + suf;
-            rc = write_record(h, name, i, DSM_SGT_RSGT, ir);
-            if (rc) return rc;
-        }
-    }
-    if (h->in.synnswitch) {
-        std::string name = od + "/RSGT/" + mn + "." + i7(h->intir0) + "." + i7(i) + ".SYNN_" + suf;
-        int rc = write_record(h, name, i, DSM_SGT_SYNN, 0);
-        if (rc) return rc;
-    }
+    if (!h->ready) return fail(h, DSM_SGT_ESTATE, "engine not initialised");
+    int rc = io_join(h);
+    if (rc) return rc;
+    std::vector<int> freqs;
+    rc = resident_list(h, i, i, freqs);
+    if (rc || freqs.empty()) return rc;
+    if (hipSetDevice(h->device) != hipSuccess) return fail(h, DSM_SGT_EHIP, "hipSetDevice failed");
+    rc = pack_catalog(h, freqs);
+    if (rc) return rc;
+    std::vector<float2> blk(record_block(h));
+    HIPCHK(h, hipMemcpyAsync(blk.data(), h->d_pack.p, sizeof(float2) * blk.size(), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    make_dirs(output_dir);
+    std::string err;
+    if (!write_block_files(h, output_dir, model_name, i, blk.data(), err)) return fail(h, DSM_SGT_EIO, err);
     return DSM_SGT_OK;
 }
 
@@ -2190,6 +2466,10 @@ const char *dsm_sgt_last_error(dsm_sgt_t *h) { return h ? h->err.c_str() : "null
 void dsm_sgt_destroy(dsm_sgt_t *h)
 {
     if (!h) return;
+    (void)io_join(h);
+    if (h->stream_io) (void)hipStreamDestroy(h->stream_io);
+    if (h->ev_pack) (void)hipEventDestroy(h->ev_pack);
+    if (h->pin) (void)hipHostFree(h->pin);
     if (h->stream) {
         (void)hipSetDevice(h->device);
         (void)hipStreamSynchronize(h->stream);

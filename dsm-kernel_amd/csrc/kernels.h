@@ -115,4 +115,18 @@ size_t synth_b16_grid(int nslot, int r_eff, int nhp, int njobs);
 void synth_b16_tiles(int nslot, int r_eff, int nhp, int &nx, int &ny);
 void launch_synth_b16(const SynthB16Args &a, int ns, hipStream_t s);
 
+// ---- catalog records (catalog.hip) ------------------------------------------
+// One kind of the component-major catalog src [pos][ncomp][nrec][theta_n]
+// cast to complex64 in the files' record order: for the k-th position of
+// pos_list, dst[k dst_stride + dst_off + (rec theta_n + t) ncomp + c].
+// ncomp <= 20.
+struct PackArgs {
+    const double2 *src;
+    int ncomp, nrec, theta_n;
+    const int *pos_list;
+    float2 *dst;
+    size_t dst_stride, dst_off;  // in complex64 values
+};
+void launch_pack_records(const PackArgs &a, int npos, hipStream_t s);
+
 }  // namespace dsm

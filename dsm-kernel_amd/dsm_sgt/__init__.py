@@ -59,7 +59,9 @@ _lib = None
 EXPORTS = ("dsm_sgt_create", "dsm_sgt_get_info", "dsm_sgt_get_grid", "dsm_sgt_run", "dsm_sgt_fetch",
            "dsm_sgt_fetch_fp64", "dsm_sgt_lcut", "dsm_sgt_write_catalog", "dsm_sgt_get_stats",
            "dsm_sgt_reset_stats", "dsm_sgt_clear", "dsm_sgt_last_error", "dsm_sgt_destroy", "dsm_sgt_psv_band",
-           "dsm_sgt_create_tuned", "dsm_sgt_get_tuning", "dsm_sgt_autotune")
+           "dsm_sgt_create_tuned", "dsm_sgt_get_tuning", "dsm_sgt_autotune", "dsm_sgt_run_list",
+           "dsm_sgt_write_catalogs_async", "dsm_sgt_copy_catalog_async", "dsm_sgt_wait", "dsm_sgt_host_alloc",
+           "dsm_sgt_host_free")
 
 
 def lib():
@@ -79,6 +81,13 @@ def lib():
         L.dsm_sgt_get_info.argtypes = [C.c_void_p, C.POINTER(Info)]
         L.dsm_sgt_get_grid.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         L.dsm_sgt_run.argtypes = [C.c_void_p, C.c_int32, C.c_int32]
+        L.dsm_sgt_run_list.argtypes = [C.c_void_p, C.c_void_p, C.c_int32]
+        L.dsm_sgt_write_catalogs_async.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_char_p, C.c_char_p]
+        L.dsm_sgt_copy_catalog_async.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_void_p, C.c_size_t]
+        L.dsm_sgt_wait.argtypes = [C.c_void_p]
+        L.dsm_sgt_host_alloc.argtypes = [C.c_size_t, C.POINTER(C.c_void_p)]
+        L.dsm_sgt_host_free.argtypes = [C.c_void_p]
+        L.dsm_sgt_host_free.restype = None
         L.dsm_sgt_fetch.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_void_p]
         L.dsm_sgt_fetch_fp64.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_void_p]
         L.dsm_sgt_lcut.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_int32)]
@@ -129,6 +138,31 @@ class Engine:
 
     def run(self, i0, i1):
         self._check(lib().dsm_sgt_run(self._h, i0, i1))
+
+    def run_list(self, freqs):
+        """Compute an arbitrary set of frequencies (dsm_sgt_run_list)."""
+        idx = np.ascontiguousarray(np.asarray(list(freqs), dtype=np.int32))
+        self._check(lib().dsm_sgt_run_list(self._h, idx.ctypes.data, idx.size))
+
+    def record_block(self):
+        """complex64 values of one frequency's records (TSGT, RSGT, SYNN)."""
+        i = self.info
+        return i.theta_n * (i.r_n * (i.num_tsgt + i.num_rsgt) + i.num_synn)
+
+    def write_catalogs_async(self, i0, i1, output_dir, model_name):
+        """Files of every frequency in [i0, i1], written beside the next run;
+        wait() returns when they are on disk."""
+        self._check(lib().dsm_sgt_write_catalogs_async(self._h, i0, i1, output_dir.encode(), model_name.encode()))
+
+    def copy_catalog_async(self, i0, i1, dst):
+        """Records of [i0, i1] into `dst` (a HostBuffer or a writable
+        complex64 numpy array of len(freqs) * record_block() values)."""
+        ptr, nbytes = (dst.ptr, dst.nbytes) if isinstance(dst, HostBuffer) else (dst.ctypes.data, dst.nbytes)
+        self._keep = dst
+        self._check(lib().dsm_sgt_copy_catalog_async(self._h, i0, i1, ptr, nbytes))
+
+    def wait(self):
+        self._check(lib().dsm_sgt_wait(self._h))
 
     def fetch(self, i, kind, ir=0):
         out = np.zeros((self.info.theta_n, self.num(kind)), np.complex64)
@@ -195,6 +229,35 @@ class Engine:
 
     def __exit__(self, *a):
         self.close()
+
+
+class HostBuffer:
+    """Pinned host memory (dsm_sgt_host_alloc); `array` views it as
+    complex64."""
+
+    def __init__(self, nbytes):
+        self.ptr_c = C.c_void_p()
+        rc = lib().dsm_sgt_host_alloc(nbytes, C.byref(self.ptr_c))
+        if rc:
+            raise DsmSgtError(rc, f"cannot allocate {nbytes} bytes of pinned host memory")
+        self.nbytes = nbytes
+        self.ptr = self.ptr_c.value
+
+    @property
+    def array(self):
+        buf = (C.c_char * self.nbytes).from_address(self.ptr)
+        return np.frombuffer(buf, dtype=np.complex64)
+
+    def close(self):
+        if self.ptr_c:
+            lib().dsm_sgt_host_free(self.ptr_c)
+            self.ptr_c = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def autotune(inp, mode=PSV, precision=BF16X6, device=0, cache_path=None):

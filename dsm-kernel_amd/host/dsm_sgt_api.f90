@@ -52,6 +52,22 @@ module dsm_sgt_api
        type(c_ptr), value :: handle
        integer(c_int32_t), value :: i0, i1
      end function dsm_sgt_run
+     integer(c_int32_t) function dsm_sgt_run_list(handle, idx, n) bind(c)
+       import :: c_int32_t, c_ptr
+       type(c_ptr), value :: handle
+       integer(c_int32_t), intent(in) :: idx(*)
+       integer(c_int32_t), value :: n
+     end function dsm_sgt_run_list
+     integer(c_int32_t) function dsm_sgt_write_catalogs_async(handle, i0, i1, outdir, model) bind(c)
+       import :: c_int32_t, c_ptr, c_char
+       type(c_ptr), value :: handle
+       integer(c_int32_t), value :: i0, i1
+       character(kind=c_char), dimension(*), intent(in) :: outdir, model
+     end function dsm_sgt_write_catalogs_async
+     integer(c_int32_t) function dsm_sgt_wait(handle) bind(c)
+       import :: c_int32_t, c_ptr
+       type(c_ptr), value :: handle
+     end function dsm_sgt_wait
      integer(c_int32_t) function dsm_sgt_lcut(handle, i, llog) bind(c)
        import :: c_int32_t, c_ptr
        type(c_ptr), value :: handle

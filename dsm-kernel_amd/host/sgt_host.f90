@@ -141,7 +141,8 @@ contains
     endif
     ! launch plan: autotuned when DSM_SGT_TUNE_CACHE names a cache file;
     ! the device catalog holds DSM_SGT_WINDOW frequencies (default 128),
-    ! written out window by window
+    ! written out window by window: each window's files are written by the
+    ! library's writer thread while the device computes the next window
     call get_environment_variable('DSM_SGT_TUNE_CACHE', envv, status=est)
     if (est == 0 .and. len_trim(envv) > 0) then
        rc = dsm_sgt_autotune(inp, mode, prec, 0_c_int32_t, trim(envv)//c_null_char, plan, probe)
@@ -168,11 +169,15 @@ contains
           rc = dsm_sgt_lcut(h, i, llog)
           if (rc /= 0) call fail(dsm_sgt_message(h))
           write(24, *) i, dble(i) / inp%tlen, llog
-          rc = dsm_sgt_write_catalog(h, i, trim(outdir)//c_null_char, trim(modelname)//c_null_char)
-          if (rc /= 0) call fail(dsm_sgt_message(h))
        enddo
+       if (i1 >= 1) then
+          rc = dsm_sgt_write_catalogs_async(h, i0, i1, trim(outdir)//c_null_char, trim(modelname)//c_null_char)
+          if (rc /= 0) call fail(dsm_sgt_message(h))
+       endif
        i0 = i1 + 1
     enddo
+    rc = dsm_sgt_wait(h)
+    if (rc /= 0) call fail(dsm_sgt_message(h))
     close(24)
     call dsm_sgt_destroy(h)
 

@@ -21,6 +21,7 @@
 #ifndef DSM_SGT_H
 #define DSM_SGT_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -154,6 +155,14 @@ int dsm_sgt_get_grid(dsm_sgt_t *h, double *radii, double *theta_deg);
  * loop, SGTsh-MPI.f90:389-560 / SGTpsv-MPI.f90:535-1095. */
 int dsm_sgt_run(dsm_sgt_t *h, int32_t i_begin, int32_t i_end);
 
+/* Compute the frequencies idx[0..n-1] (any order; duplicates and i = 0
+ * are dropped), batched as a range is: ascending, consecutive entries of
+ * the set in one batch.  For one MPI rank's frequency set of the
+ * reference's zig-zag distribution (SGTpsv-MPI.f90:550, SGTsh-MPI.f90:389)
+ * or any other subset; replaces the same loop body as dsm_sgt_run.  With
+ * a catalog window the set must fit in the window. */
+int dsm_sgt_run_list(dsm_sgt_t *h, const int32_t *idx, int32_t n);
+
 /* Copy one catalog record: kind (DSM_SGT_TSGT/RSGT/SYNN), frequency i and
  * receiver index ir (0-based; ignored for SYNN) into `out`, a Fortran
  * array complex(4) out(num, theta_n), exactly the record the reference
@@ -175,6 +184,33 @@ int dsm_sgt_psv_band(dsm_sgt_t *h, int32_t i, double *band);
  * reference's names and byte layout (raw little-endian complex(4),
  * component fastest, no record markers). */
 int dsm_sgt_write_catalog(dsm_sgt_t *h, int32_t i, const char *output_dir, const char *model_name);
+
+/* Catalog output of a whole run (the write loop SGTpsv-MPI.f90:1103-1150,
+ * SGTsh-MPI.f90:567-612, for every frequency of [i_begin, i_end] except
+ * i = 0; each must be computed and resident).  The records are cast to
+ * complex64 and put in the files' order on the device; the call returns
+ * once that is queued, and a copy stream and a writer thread move the
+ * records to the host and write the files beside the engine's next
+ * dsm_sgt_run.  Files, names and bytes are those of
+ * dsm_sgt_write_catalog.  At most one output is in flight per handle: the
+ * next output call (or dsm_sgt_write_catalog, dsm_sgt_destroy) first waits
+ * for the previous one.  dsm_sgt_wait returns the writer's status. */
+int dsm_sgt_write_catalogs_async(dsm_sgt_t *h, int32_t i_begin, int32_t i_end, const char *output_dir,
+                                 const char *model_name);
+/* The same records copied into host memory `dst` instead of files: per
+ * frequency (ascending, i = 0 skipped) one block of complex(4) values,
+ * TSGT a(num_tsgt, theta_n) of every receiver, then RSGT a(num_rsgt,
+ * theta_n) of every receiver, then SYNN a(num_synn, theta_n); block size
+ * theta_n (r_n (num_tsgt + num_rsgt) + num_synn) complex values.  dst
+ * from dsm_sgt_host_alloc (pinned) copies at full bus speed; it must stay
+ * valid until dsm_sgt_wait.  No counterpart in the reference (it writes
+ * each record to disk). */
+int dsm_sgt_copy_catalog_async(dsm_sgt_t *h, int32_t i_begin, int32_t i_end, void *dst, size_t dst_bytes);
+/* Wait for the handle's asynchronous output; its status. */
+int dsm_sgt_wait(dsm_sgt_t *h);
+/* Pinned host memory for dsm_sgt_copy_catalog_async (hipHostMalloc). */
+int dsm_sgt_host_alloc(size_t bytes, void **ptr);
+void dsm_sgt_host_free(void *ptr);
 
 int dsm_sgt_get_stats(dsm_sgt_t *h, dsm_sgt_stats_t *stats);
 int dsm_sgt_reset_stats(dsm_sgt_t *h);

@@ -7,9 +7,9 @@ The plain FP64 form against the oracle at the lowest frequency (where
 A(omega, l) is worst conditioned; the surface source runs to lmax, about a
 minute of oracle time): bar 1e-9 relative L2 per component and the same
 l cut-off.  The batched mixed
-form (BF16x6) on the same frequencies against the oracle within the
-mixed form's small-grid bound (a receiver at the source depth; see
-test_mixed_gpu.py)."""
+form (BF16x6) on the same frequencies against the oracle at the 1e-6 bar
+(measured 4.6e-7: the receiver at the source depth sums 361 distances
+here, not the few of the small grids)."""
 import pytest
 
 import dsm_inputs as di
@@ -30,7 +30,7 @@ def _oracle(inp, i):
     return _ORACLE[i]
 
 
-@pytest.mark.parametrize("form,tol", [("FP64", 1e-9), ("BF16X6", 8e-6)])
+@pytest.mark.parametrize("form,tol", [("FP64", 1e-9), ("BF16X6", 1e-6)])
 def test_c1_configuration_vs_oracle(form, tol):
     import dsm_sgt
     inp = di.synthetic_input(**C1)
