@@ -89,6 +89,8 @@ def cpu_baseline(mode, p, r_n, lsample=1000):
     run reports the l cut-off of the first and last frequency).  Worker
     processes are started fresh (spawn): they never touch the GPU."""
     import multiprocessing as mp
+    # the GPU box gives one GPU's job a share of 16 host cores, whatever
+    # os.cpu_count() shows there
     cores = max(1, min(16, os.cpu_count() or 1))
     freqs = sorted(set(max(1, int(round(p["imax"] * (k + 0.5) / cores))) for k in range(cores)))
     t0 = time.perf_counter()
@@ -102,11 +104,32 @@ def cpu_baseline(mode, p, r_n, lsample=1000):
         a = max(t1 - b * n1, 0.0)
         per_omega.append((a + b * nfull) * (p["maxlmax"] + 1) / (lsample + 1))
     core_s = sum(per_omega) / len(per_omega) * p["imax"]
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), "")
+    except OSError:
+        pass
+    ratio = REF_ROUTINE_RATIO[mode]
     return dict(value=core_s / len(freqs), unit="s", cores=len(freqs), kind="port", core_seconds=core_s,
+                nproc=os.cpu_count(), cpu_model=model,
+                reference_routines_ratio=ratio, reference_estimate_s=ratio * core_s / len(freqs),
+                reference_note=("per-frequency time of the same driver with the reference's own routines "
+                                "(flang -O2 builds of src/SGT*/ *.f90) over the restated ones (gcc -O3), "
+                                "measured on one core of the build container at l <= 400, 12 distances, "
+                                "frequencies 100 and 500; the restated port is the faster baseline"),
                 sample=f"oracle (gcc -O3, one frequency per core) on frequencies {freqs} of the same grid, "
                        f"{r_n} radii, l=0..{lsample}, 2 and 12 distances ({wall:.1f} s wall); per frequency "
                        f"t = L*(a + b*theta_n) extrapolated to l=0..{p['maxlmax']}, {nfull} distances; mean x "
                        f"{p['imax']} frequencies / {len(freqs)} cores")
+
+
+# Time of one frequency with the reference's own numerical routines in the
+# oracle's driver (oracle/_ref, flang -O2) over the restated C routines
+# (gcc -O3, the timed baseline), measured in the build container on the
+# 5 s grid at l <= 400 and 12 distances, frequencies 100 and 500:
+# P-SV 2.38 and 2.54, SH 2.89 and 2.76.
+REF_ROUTINE_RATIO = {"psv": 2.46, "sh": 2.82}
 
 
 # Multi-depth 2.5 s workload: P-SV and SH catalogs for 16 source depths
@@ -224,6 +247,7 @@ FORMS = {"bf16x6": ("BF16X6", "bf16x6+f64"), "bf16x3": ("BF16X3", "bf16x3+f64"),
          "fp64": ("FP64", "f64")}
 BF16_PEAK = 2.5e15
 FP32_MFMA_PEAK = 157.3e12
+PSV_ROWS, SH_ROWS = 63, 36  # coefficient rows per receiver of the synthesis (psv_jobs.h, sh_jobs.h)
 
 
 def catalog_errors(got_all, ref, freqs):
@@ -257,13 +281,101 @@ SOLVE_STAGE_BYTES = {("psv", "bf16x6"): 1.9994e11, ("psv", "fp64"): 1.6449e12}
 
 
 def band_solve(mode, form, st, full=True):
-    """HBM throughput of the solve stage: its rocprof-measured bytes over
-    the stage's live device time (HIP events on the engine stream)."""
-    out = {"systems": st.systems, "stage_ms": st.solve_ms, "peak": HBM_PEAK / 1e9, "unit": "GB/s"}
+    """HBM roofline of the solve stage: the algorithmic bytes of the
+    reference's systems, 16 (w n + 4 R r_n) per (omega, l >= 1) system below
+    the cut-off with n the rows solved after the depth cut (w = 4, R = 18
+    for P-SV; w = 2, R = 14 for SH; dsm_sgt_stats_t.solve_alg_bytes), over
+    the stage's live device time (HIP events on the engine stream).  The
+    rocprof-measured HBM bytes of the stage (FETCH_SIZE + WRITE_SIZE) are
+    given beside it where a profile of the same form and workload exists."""
+    out = {"systems": st.systems, "stage_ms": st.solve_ms, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+           "alg_bytes": st.solve_alg_bytes, "design_bytes": st.solve_bytes}
+    if st.solve_ms > 0:
+        a = st.solve_alg_bytes / (st.solve_ms * 1e-3)
+        out.update(achieved=a / 1e9, frac=a / HBM_PEAK)
     b = SOLVE_STAGE_BYTES.get((mode, form)) if full else None
+    out["rocprof_bytes"] = b
     if b is not None and st.solve_ms > 0:
-        out.update(hbm_bytes=b, achieved=b / (st.solve_ms * 1e-3) / 1e9,
-                   frac=b / (st.solve_ms * 1e-3) / HBM_PEAK)
+        out["rocprof_gbs"] = b / (st.solve_ms * 1e-3) / 1e9
+    return out
+
+
+def profile_traffic(mode, form, plan):
+    """HBM bytes per launch of the dominant kernel for this workload, form
+    and launch plan, from the rocprof summaries under profiles/: a line
+    `traffic per launch: <mode> <form> nslot=<n> xcd_group=<x> graph=<g> = <bytes> B`
+    (FETCH_SIZE + WRITE_SIZE scaled by a plain copy's counters); None when
+    no profile of this plan exists."""
+    import re
+    want = (mode, form, plan.get("nslot"), plan.get("xcd_group"), plan.get("graph"))
+    pat = re.compile(r"traffic per launch: (\w+) (\w+) nslot=(\d+) xcd_group=(-?\d+) graph=(\d+) = ([0-9.eE+]+) B")
+    d = os.path.join(HERE, "profiles")
+    for name in sorted(os.listdir(d)) if os.path.isdir(d) else []:
+        with open(os.path.join(d, name)) as f:
+            for m in pat.finditer(f.read()):
+                if (m.group(1), m.group(2), int(m.group(3)), int(m.group(4)), int(m.group(5))) == want:
+                    return float(m.group(6))
+    return None
+
+
+def output_legs(inp, mode, prec, plan, imax, out_dir=None):
+    """The catalog out of HBM, timed from engine creation as the timed step
+    is: (1) into pinned host memory (dsm_sgt_copy_catalog_async per window
+    of 128 frequencies, each window's copy beside the next window's
+    compute); (2) into the catalog files (dsm_sgt_write_catalogs_async per
+    window, the writer thread beside the next window's compute), in a
+    scratch directory removed afterwards.  The pinned buffer is allocated
+    before the timed region."""
+    import shutil
+    import tempfile
+    import numpy as np
+    import dsm_sgt
+    W = 128
+    t = dsm_sgt.Tuning(**dict(plan.as_dict() if plan is not None else {}, catalog_window=W))
+    out = {"window": W}
+    probe = dsm_sgt.Engine(inp, mode=mode, precision=prec, tuning=t)
+    blk = probe.record_block()
+    probe.close()
+    total = 8 * blk * imax
+    out["catalog_bytes"] = total
+    buf = dsm_sgt.HostBuffer(total)
+    t0 = time.perf_counter()
+    eng = dsm_sgt.Engine(inp, mode=mode, precision=prec, tuning=t)
+    for i0 in range(1, imax + 1, W):
+        i1 = min(imax, i0 + W - 1)
+        eng.run(i0, i1)
+        eng.copy_catalog_async(i0, i1, buf.view(8 * blk * (i0 - 1), 8 * blk * (i1 - i0 + 1)))
+    eng.wait()
+    out["host_memory_s"] = time.perf_counter() - t0
+    # spot check: the last frequency's SYNN record in host memory equals the fetched one
+    a = buf.array[blk * (imax - 1):blk * imax]
+    n = eng.info
+    syn = a[n.theta_n * n.r_n * (n.num_tsgt + n.num_rsgt):].reshape(n.theta_n, n.num_synn)
+    out["host_memory_check"] = bool(np.array_equal(syn, eng.fetch(imax, dsm_sgt.SYNN)))
+    eng.close()
+    buf.close()
+    base = out_dir or tempfile.gettempdir()
+    free = shutil.disk_usage(base).free
+    if free < 1.5 * total:
+        out["files_s"] = None
+        out["files_note"] = f"skipped: {free / 1e9:.1f} GB free under {base}, the catalog is {total / 1e9:.1f} GB"
+        return out
+    d = tempfile.mkdtemp(prefix="dsm_catalog_", dir=base)
+    try:
+        t0 = time.perf_counter()
+        eng = dsm_sgt.Engine(inp, mode=mode, precision=prec, tuning=t)
+        for i0 in range(1, imax + 1, W):
+            i1 = min(imax, i0 + W - 1)
+            eng.run(i0, i1)
+            eng.write_catalogs_async(i0, i1, d, "model4")
+        eng.wait()
+        out["files_s"] = time.perf_counter() - t0
+        nfiles = sum(len(fs) for _, _, fs in os.walk(d))
+        out["files"] = nfiles
+        out["files_dir_fs"] = base
+        eng.close()
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
     return out
 
 
@@ -285,6 +397,9 @@ def main():
     ap.add_argument("--plan", default=None,
                     help='launch plan as JSON (e.g. \'{"nslot": 64, "xcd_group": 2}\') instead of the autotuner')
     ap.add_argument("--c5-depths", type=int, default=16, help="(profiling) first N of the 16 source depths")
+    ap.add_argument("--no-output", action="store_true",
+                    help="skip the catalog output legs (host memory, files)")
+    ap.add_argument("--out-dir", default=None, help="directory for the files leg's scratch catalog")
     ap.add_argument("--workload", choices=("catalog", "c5"), default="catalog",
                     help="catalog: the 5 s P-SV (or --mode sh 10 s SH) catalog; c5: 16 source depths at 2.5 s, "
                          "P-SV + SH")
@@ -367,27 +482,42 @@ def main():
     r_n, th_n, nn = info.r_n, info.theta_n, info.nn
     launches = max(st.synth_launches, 1)
     launch_ms = st.synth_ms / launches
+    # Synthesis flops, three counts over the launches of the dominant kernel
+    # (the TSGT/RSGT contraction; main and accumulate launches):
+    #  alg:    the algorithmic count 4 sum_omega L(omega) sum_s P_s r_n theta_n,
+    #          P_s the real (coefficient, basis) products per right-hand side and
+    #          (r, theta): 8 per solid receiver over 18 P-SV right-hand sides
+    #          (144), 5 over 14 SH right-hand sides (70);
+    #  useful: this formulation's count, 4 flops per (coefficient row, l, r,
+    #          theta) with 63 (P-SV) / 36 (SH) rows per receiver, halved where the
+    #          distance symmetry computes theta and pi - theta together;
+    #  issued: the MFMA flops the bf16 kernels issue (all split cross products
+    #          over the tiles' degree extents).
+    rows, p_s = (PSV_ROWS, 144.0) if args.mode == "psv" else (SH_ROWS, 70.0)
+    sum_l = st.synth_flops / (4.0 * th_n * rows * r_n)
+    alg = 4.0 * sum_l * p_s * r_n * th_n
+    mirror = args.form in ("bf16x6", "bf16x3")
+    useful = st.synth_flops / (2.0 if mirror else 1.0)
     if args.form in ("bf16x6", "bf16x3"):
-        kernel = "synth_b16_kernel<%d, 0>" % (3 if args.form == "bf16x6" else 2)
-        flops = st.synth_mfma_flops / launches
-        peak, bound_note = BF16_PEAK, "MFMA flops issued (all bf16 cross products), bf16 dense peak"
+        kernel, peak = "synth_b16_kernel<%d, 0>" % (3 if args.form == "bf16x6" else 2), BF16_PEAK
+        issued = st.synth_mfma_flops
     elif args.form == "fp32":
-        kernel, flops, peak = "synth_f32_kernel<0>", st.synth_flops / launches, FP32_MFMA_PEAK
-        bound_note = "algorithmic flops, FP32 MFMA peak"
+        kernel, peak, issued = "synth_f32_kernel<0>", FP32_MFMA_PEAK, st.synth_flops
     else:
-        kernel, flops, peak = "synth_f64_kernel<0>", st.synth_flops / launches, FP64_MFMA_PEAK
-        bound_note = "algorithmic flops, FP64 MFMA peak"
-    achieved = flops / (launch_ms * 1e-3) if launch_ms > 0 else 0.0
+        kernel, peak, issued = "synth_f64_kernel<0>", FP64_MFMA_PEAK, st.synth_flops
+    flops = alg / launches
+    achieved = alg / (st.synth_ms * 1e-3) if st.synth_ms > 0 else 0.0
+    issued_rate = issued / (st.synth_ms * 1e-3) if st.synth_ms > 0 else 0.0
+    bound_note = ("achieved: the algorithmic flops per launch (4 sum L sum_s P_s r_n theta_n) over the average "
+                  "launch time; this formulation does alg/useful times fewer flops (63 coefficient rows per "
+                  "receiver instead of 144 products, and the distance symmetry), so frac can exceed 1; "
+                  "issued_frac is the MFMA issue rate of the kernel against the same peak")
     # HBM bytes per launch of the dominant kernel, P-SV bf16x6 form, 128
     # frequencies per batch, by the synthesis tile grouping of the plan:
     # rocprofv3 FETCH_SIZE and WRITE_SIZE per dispatch of
     # synth_b16_kernel<3, 0>, scaled by the counters of a plain 1 GiB copy
     # (x1.991, x0.967); profiles/synth_b16_psv_5s_catalog_mixed.md
-    traffic = None
-    if args.mode == "psv" and args.form == "bf16x6" and not args.imax:
-        tp = eng_plan if eng_plan is not None else {"nslot": 128, "xcd_group": 0}
-        if tp.get("nslot") in (0, 128):
-            traffic = {0: 2.351e10, 2: 2.847e10}.get(tp.get("xcd_group"))
+    traffic = None if args.imax else profile_traffic(args.mode, args.form, eng_plan)
     form_txt = {"bf16x6": "batched mixed: bf16x6 synthesis, FP64 solve with the depth cut",
                 "bf16x3": "batched mixed: bf16x3 synthesis, FP64 solve with the depth cut",
                 "fp32": "mixed: FP32 synthesis, FP64 solve with the depth cut", "fp64": "plain FP64"}[args.form]
@@ -415,12 +545,19 @@ def main():
                    "lmax": p["maxlmax"], "band_rows": nn, "radii": r_n, "distances": th_n, "l_cut": cuts},
         "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
                      "frac": achieved / peak, "traffic": traffic, "kernel": kernel, "avg_launch_ms": launch_ms,
-                     "flops_per_launch": flops, "note": bound_note},
+                     "launches": launches, "flops_per_launch": flops, "alg_flops": alg, "useful_flops": useful,
+                     "issued_flops": issued, "alg_over_useful": alg / useful if useful else None,
+                     "issued_tflops": issued_rate / 1e12, "issued_frac": issued_rate / peak, "note": bound_note},
         "stages_ms": {"solve+coef": st.solve_ms, "legendre": st.legendre_ms, "synth": st.synth_ms,
                       "other": st.other_ms},
         "steps_detail": steps_detail,
         "band_solve": band_solve(args.mode, args.form, st, not args.imax),
     }
+    if not args.no_output:
+        o = output_legs(inp, mode, prec, plan, p["imax"], args.out_dir)
+        o["in_hbm_s"] = ms / 1e3
+        out["catalog_output"] = o
+        print("catalog output: %s" % o, file=sys.stderr, flush=True)
     if tune is not None:
         out["autotune"] = tune
     if fp64 is not None:

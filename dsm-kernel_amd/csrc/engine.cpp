@@ -802,6 +802,8 @@ static int sh_run_batch(dsm_sgt_t *h, const ShPrep &p)
     std::vector<int> active(S);
     for (int k = 0; k < S; k++) active[k] = k;
     std::vector<double> amp((size_t)S * LC);
+    // rows solved per (slot, l): nn, or from the depth cut's first row
+    std::vector<int> neff((size_t)S * LC, nn), jst;
     int l0 = 0;
     while (!active.empty()) {
         const int lend = std::min(l0 + h->lchunk - 1, s.maxlmax);
@@ -863,7 +865,17 @@ static int sh_run_batch(dsm_sgt_t *h, const ShPrep &p)
         // |g0(nn)| of the chunk for every slot: one strided copy
         HIPCHK(h, hipMemcpy2DAsync(&amp[l0], sizeof(double) * LC, h->d_amp.as<double>() + l0, sizeof(double) * LC,
                                    sizeof(double) * (lend - l0 + 1), S, hipMemcpyDeviceToHost, h->stream));
+        if (h->dcut > 0.0) {
+            jst.resize((size_t)active.size() * nlb);
+            HIPCHK(h, hipMemcpyAsync(jst.data(), h->d_kstart.p, sizeof(int) * jst.size(), hipMemcpyDeviceToHost,
+                                     h->stream));
+        }
         HIPCHK(h, hipStreamSynchronize(h->stream));
+        if (h->dcut > 0.0)
+            for (size_t a = 0; a < active.size(); a++)
+                for (int b = 0; b < nlb; b++)
+                    for (int l = l0 + 64 * b; l <= std::min(lend, l0 + 64 * b + 63); l++)
+                        neff[(size_t)active[a] * LC + l] = nn - jst[a * nlb + b] + 1;
         // replay the cut-off in the reference's order (SGTsh-MPI.f90:423-478)
         std::vector<int> still;
         for (int k : active) {
@@ -959,6 +971,11 @@ static int sh_run_batch(dsm_sgt_t *h, const ShPrep &p)
     // 4 flops per (complex coefficient x real basis) per (r,theta); 36 rows
     // per receiver
     h->stats.synth_flops += 4.0 * sumL * s.theta_n * 36.0 * R;
+    // algorithmic bytes of the reference's systems l = 1 .. nl-1: the band
+    // (w = 2) of the rows solved and the (value, derivative) of 14
+    // right-hand sides at every receiver
+    for (int k = 0; k < S; k++)
+        for (int l = 1; l < nl[k]; l++) h->stats.solve_alg_bytes += 16.0 * (2.0 * neff[(size_t)k * LC + l] + 4.0 * 14 * R);
     HIPCHK(h, hipStreamSynchronize(h->stream));
     ev_collect(h);
     return DSM_SGT_OK;
@@ -1738,6 +1755,16 @@ static int psv_run_batch(dsm_sgt_t *h, const PsvPrep &p, bool full_range)
     for (int k = 0; k < S; k++) sumL += nl[k];
     // 4 flops per (complex coefficient x real function) per (r,theta)
     h->stats.synth_flops += 4.0 * sumL * s.theta_n * (double)PSV_NQ * R;
+    // algorithmic bytes of the reference's systems l = 1 .. nl-1: the band
+    // (w = 4 entries per row) of the rows solved -- from the depth cut's
+    // first row of the system's block of 256 degrees down to the surface --
+    // and (U, V, U', V') of 18 right-hand sides at every receiver
+    for (int k = 0; k < S; k++)
+        for (int l = 1; l < nl[k]; l++) {
+            int ne = n;
+            if (!kst_host.empty() && (l >> 8) < kst_nblk) ne = n - kst_host[(size_t)k * kst_nblk + (l >> 8)] + 1;
+            h->stats.solve_alg_bytes += 16.0 * (4.0 * ne + 4.0 * 18 * R);
+        }
     HIPCHK(h, dev_sync(h));
     free_graphs(h);
     if (h->ns) h->stats.synth_mfma_flops += b16_issued_flops(h, nlr_dev, S, R);

@@ -34,7 +34,8 @@ class Stats(C.Structure):
                 ("other_ms", C.c_double), ("solve_launches", C.c_int64), ("synth_launches", C.c_int64),
                 ("systems", C.c_int64), ("solve_bytes", C.c_double), ("synth_flops", C.c_double),
                 ("synth_bytes", C.c_double), ("synth_mfma_flops", C.c_double),
-                ("prep_wait_ms", C.c_double), ("device_wait_ms", C.c_double), ("solve_retries", C.c_int64)]
+                ("prep_wait_ms", C.c_double), ("device_wait_ms", C.c_double), ("solve_retries", C.c_int64),
+                ("solve_alg_bytes", C.c_double)]
 
 
 class Tuning(C.Structure):
@@ -157,7 +158,7 @@ class Engine:
     def copy_catalog_async(self, i0, i1, dst):
         """Records of [i0, i1] into `dst` (a HostBuffer or a writable
         complex64 numpy array of len(freqs) * record_block() values)."""
-        ptr, nbytes = (dst.ptr, dst.nbytes) if isinstance(dst, HostBuffer) else (dst.ctypes.data, dst.nbytes)
+        ptr, nbytes = (dst.ptr, dst.nbytes) if hasattr(dst, "ptr") else (dst.ctypes.data, dst.nbytes)
         self._keep = dst
         self._check(lib().dsm_sgt_copy_catalog_async(self._h, i0, i1, ptr, nbytes))
 
@@ -242,6 +243,14 @@ class HostBuffer:
             raise DsmSgtError(rc, f"cannot allocate {nbytes} bytes of pinned host memory")
         self.nbytes = nbytes
         self.ptr = self.ptr_c.value
+
+    def view(self, offset, nbytes):
+        """A part of the buffer, for copy_catalog_async."""
+        if offset < 0 or offset + nbytes > self.nbytes:
+            raise ValueError("view outside the buffer")
+        v = type("HostView", (), {})()
+        v.ptr, v.nbytes, v.parent = self.ptr + offset, nbytes, self
+        return v
 
     @property
     def array(self):

@@ -101,6 +101,9 @@ typedef struct dsm_sgt_stats {
     double device_wait_ms;    /* host time the run blocked on the device inside batches */
     int64_t solve_retries;    /* batched P-SV forms: batches run again with the full l range because a
                                  cut-off fell past the batch's l bound */
+    double solve_alg_bytes;   /* algorithmic bytes of the reference's (omega, l >= 1) systems below the
+                                 cut-off: 16 (w n + 4 R r_n) each, w = 4 / 2 band entries per row, n the rows
+                                 solved (after the depth cut), R = 18 / 14 right-hand sides */
 } dsm_sgt_stats_t;
 
 /* Create an engine on HIP device `device` for `mode` and `precision`.

@@ -97,7 +97,7 @@ def test_struct_layouts_match_the_header(tmp_path):
 int main(void) {
   printf("%zu %zu %zu %zu\\n", sizeof(dsm_sgt_tuning_t), sizeof(dsm_sgt_stats_t), sizeof(dsm_sgt_info_t),
          sizeof(dsm_sgt_input_t));
-  printf("%zu %zu %zu\\n", offsetof(dsm_sgt_tuning_t, catalog_window), offsetof(dsm_sgt_stats_t, solve_retries),
+  printf("%zu %zu %zu\\n", offsetof(dsm_sgt_tuning_t, catalog_window), offsetof(dsm_sgt_stats_t, solve_alg_bytes),
          offsetof(dsm_sgt_info_t, r0));
   return 0;
 }
@@ -108,4 +108,4 @@ int main(void) {
                                                                         check=True).stdout.split("\n")[:2]]
     assert sizes == [C.sizeof(dsm_sgt.Tuning), C.sizeof(dsm_sgt.Stats), C.sizeof(dsm_sgt.Info),
                      C.sizeof(dsm_sgt.DsmSgtInput)]
-    assert offs == [dsm_sgt.Tuning.catalog_window.offset, dsm_sgt.Stats.solve_retries.offset, dsm_sgt.Info.r0.offset]
+    assert offs == [dsm_sgt.Tuning.catalog_window.offset, dsm_sgt.Stats.solve_alg_bytes.offset, dsm_sgt.Info.r0.offset]
