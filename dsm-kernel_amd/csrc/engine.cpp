@@ -198,7 +198,7 @@ struct dsm_sgt {
     // BF16 forms: split operands, distance symmetry
     int ns = 0;                // bf16 terms per operand (0: not a BF16 form)
     int mirror = 0, nh = 0, nhp = 0;
-    DevBuf d_tb, d_eb, d_esb, d_stash, d_nlr, d_tiles;
+    DevBuf d_tb, d_eb, d_stash, d_nlr, d_tiles;
     std::vector<int> tiles_main, tiles_acc;  // host copies of the tile orders (launch sizes)
     int tiles_main_n = 0, tiles_acc_n = 0;
     DevBuf d_bfe, d_bzbase, d_bzsolid, d_bznl, d_bzkkdr, d_bcolz, d_bovrii, d_bovr, d_bscal;  // device band assembly
@@ -213,9 +213,9 @@ struct dsm_sgt {
     DevBuf d_rrow, d_vrow, d_omg, d_kstart, d_rlow, d_rneed, d_zs, d_loff, d_lsuf, d_llog, d_wsave;
     // fused FP64 family (mixed forms): compact band and its selector bits
     DevBuf d_cband, d_csel;
-    // l = 0 rows of the BF16 forms, written beside the solves and copied in
-    // after the coefficient kernel: [nq][ns][m] (one degree), slot list
-    DevBuf d_l0row, d_l0rows, d_l0slots;
+    // l = 0 receiver rows of the BF16 forms, written beside the solves and
+    // copied in after the coefficient kernel: [nq][ns][m] (one degree), slot list
+    DevBuf d_l0row, d_l0slots;
     // graph mode: the executable graphs of the current batch (destroyed once
     // the batch has synchronised)
     std::vector<hipGraphExec_t> gexec;
@@ -469,30 +469,26 @@ static double b16_issued_flops(const dsm_sgt_t *h, const std::vector<int> &nlr, 
     return fl;
 }
 
-// Device buffers of the BF16 forms for nslot frequencies: split rows and
-// the stash of the largest launch.
+// Device buffers of the BF16 forms for nslot frequencies: split receiver
+// rows and the stash of the largest launch (the SYNN rows are FP64).
 // lpad >= 0: only the degrees l >= lpad are zeroed (the caller writes every
 // lower degree a synthesis tile reads in each batch); < 0: all
-static int alloc_b16(dsm_sgt_t *h, int nslot, int nq, int nqs, int R, int lpad)
+static int alloc_b16(dsm_sgt_t *h, int nslot, int nq, int R, int lpad)
 {
     if (!h->ns) return DSM_SGT_OK;
-    const size_t M = (size_t)nslot * R * 2, Ms = (size_t)nslot * 2;
+    const size_t M = (size_t)nslot * R * 2;
     // the synthesis kernel addresses a split's rows with 32-bit offsets
     if (M * h->lcap >= (size_t)1 << 32)
         return fail(h, DSM_SGT_EUNSUPPORTED, "too many receivers x frequencies per batch for the bf16 synthesis");
     HIPCHK(h, h->d_eb.ensure(sizeof(unsigned short) * nq * h->ns * M * h->lcap));
-    HIPCHK(h, h->d_esb.ensure(sizeof(unsigned short) * nqs * h->ns * Ms * h->lcap));
     if (lpad >= 0) {
         launch_zero_pad_b16(h->d_eb.as<unsigned short>(), (size_t)nq * h->ns * M, h->lcap, lpad, h->stream);
-        launch_zero_pad_b16(h->d_esb.as<unsigned short>(), (size_t)nqs * h->ns * Ms, h->lcap, lpad, h->stream);
         HIPCHK(h, hipGetLastError());
     } else {
         HIPCHK(h, hipMemset(h->d_eb.p, 0, h->d_eb.n));
-        HIPCHK(h, hipMemset(h->d_esb.p, 0, h->d_esb.n));
     }
     HIPCHK(h, h->d_nlr.ensure(sizeof(int) * (size_t)nslot * R));
-    size_t g = synth_b16_grid(nslot, R, h->nhp, std::max(h->njobs_main, h->njobs_acc));
-    g = std::max(g, synth_b16_grid(nslot, 1, h->nhp, h->njobs_syn));
+    const size_t g = synth_b16_grid(nslot, R, h->nhp, std::max(h->njobs_main, h->njobs_acc));
     HIPCHK(h, h->d_stash.ensure(sizeof(double) * g * 8 * 32 * 64));
     return DSM_SGT_OK;
 }
@@ -718,7 +714,7 @@ static int sh_create(dsm_sgt_t *h)
             for (int b = 0; b < 6; b++) par[(mu - 1) * 6 + b] = fpar[b] * ((mu & 1) ? -1 : 1);
         set_mirror(h, par);
     }
-    rc = alloc_b16(h, S, 36, 5, R, -1);
+    rc = alloc_b16(h, S, 36, R, -1);
     if (rc) return rc;
     HIPCHK(h, h->d_jobs.ensure(sizeof(SynthJob) * h->jobs.size()));
     HIPCHK(h, hipMemcpy(h->d_jobs.p, h->jobs.data(), sizeof(SynthJob) * h->jobs.size(), hipMemcpyHostToDevice));
@@ -914,13 +910,13 @@ static int sh_run_batch(dsm_sgt_t *h, const ShPrep &p)
     HIPCHK(h, hipMemcpyAsync(h->d_slots.p, all.data(), sizeof(int) * S, hipMemcpyHostToDevice, h->stream));
     TimedLaunch tz;
     ev_begin(h, tz, 3);
-    if (!h->ns) {  // (the bf16 split writes the zeros of the BF16 forms)
+    // (the bf16 split writes the zeros of the receiver rows of the BF16
+    // forms; the SYNN rows stay FP64)
+    if (!h->ns)
         launch_zero_tail(h->d_E.as<double2>(), S, 36 * R, h->d_slots.as<int>(), h->d_nl.as<int>(), LC, h->stream);
-        launch_zero_tail(h->d_Es.as<double2>(), S, 5, h->d_slots.as<int>(), h->d_nl.as<int>(), LC, h->stream);
-    }
+    launch_zero_tail(h->d_Es.as<double2>(), S, 5, h->d_slots.as<int>(), h->d_nl.as<int>(), LC, h->stream);
     HIPCHK(h, hipGetLastError());
     int rc = split_rows(h, h->d_E, S, 36, R, h->d_eb);
-    if (!rc) rc = split_rows(h, h->d_Es, S, 5, 1, h->d_esb);
     if (rc) return rc;
     if (h->ns) {
         std::vector<int> nlr((size_t)R * S);
@@ -962,7 +958,7 @@ static int sh_run_batch(dsm_sgt_t *h, const ShPrep &p)
     ga.njobs = h->njobs_syn;
     TimedLaunch tg2;
     ev_begin(h, tg2, 3);
-    launch_synth(h, ga, h->d_esb);
+    launch_synth_f64(ga, h->stream);  // SYNN in FP64 in every form (as P-SV)
     HIPCHK(h, hipGetLastError());
     ev_end(h, tg2);
     h->stats.synth_launches += h->njobs_acc ? 2 : 1;
@@ -1129,10 +1125,10 @@ static int psv_create(dsm_sgt_t *h)
         HIPCHK(h, h->d_llog.ensure(sizeof(int) * S));
     }
     HIPCHK(h, h->d_xn.ensure(sizeof(double2) * (size_t)S * h->nneed * PSV_NSOL * LC));
-    if (!h->ns) {  // the bf16 forms write split rows directly (alloc_b16)
-        HIPCHK(h, h->d_E.ensure(sizeof(double2) * (size_t)S * PSV_NQ * R * LC));
-        HIPCHK(h, h->d_Es.ensure(sizeof(double2) * (size_t)S * PSV_NQS * LC));
-    }
+    // the bf16 forms write the receiver rows split (alloc_b16); the SYNN
+    // rows are FP64 in every form
+    if (!h->ns) HIPCHK(h, h->d_E.ensure(sizeof(double2) * (size_t)S * PSV_NQ * R * LC));
+    HIPCHK(h, h->d_Es.ensure(sizeof(double2) * (size_t)S * PSV_NQS * LC));
     HIPCHK(h, h->d_amp.ensure(sizeof(double2) * 2 * (size_t)S * LC));
     const size_t np = (size_t)std::max(1, h->ncat);
     HIPCHK(h, h->d_cat_t.ensure(sizeof(double2) * np * R * s.theta_n * 20));
@@ -1152,11 +1148,10 @@ static int psv_create(dsm_sgt_t *h)
     }
     // P-SV batches write every degree up to maxlmax that a tile reads:
     // coefficient rows, l = 0 rows, cut-off tails, zero rows below the cut
-    rc = alloc_b16(h, S, PSV_NQ, PSV_NQS, R, s.maxlmax + 1);
+    rc = alloc_b16(h, S, PSV_NQ, R, s.maxlmax + 1);
     if (rc) return rc;
     if (h->ns) {
         HIPCHK(h, h->d_l0row.ensure(sizeof(unsigned short) * PSV_NQ * h->ns * (size_t)S * R * 2));
-        HIPCHK(h, h->d_l0rows.ensure(sizeof(unsigned short) * PSV_NQS * h->ns * (size_t)S * 2));
         HIPCHK(h, h->d_l0slots.ensure(sizeof(int) * S));
     }
     HIPCHK(h, h->d_jobs.ensure(sizeof(SynthJob) * h->jobs.size()));
@@ -1378,7 +1373,6 @@ static int psv_run_batch(dsm_sgt_t *h, const PsvPrep &p, bool full_range)
             // one-degree rows (lcap 1) beside the solves, on the second
             // stream once the band and the scalars are in place
             za.eb = B16Rows{h->d_l0row.as<unsigned short>(), h->ns, S, R, 1};
-            za.esb = B16Rows{h->d_l0rows.as<unsigned short>(), h->ns, S, 1, 1};
             za.slot_list = h->d_l0slots.as<int>();
             HIPCHK(h, hipMemcpyAsync(h->d_l0slots.p, active.data(), sizeof(int) * S, hipMemcpyHostToDevice,
                                      h->stream));
@@ -1386,7 +1380,6 @@ static int psv_run_batch(dsm_sgt_t *h, const PsvPrep &p, bool full_range)
             HIPCHK(h, hipStreamWaitEvent(h->stream2, h->ev_l0in, 0));
             // rows the l = 0 system leaves unwritten stay zero (not captured)
             HIPCHK(h, hipMemsetAsync(h->d_l0row.p, 0, h->d_l0row.n, h->stream2));
-            HIPCHK(h, hipMemsetAsync(h->d_l0rows.p, 0, h->d_l0rows.n, h->stream2));
             launch_psv_l0(za, h->stream2);
             HIPCHK(h, hipGetLastError());
             HIPCHK(h, hipEventRecord(h->ev_l0out, h->stream2));
@@ -1560,7 +1553,6 @@ static int psv_run_batch(dsm_sgt_t *h, const PsvPrep &p, bool full_range)
         }
         if (h->ns) {
             ca.eb = B16Rows{h->d_eb.as<unsigned short>(), h->ns, S, R, LC};
-            ca.esb = B16Rows{h->d_esb.as<unsigned short>(), h->ns, S, 1, LC};
             if (h->dcut > 0.0 && l0 == 0 && lend == lmaxb) ca.rneed = h->d_rneed.as<unsigned char>();
         }
         launch_psv_coef(ca, nblocks, h->stream);
@@ -1569,8 +1561,6 @@ static int psv_run_batch(dsm_sgt_t *h, const PsvPrep &p, bool full_range)
             if (!gmode) HIPCHK(h, hipStreamWaitEvent(h->stream, h->ev_l0out, 0));
             launch_b16_put_l0(h->d_l0row.as<unsigned short>(), h->d_eb.as<unsigned short>(),
                               (size_t)PSV_NQ * h->ns * S * R * 2, LC, h->stream);
-            launch_b16_put_l0(h->d_l0rows.as<unsigned short>(), h->d_esb.as<unsigned short>(),
-                              (size_t)PSV_NQS * h->ns * S * 2, LC, h->stream);
             HIPCHK(h, hipGetLastError());
         }
         if (dev_cut) {
@@ -1648,8 +1638,7 @@ static int psv_run_batch(dsm_sgt_t *h, const PsvPrep &p, bool full_range)
         // blocks of l whose solved rows reach the receiver's stencil
         launch_zero_tail_b16(h->d_eb.as<unsigned short>(), PSV_NQ, h->ns, S, R, LC, h->d_nl.as<int>(), s.maxlmax,
                              h->stream);
-        launch_zero_tail_b16(h->d_esb.as<unsigned short>(), PSV_NQS, h->ns, S, 1, LC, h->d_nl.as<int>(), s.maxlmax,
-                             h->stream);
+        launch_zero_tail(h->d_Es.as<double2>(), S, PSV_NQS, h->d_slots.as<int>(), h->d_nl.as<int>(), LC, h->stream);
         if (h->dcut > 0.0 && !kst_host.empty()) {
             launch_psv_nlr(h->d_kstart.as<int>(), h->d_rlow.as<int>(), h->d_nl.as<int>(), S, kst_nblk, R,
                            h->d_nlr.as<int>(), h->stream);
@@ -1721,6 +1710,10 @@ static int psv_run_batch(dsm_sgt_t *h, const PsvPrep &p, bool full_range)
     } else {
         ev_end(h, tg);
     }
+    // SYNN rows (10 per frequency, a few percent of the synthesis): FP64
+    // MFMA in every form.  The surface displacement of a shallow source at
+    // far distances is a near-field series that cancels over l, where the
+    // FP32 sums of the mixed forms lose up to 1e-4 of it.
     ga.E = h->d_Es.as<double2>();
     ga.nq = PSV_NQS;
     ga.r_eff = 1;
@@ -1728,7 +1721,7 @@ static int psv_run_batch(dsm_sgt_t *h, const PsvPrep &p, bool full_range)
     ga.njobs = h->njobs_syn;
     TimedLaunch tg2;
     ev_begin(h, tg2, 3);
-    launch_synth(h, ga, h->d_esb);
+    launch_synth_f64(ga, h->stream);
     HIPCHK(h, hipGetLastError());
     ev_end(h, tg2);
     h->stats.synth_launches += 2;

@@ -883,9 +883,15 @@ __global__ __launch_bounds__(64) void psv_coef_b16_kernel(PsvCoefArgs a)
             vs[s] = xv(a.syn_i1, s);
         }
         psv_synn_rows(us, vs, il, mu1, mu2, es);
-        unsigned short *base = wave_row_base(a.esb, 0, sl, lw0, t);
-        const size_t qs = (size_t)a.esb.ns * a.esb.nslot * 2 * a.esb.lcap;
-        for (int q = 0; q < PSV_NQS; q++) wave_put(a.esb, st, base, qs, q, live ? es[q] : cz(), t);
+        if (a.Es) {  // SYNN rows in FP64 (degree 0 is psv_l0_kernel's)
+            double2 *Es = a.Es + (size_t)sl * PSV_NQS * LC + lraw;
+            if (lraw != 0)
+                for (int q = 0; q < PSV_NQS; q++) Es[(size_t)q * LC] = live ? es[q] : cz();
+        } else {
+            unsigned short *base = wave_row_base(a.esb, 0, sl, lw0, t);
+            const size_t qs = (size_t)a.esb.ns * a.esb.nslot * 2 * a.esb.lcap;
+            for (int q = 0; q < PSV_NQS; q++) wave_put(a.esb, st, base, qs, q, live ? es[q] : cz(), t);
+        }
     }
     const size_t qstride = (size_t)a.eb.ns * a.eb.nslot * a.eb.r_n * 2 * a.eb.lcap;
     const int kst = a.kstart ? a.kstart[b / a.nlb * a.nblk + ((lw0 - a.l0) >> 8)] : 1;
@@ -922,7 +928,7 @@ __global__ __launch_bounds__(64) void psv_coef_kernel(PsvCoefArgs a)
 
 void launch_psv_coef(const PsvCoefArgs &a, int nblocks, hipStream_t s)
 {
-    if (a.eb.Eb && a.esb.Eb && (a.l0 % 64) == 0 && a.eb.ns <= 3)
+    if (a.eb.Eb && (a.esb.Eb || a.Es) && (a.l0 % 64) == 0 && a.eb.ns <= 3)
         hipLaunchKernelGGL(psv_coef_b16_kernel, dim3(nblocks), dim3(64), 0, s, a);
     else
         hipLaunchKernelGGL(psv_coef_kernel, dim3(nblocks), dim3(64), 0, s, a);

@@ -854,11 +854,11 @@ __host__ __device__ inline void psv_coef_lane(const PsvCoefArgs &a, int b, int t
             vs[s] = xv(a.syn_i1, s);
         }
         psv_synn_rows(us, vs, il, mu1, mu2, es);
-        if (a.esb.Eb) {
-            for (int q = 0; q < PSV_NQS; q++) a.esb.put(q, 0, sl, l, es[q]);
-        } else {
+        if (a.Es) {
             double2 *Es = a.Es + (size_t)sl * PSV_NQS * LC + l;
             for (int q = 0; q < PSV_NQS; q++) Es[(size_t)q * LC] = es[q];
+        } else {
+            for (int q = 0; q < PSV_NQS; q++) a.esb.put(q, 0, sl, l, es[q]);
         }
     }
     double2 *E = a.E + (size_t)sl * PSV_NQ * R * LC + l;
@@ -1046,11 +1046,11 @@ __host__ __device__ inline void psv_l0_rows(const PsvL0Args &a, int t, int r)
         us[0] = z[N0 - 1];
         us[1] = z[(size_t)N0 + N0 - 1];
         psv_synn_rows(us, vs, 0.0, false, false, es);
-        if (a.esb.Eb) {
-            for (int q = 0; q < PSV_NQS; q++) a.esb.put(q, 0, sl, 0, es[q]);
-        } else {
+        if (a.Es) {
             double2 *Es = a.Es + (size_t)sl * PSV_NQS * LC;
             for (int q = 0; q < PSV_NQS; q++) Es[(size_t)q * LC] = es[q];
+        } else {
+            for (int q = 0; q < PSV_NQS; q++) a.esb.put(q, 0, sl, 0, es[q]);
         }
     }
 }

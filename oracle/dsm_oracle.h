@@ -79,4 +79,15 @@ int orc_psv_omega(const orc_input_t *in, int32_t i, int32_t use_ref,
                   double complex *tsgt, double complex *rsgt,
                   double complex *synn, int32_t *llog);
 
+/* Test hook of both drivers: while on, every call of calamp (the l cut-off
+ * update, SGTpsv-MPI.f90:902-905, SGTsh-MPI.f90:476-478) is recorded as
+ * (l, amplitude the call computes) into the caller's arrays, at most cap
+ * entries.  orc_calamp_trace(l, amp, cap) turns it on and clears the count;
+ * cap = 0 turns it off.  orc_calamp_count() returns the calls recorded
+ * (possibly more than cap).  Not thread-safe; one driver call at a time. */
+int orc_calamp_trace(int32_t *l, double *amp, int32_t cap);
+int orc_calamp_count(void);
+/* (internal) record one call: l and the n complex values calamp reads */
+void orc_trace_calamp(int l, const double complex *g, int n);
+
 #endif

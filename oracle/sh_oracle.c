@@ -757,6 +757,34 @@ static void o_calcoef(const int *nzone, const double *omega, const double *q, zc
     }
 }
 
+/* calamp trace (dsm_oracle.h) */
+static int32_t *TR_L = NULL;
+static double *TR_A = NULL;
+static int32_t TR_CAP = 0, TR_N = 0;
+
+int orc_calamp_trace(int32_t *l, double *amp, int32_t cap)
+{
+    TR_L = cap > 0 ? l : NULL;
+    TR_A = cap > 0 ? amp : NULL;
+    TR_CAP = cap > 0 ? cap : 0;
+    TR_N = 0;
+    return 0;
+}
+
+int orc_calamp_count(void) { return TR_N; }
+
+void orc_trace_calamp(int l, const double complex *g, int n)
+{
+    if (!TR_CAP) return;
+    double s = 0.0;
+    for (int k = 0; k < n; k++) s += cabs(g[k]) * cabs(g[k]);
+    if (TR_N < TR_CAP) {
+        TR_L[TR_N] = l;
+        TR_A[TR_N] = sqrt(s);
+    }
+    TR_N++;
+}
+
 /* calamp  others.f90:805-824 */
 static void o_calamp(const zc *g, const int *l, const int *lsuf, double *maxamp, int *ismall,
                      const double *ratl)
@@ -1318,7 +1346,10 @@ int orc_sh_omega(const orc_input_t *in, int32_t i, int32_t use_ref, double compl
                 } else {
                     F->dcsbsub0(a, &nn, &one, &lda, g0, &eps, dr, z, &ier);
                 }
-                if (imt == 3) F->calamp(&g0[nn - 1], &l, &lsuf, &maxamp, &ismall, &s->ratl);
+                if (imt == 3) {
+                    orc_trace_calamp(l, &g0[nn - 1], 1);
+                    F->calamp(&g0[nn - 1], &l, &lsuf, &maxamp, &ismall, &s->ratl);
+                }
                 if (s->synnswitch == 1) {
                     for (int it = 0; it < theta_n; it++) {
                         zc u[3] = {0, 0, 0};

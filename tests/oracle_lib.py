@@ -36,7 +36,14 @@ def lib(fast=False):
         path = os.path.join(ORACLE_DIR, name)
         if not os.path.exists(path):
             raise RuntimeError(f"oracle not built: {path} (run __graft_entry__.build())")
-        L = C.CDLL(path)
+        _LIBS[name] = load(path)
+    return _LIBS[name]
+
+
+def load(path):
+    """An oracle build at `path` (e.g. a scratch build of modified sources)."""
+    L = C.CDLL(path)
+    if True:
         L.orc_sh_info.argtypes = [C.POINTER(DsmInput), C.POINTER(ShInfo)]
         L.orc_sh_load_ref.argtypes = [C.c_char_p]
         L.orc_sh_omega.argtypes = [C.POINTER(DsmInput), C.c_int32, C.c_int32,
@@ -46,8 +53,9 @@ def lib(fast=False):
         L.orc_psv_mix.argtypes = [C.c_uint64]
         L.orc_psv_omega.argtypes = [C.POINTER(DsmInput), C.c_int32, C.c_int32,
                                     C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_int32)]
-        _LIBS[name] = L
-    return _LIBS[name]
+        L.orc_calamp_trace.argtypes = [C.c_void_p, C.c_void_p, C.c_int32]
+        L.orc_calamp_count.argtypes = []
+    return L
 
 
 def ref_path(kind):
@@ -66,10 +74,11 @@ def sh_info(inp):
     return info
 
 
-def sh_omega(inp, i, use_ref=False, fast=False):
+def sh_omega(inp, i, use_ref=False, fast=False, L=None):
     """One SGTsh frequency: FP64 accumulators (tsgt[theta][r][10],
-    rsgt[theta][r][5], synn[theta][5]) and the logged l cut-off."""
-    L = lib(fast)
+    rsgt[theta][r][5], synn[theta][5]) and the logged l cut-off.  L: an
+    oracle build from load() instead of the built one."""
+    L = L or lib(fast)
     if use_ref:
         rc = L.orc_sh_load_ref(ref_path("sh").encode())
         if rc:
@@ -95,12 +104,13 @@ def psv_info(inp):
     return info
 
 
-def psv_omega(inp, i, use_ref=False, fast=False, mix=None):
+def psv_omega(inp, i, use_ref=False, fast=False, mix=None, L=None):
     """One SGTpsv frequency: FP64 accumulators (tsgt[theta][r][20],
     rsgt[theta][r][10], synn[theta][10]) and the logged l cut-off.
     use_ref: the reference's routines for every call; mix: a bit mask
-    selecting the reference routine per entry of the oracle's table."""
-    L = lib(fast)
+    selecting the reference routine per entry of the oracle's table; L: an
+    oracle build from load() instead of the built one."""
+    L = L or lib(fast)
     if use_ref or mix is not None:
         rc = L.orc_psv_load_ref(ref_path("psv").encode())
         if rc:
@@ -122,3 +132,20 @@ def psv_omega(inp, i, use_ref=False, fast=False, mix=None):
     if rc:
         raise RuntimeError(f"orc_psv_omega rc={rc}")
     return tsgt, rsgt, synn, llog.value
+
+
+def calamp_trace(fn, inp, i, L=None, cap=1 << 20):
+    """Run one frequency (fn = psv_omega or sh_omega) with the calamp trace
+    on: (l of every call, amplitude every call computed, result of fn)."""
+    L = L or lib()
+    ls = np.zeros(cap, np.int32)
+    amps = np.zeros(cap, np.float64)
+    L.orc_calamp_trace(ls.ctypes.data, amps.ctypes.data, cap)
+    try:
+        res = fn(inp, i, L=L)
+        n = L.orc_calamp_count()
+    finally:
+        L.orc_calamp_trace(None, None, 0)
+    if n > cap:
+        raise RuntimeError("calamp trace overflow")
+    return ls[:n].copy(), amps[:n].copy(), res
