@@ -26,8 +26,10 @@
 #include <thread>
 #include <vector>
 
+#include "dsm_km.h"
 #include "dsm_sgt.h"
 #include "fe_setup.h"
+#include "km_stage.h"
 #include "kernels.h"
 #include "legendre_lane.h"
 #include "psv_jobs.h"
@@ -239,6 +241,11 @@ struct dsm_sgt {
     std::string io_err;
     float2 *pin = nullptr;  // pinned staging of the writer (NPIN blocks)
     size_t pin_n = 0;
+    // KernelMaker catalog stage (dsm_km_points): one radius's catalog, its
+    // phase tables, the points and their outputs
+    DevBuf d_km_omt, d_km_omr, d_km_ph, d_km_pos, d_km_thd, d_km_pts, d_km_php, d_km_out;
+    DevBuf d_km_fin, d_km_fout, d_km_tw;  // dsm_km_vectorfft: spectra, windows, twiddles
+    int km_tw_n = 0;
     // timing
     bool timing = true;
     std::vector<TimedLaunch> pending;
@@ -2456,6 +2463,225 @@ int dsm_sgt_write_catalog(dsm_sgt_t *h, int32_t i, const char *output_dir, const
     make_dirs(output_dir);
     std::string err;
     if (!write_block_files(h, output_dir, model_name, i, blk.data(), err)) return fail(h, DSM_SGT_EIO, err);
+    return DSM_SGT_OK;
+}
+
+// ------------------------------------------------------------------------
+// KernelMaker catalog stage
+// ------------------------------------------------------------------------
+// The reference's exp(cmplx(0.d0, x)): CMPLX of default kind, so x rounded
+// to float and the factor's cos and sin taken in float, as the C library's
+// cosf / sinf give them (flang's runtime calls the same functions).
+static float2 km_phase(double x)
+{
+    const float xf = (float)x;
+    return make_float2(std::cos(xf), std::sin(xf));
+}
+
+// catalog position of frequency i in engine e, -1 if i = 0; -2 if not resident
+static int km_pos(const dsm_sgt_t *e, int i)
+{
+    if (i == 0) return -1;
+    const int p = i - e->ipos0;
+    if (p < 0 || p >= e->npos || !e->done[p] || e->catpos[p] < 0) return -2;
+    return e->catpos[p];
+}
+
+// fn(k) for k in [0, n) split over host threads
+extern "C++" template <class F> inline void km_host_parallel(long n, F fn)
+{
+    const unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    if (n < 4096 || nt == 1) {
+        for (long k = 0; k < n; k++) fn(k);
+        return;
+    }
+    std::vector<std::thread> th;
+    for (unsigned t = 0; t < nt; t++)
+        th.emplace_back([&, t] {
+            for (long k = (long)t * n / nt; k < (long)(t + 1) * n / nt; k++) fn(k);
+        });
+    for (auto &x : th) x.join();
+}
+
+int dsm_km_points(dsm_sgt_t *h, dsm_sgt_t *sh, int32_t ir, const dsm_km_params_t *prm, int32_t npoints,
+                  const double *deltar, const double *deltas, const double *trig, void *h3, void *h4)
+{
+    if (!h || !prm) return DSM_SGT_EINVAL;
+    if (!h->ready || h->mode != DSM_SGT_PSV) return fail(h, DSM_SGT_ESTATE, "needs a created P-SV engine");
+    if (npoints < 0 || (npoints > 0 && (!deltar || !deltas || !trig || !h3 || !h4)))
+        return fail(h, DSM_SGT_EINVAL, "bad point arrays");
+    if (ir < 0 || ir >= h->r_n) return fail(h, DSM_SGT_EINVAL, "receiver index out of range");
+    if (prm->compo < 0 || prm->compo > 2) return fail(h, DSM_SGT_EINVAL, "compo must be 0 (Z), 1 (R) or 2 (T)");
+    if (!(prm->ipdistance > 0.0)) return fail(h, DSM_SGT_EINVAL, "ipdistance must be positive");
+    if (prm->fmin < 0 || prm->fmax < prm->fmin) return fail(h, DSM_SGT_EINVAL, "bad frequency range");
+    const int nf = prm->fmax - prm->fmin + 1, T = h->theta_n;
+    std::vector<int> pos(nf), pos_sh(nf, -1);
+    for (int j = 0; j < nf; j++) {
+        pos[j] = km_pos(h, prm->fmin + j);
+        if (pos[j] == -2)
+            return fail(h, DSM_SGT_ESTATE, "frequency " + std::to_string(prm->fmin + j) + " not resident in the P-SV engine");
+    }
+    // the SH engine's records of the same radius, if it holds that radius
+    int ir_sh = -1;
+    if (sh) {
+        if (!sh->ready || sh->mode != DSM_SGT_SH) return fail(h, DSM_SGT_EINVAL, "the second engine must be an SH engine");
+        if (sh->theta_n != T || sh->in.tlen != h->in.tlen || sh->in.thetamin != h->in.thetamin ||
+            sh->in.thetadelta != h->in.thetadelta)
+            return fail(h, DSM_SGT_EINVAL, "the SH engine's distances or tlen differ from the P-SV engine's");
+        for (int k = 0; k < sh->r_n; k++)
+            if (sh->radii[k] == h->radii[ir]) ir_sh = k;
+        if (ir_sh >= 0)
+            for (int j = 0; j < nf; j++) {
+                pos_sh[j] = km_pos(sh, prm->fmin + j);
+                if (pos_sh[j] == -2)
+                    return fail(h, DSM_SGT_ESTATE,
+                                "frequency " + std::to_string(prm->fmin + j) + " not resident in the SH engine");
+            }
+    }
+    if (hipSetDevice(h->device) != hipSuccess) return fail(h, DSM_SGT_EHIP, "hipSetDevice failed");
+    if (sh && ir_sh >= 0 && sh->device != h->device) return fail(h, DSM_SGT_EINVAL, "engines on different devices");
+    const double tlen = h->in.tlen, cred = prm->c_red_reci;
+    constexpr double PI_KM = 3.1415926535897932;  // KernelMaker's module parameters
+    // phase tables: rdsgtomega's per (frequency, distance), clsgt's per
+    // (point, frequency) for deltar and deltas
+    std::vector<float2> ph((size_t)nf * T), php((size_t)2 * npoints * nf);
+    km_host_parallel((long)nf * T, [&](long k) {
+        const int j = (int)(k / T), t = (int)(k % T);
+        const double redtime = -h->theta_deg[t] * cred;
+        ph[k] = km_phase(-2.0 * PI_KM * (double)(prm->fmin + j) / tlen * redtime);
+    });
+    km_host_parallel((long)npoints * nf, [&](long k) {
+        const int p = (int)(k / nf), j = (int)(k % nf);
+        const double f = -2.0 * PI_KM * (double)(prm->fmin + j) / tlen;
+        php[k] = km_phase(f * deltar[p] * cred);
+        php[(size_t)npoints * nf + k] = km_phase(f * deltas[p] * cred);
+    });
+    std::vector<int> posall(pos);
+    posall.insert(posall.end(), pos_sh.begin(), pos_sh.end());
+    HIPCHK(h, h->d_km_omt.ensure(sizeof(double2) * (size_t)nf * T * 20));
+    HIPCHK(h, h->d_km_omr.ensure(sizeof(double2) * (size_t)nf * T * 10));
+    HIPCHK(h, h->d_km_ph.ensure(sizeof(float2) * ph.size()));
+    HIPCHK(h, h->d_km_pos.ensure(sizeof(int) * posall.size()));
+    HIPCHK(h, h->d_km_thd.ensure(sizeof(double) * T));
+    HIPCHK(h, h->d_km_pts.ensure(sizeof(double) * std::max<size_t>(1, (size_t)npoints * 14)));
+    HIPCHK(h, h->d_km_php.ensure(sizeof(float2) * std::max<size_t>(1, php.size())));
+    const size_t nout = (size_t)npoints * nf * 6;
+    HIPCHK(h, h->d_km_out.ensure(sizeof(double2) * std::max<size_t>(1, 2 * nout)));
+    HIPCHK(h, hipMemcpyAsync(h->d_km_ph.p, ph.data(), sizeof(float2) * ph.size(), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipMemcpyAsync(h->d_km_pos.p, posall.data(), sizeof(int) * posall.size(), hipMemcpyHostToDevice,
+                             h->stream));
+    HIPCHK(h, hipMemcpyAsync(h->d_km_thd.p, h->theta_deg.data(), sizeof(double) * T, hipMemcpyHostToDevice,
+                             h->stream));
+    if (npoints > 0) {
+        double *pts = h->d_km_pts.as<double>();
+        HIPCHK(h, hipMemcpyAsync(pts, deltar, sizeof(double) * npoints, hipMemcpyHostToDevice, h->stream));
+        HIPCHK(h, hipMemcpyAsync(pts + npoints, deltas, sizeof(double) * npoints, hipMemcpyHostToDevice, h->stream));
+        HIPCHK(h, hipMemcpyAsync(pts + 2 * (size_t)npoints, trig, sizeof(double) * 12 * npoints,
+                                 hipMemcpyHostToDevice, h->stream));
+        HIPCHK(h, hipMemcpyAsync(h->d_km_php.p, php.data(), sizeof(float2) * php.size(), hipMemcpyHostToDevice,
+                                 h->stream));
+    }
+    KmRdArgs ra{};
+    ra.cat_t_psv = h->d_cat_t.as<double2>();
+    ra.cat_r_psv = h->d_cat_r.as<double2>();
+    ra.r_psv = h->r_n;
+    ra.ir_psv = ir;
+    if (sh && ir_sh >= 0) {
+        ra.cat_t_sh = sh->d_cat_t.as<double2>();
+        ra.cat_r_sh = sh->d_cat_r.as<double2>();
+        ra.r_sh = sh->r_n;
+        ra.ir_sh = ir_sh;
+    }
+    ra.pos_psv = h->d_km_pos.as<int>();
+    ra.pos_sh = h->d_km_pos.as<int>() + nf;
+    ra.ph = h->d_km_ph.as<float2>();
+    ra.nf = nf;
+    ra.theta_n = T;
+    ra.om_t = h->d_km_omt.as<double2>();
+    ra.om_r = h->d_km_omr.as<double2>();
+    // the SH engine's last run must be complete before its catalog is read
+    if (sh && ir_sh >= 0) HIPCHK(sh, hipStreamSynchronize(sh->stream));
+    TimedLaunch t;
+    ev_begin(h, t, 3);
+    launch_km_rd(ra, h->stream);
+    HIPCHK(h, hipGetLastError());
+    KmPointArgs pa{};
+    pa.om_t = ra.om_t;
+    pa.om_r = ra.om_r;
+    pa.thetaD = h->d_km_thd.as<double>();
+    pa.theta_n = T;
+    pa.nf = nf;
+    pa.npoints = npoints;
+    pa.compo = prm->compo;
+    pa.thetadelta = h->in.thetadelta;
+    pa.ipdistance = prm->ipdistance;
+    pa.deltar = h->d_km_pts.as<double>();
+    pa.deltas = h->d_km_pts.as<double>() + npoints;
+    pa.trig = h->d_km_pts.as<double>() + 2 * (size_t)npoints;
+    pa.ph_r = h->d_km_php.as<float2>();
+    pa.ph_s = h->d_km_php.as<float2>() + (size_t)npoints * nf;
+    for (int k = 0; k < 6; k++) pa.mt[k] = prm->mt[k];
+    pa.h3 = h->d_km_out.as<double2>();
+    pa.h4 = h->d_km_out.as<double2>() + nout;
+    launch_km_points(pa, h->stream);
+    HIPCHK(h, hipGetLastError());
+    ev_end(h, t);
+    if (npoints > 0) {
+        HIPCHK(h, hipMemcpyAsync(h3, pa.h3, sizeof(double2) * nout, hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(h, hipMemcpyAsync(h4, pa.h4, sizeof(double2) * nout, hipMemcpyDeviceToHost, h->stream));
+    }
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    ev_collect(h);
+    return DSM_SGT_OK;
+}
+
+int dsm_km_vectorfft(dsm_sgt_t *h, int32_t nvec, int32_t imin, int32_t imax, int32_t np1, const void *ccvec,
+                     double omegai, double tlen, int32_t iws, int32_t iwe, double *rvec)
+{
+    if (!h) return DSM_SGT_EINVAL;
+    if (!h->ready) return fail(h, DSM_SGT_ESTATE, "engine not initialised");
+    if (np1 < 2 || np1 > KM_FFT_MAX_NP1 || (np1 & (np1 - 1)))
+        return fail(h, DSM_SGT_EINVAL, "np1 must be a power of two in 2.." + std::to_string(KM_FFT_MAX_NP1));
+    const int N = 2 * np1;
+    if (imin < 0 || imax < imin || imax >= N || iws < 0 || iwe < iws || iwe >= N)
+        return fail(h, DSM_SGT_EINVAL, "spectrum or window outside 0..2 np1 - 1");
+    if (nvec < 0 || (nvec > 0 && (!ccvec || !rvec))) return fail(h, DSM_SGT_EINVAL, "bad arrays");
+    if (!(tlen > 0.0)) return fail(h, DSM_SGT_EINVAL, "tlen must be positive");
+    if (nvec == 0) return DSM_SGT_OK;
+    if (hipSetDevice(h->device) != hipSuccess) return fail(h, DSM_SGT_EHIP, "hipSetDevice failed");
+    const size_t nin = (size_t)nvec * (imax - imin + 1), nout = (size_t)nvec * (iwe - iws + 1);
+    HIPCHK(h, h->d_km_fin.ensure(sizeof(double2) * nin));
+    HIPCHK(h, h->d_km_fout.ensure(sizeof(double) * nout));
+    if (h->km_tw_n != N) {  // e^{2 pi i k / N}, k < N / 2
+        std::vector<double2> tw(N / 2);
+        for (int k = 0; k < N / 2; k++) {
+            const double a = 2.0 * 3.141592653589793 * (double)k / (double)N;
+            tw[k] = make_double2(std::cos(a), std::sin(a));
+        }
+        HIPCHK(h, h->d_km_tw.ensure(sizeof(double2) * tw.size()));
+        HIPCHK(h, hipMemcpy(h->d_km_tw.p, tw.data(), sizeof(double2) * tw.size(), hipMemcpyHostToDevice));
+        h->km_tw_n = N;
+    }
+    HIPCHK(h, hipMemcpyAsync(h->d_km_fin.p, ccvec, sizeof(double2) * nin, hipMemcpyHostToDevice, h->stream));
+    KmFftArgs fa{};
+    fa.ccvec = h->d_km_fin.as<double2>();
+    fa.imin = imin;
+    fa.imax = imax;
+    fa.np1 = np1;
+    fa.iws = iws;
+    fa.iwe = iwe;
+    fa.omegai = omegai;
+    fa.tlen = tlen;
+    fa.tw2 = h->d_km_tw.as<double2>();
+    fa.rvec = h->d_km_fout.as<double>();
+    TimedLaunch t;
+    ev_begin(h, t, 3);
+    launch_km_fft(fa, nvec, h->stream);
+    HIPCHK(h, hipGetLastError());
+    ev_end(h, t);
+    HIPCHK(h, hipMemcpyAsync(rvec, fa.rvec, sizeof(double) * nout, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    ev_collect(h);
     return DSM_SGT_OK;
 }
 

@@ -48,6 +48,12 @@ class Tuning(C.Structure):
                 "catalog_window": self.catalog_window}
 
 
+class KmParams(C.Structure):
+    """dsm_km_params_t (include/dsm_km.h)."""
+    _fields_ = [("fmin", C.c_int32), ("fmax", C.c_int32), ("compo", C.c_int32), ("reserved", C.c_int32),
+                ("c_red_reci", C.c_double), ("ipdistance", C.c_double), ("mt", C.c_double * 6)]
+
+
 class DsmSgtError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__(f"{ERRORS.get(code, code)}: {msg}")
@@ -62,7 +68,7 @@ EXPORTS = ("dsm_sgt_create", "dsm_sgt_get_info", "dsm_sgt_get_grid", "dsm_sgt_ru
            "dsm_sgt_reset_stats", "dsm_sgt_clear", "dsm_sgt_last_error", "dsm_sgt_destroy", "dsm_sgt_psv_band",
            "dsm_sgt_create_tuned", "dsm_sgt_get_tuning", "dsm_sgt_autotune", "dsm_sgt_run_list",
            "dsm_sgt_write_catalogs_async", "dsm_sgt_copy_catalog_async", "dsm_sgt_wait", "dsm_sgt_host_alloc",
-           "dsm_sgt_host_free")
+           "dsm_sgt_host_free", "dsm_km_points", "dsm_km_vectorfft")
 
 
 def lib():
@@ -89,6 +95,10 @@ def lib():
         L.dsm_sgt_host_alloc.argtypes = [C.c_size_t, C.POINTER(C.c_void_p)]
         L.dsm_sgt_host_free.argtypes = [C.c_void_p]
         L.dsm_sgt_host_free.restype = None
+        L.dsm_km_points.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.POINTER(KmParams), C.c_int32, C.c_void_p,
+                                    C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.dsm_km_vectorfft.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_void_p,
+                                       C.c_double, C.c_double, C.c_int32, C.c_int32, C.c_void_p]
         L.dsm_sgt_fetch.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_void_p]
         L.dsm_sgt_fetch_fp64.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_void_p]
         L.dsm_sgt_lcut.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_int32)]
@@ -230,6 +240,41 @@ class Engine:
 
     def __exit__(self, *a):
         self.close()
+
+
+def km_points(psv, sh, ir, fmin, fmax, compo, c_red_reci, ipdistance, mt, deltar, deltas, trig):
+    """KernelMaker's catalog stage on the GPU (dsm_km_points): h3, h4 as
+    complex128 arrays (npoints, fmax - fmin + 1, 6)."""
+    prm = KmParams(fmin=fmin, fmax=fmax, compo=compo, c_red_reci=c_red_reci, ipdistance=ipdistance)
+    for k in range(6):
+        prm.mt[k] = float(mt[k])
+    dr = np.ascontiguousarray(deltar, dtype=np.float64)
+    ds = np.ascontiguousarray(deltas, dtype=np.float64)
+    tg = np.ascontiguousarray(trig, dtype=np.float64).reshape(-1, 12)
+    n = dr.size
+    h3 = np.zeros((n, fmax - fmin + 1, 6), np.complex128)
+    h4 = np.zeros_like(h3)
+    rc = lib().dsm_km_points(psv._h, sh._h if sh is not None else None, ir, C.byref(prm), n, dr.ctypes.data,
+                             ds.ctypes.data, tg.ctypes.data, h3.ctypes.data, h4.ctypes.data)
+    if rc:
+        raise DsmSgtError(rc, lib().dsm_sgt_last_error(psv._h).decode())
+    return h3, h4
+
+
+def km_vectorfft(eng, spectra, imin, np1, omegai, tlen, iws, iwe):
+    """KernelMaker's frequency-to-time transform on the GPU
+    (dsm_km_vectorfft): spectra complex (nvec, imax - imin + 1) -> real
+    (nvec, iwe - iws + 1)."""
+    sp = np.ascontiguousarray(spectra, dtype=np.complex128)
+    if sp.ndim == 1:
+        sp = sp[None, :]
+    nvec, nk = sp.shape
+    out = np.zeros((nvec, iwe - iws + 1))
+    rc = lib().dsm_km_vectorfft(eng._h, nvec, imin, imin + nk - 1, np1, sp.ctypes.data, omegai, tlen, iws, iwe,
+                                out.ctypes.data)
+    if rc:
+        raise DsmSgtError(rc, lib().dsm_sgt_last_error(eng._h).decode())
+    return out
 
 
 class HostBuffer:

@@ -19,7 +19,7 @@ def _declared():
     for f in os.listdir(inc):
         if f.endswith(".h"):
             txt = open(os.path.join(inc, f)).read()
-            names |= set(re.findall(r"^\s*(?:const\s+)?\w[\w\s\*]*?\b(dsm_sgt_\w+)\s*\(", txt, re.M))
+            names |= set(re.findall(r"^\s*(?:const\s+)?\w[\w\s\*]*?\b(dsm_(?:sgt|km)_\w+)\s*\(", txt, re.M))
     return names
 
 

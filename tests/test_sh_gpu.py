@@ -98,3 +98,20 @@ def test_sh_catalog_files(tmp_path):
         assert os.path.getsize(g) == 8 * 5 * nt
     s = tmp_path / "RSGT" / f"model4.{intir0:07d}.{3:07d}.SYNN_SH"
     assert os.path.getsize(s) == 8 * 5 * nt
+
+
+def test_sh_receivers_at_and_below_the_cmb():
+    """Receivers from 3030 km every 450 km: 3030 km lies in the core and
+    3480 km exactly on the CMB; SGTsh drops both (SGTsh-MPI.f90:152-162:
+    r <= rmin of the remaining solid model), keeps 3930..6180 km, and the
+    catalog matches the oracle's on the trimmed set."""
+    import dsm_sgt
+    p = dict(tlen=3276.8, imax=8, rmin_=3030., rmax_=6371., rdelta_=450., thetamin=0., thetamax=180.,
+             thetadelta=30., maxlmax=600)
+    inp = di.synthetic_input(**p)
+    eng = _engine(inp)
+    info = ol.sh_info(inp)
+    assert eng.info.r_n == info.r_n == 6
+    assert np.allclose(eng.radii, 3930.0 + 450.0 * np.arange(6))
+    eng.run(1, p["imax"])
+    _check(eng, inp, (1, 4, 8))
