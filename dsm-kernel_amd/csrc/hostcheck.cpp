@@ -391,3 +391,50 @@ extern "C" int dsm_hostcheck_psv(const dsm_sgt_input_t *in, int32_t i, double *t
         }
     return 0;
 }
+
+// The depth cut of the batched P-SV forms on the host: the first solved band
+// row (1-based, full system) of frequency i and degree l, by the rule of
+// psv_depth_kernel (psv_kernels.hip) taken row by row -- the decay exponent
+// sum min(sqrt((l + 1/2)^2 / r^2 - omega^2 / v^2) dr, dcap) over the
+// intervals below the source row, the cut where it passes dcut, at least
+// three rows below the source -- with the per-row radius and wave speed of
+// psv_row_profile.  Also the system's first row (row0), the source row
+// (jtmp), the row count nn and, if urow is not null, urow[j - 1] = 1 for the
+// rows holding a radial displacement U (every fluid row; the solid rows at
+// even offsets from their zone's first row kkdr, as the receiver stencils
+// ksta - 1 = kkdr + 2 (iista - 1) read them).
+extern "C" int dsm_hostcheck_psv_depth(const dsm_sgt_input_t *in, int32_t i, int32_t l, double dcut, double dcap,
+                                       int32_t *first_row, int32_t *row0, int32_t *jtmp, int32_t *nn,
+                                       int32_t *urow)
+{
+    PsvSetup s;
+    std::string err;
+    int rc = psv_setup(*in, s, err);
+    if (rc) return rc;
+    PsvOmega o;
+    psv_omega_scalars(s, i, o);
+    std::vector<double> r, v;
+    psv_row_profile(s, r, v);
+    const double al = (double)std::max(1, l) + 0.5, om = o.omega;
+    int kst = s.row0;
+    double sum = 0.0;
+    for (int j = s.jtmp - 1; j >= s.row0; j--) {  // interval between rows j and j + 1 (1-based)
+        const double rm = 0.5 * (r[j - 1] + r[j]), vm = 0.5 * (v[j - 1] + v[j]), dr = r[j] - r[j - 1];
+        const double q = al / rm, p = om / vm, k2 = q * q - p * p;
+        if (k2 > 0.0 && dr > 0.0) sum += std::min(std::sqrt(k2) * dr, dcap);
+        if (sum > dcut) {
+            kst = j;
+            break;
+        }
+    }
+    *first_row = std::min(kst, s.jtmp - 3);
+    if (urow)
+        for (int iz = 0; iz < s.nzone; iz++) {
+            const int j0 = s.kkdr[iz], j1 = (iz + 1 < s.nzone) ? s.kkdr[iz + 1] - 1 : s.nn;
+            for (int j = j0; j <= j1; j++) urow[j - 1] = (s.iphase[iz] != 1 || (j - j0) % 2 == 0) ? 1 : 0;
+        }
+    *row0 = s.row0;
+    *jtmp = s.jtmp;
+    *nn = s.nn;
+    return 0;
+}

@@ -87,6 +87,17 @@ int orc_psv_omega(const orc_input_t *in, int32_t i, int32_t use_ref,
  * (possibly more than cap).  Not thread-safe; one driver call at a time. */
 int orc_calamp_trace(int32_t *l, double *amp, int32_t cap);
 int orc_calamp_count(void);
+/* calcutd of SGTpsv and SGTsh (never called by the reference's drivers),
+ * with the reference's argument lists. */
+void orc_psv_calcutd(const int *nzone, const int *nnlayer, const int *nnl, const double complex *tmpc,
+                     const double *rat, const int *nn, const int *iphase, const double *spo, const int *spn,
+                     const double *ra, const int *kkdr, int *kc);
+void orc_sh_calcutd(const int *nzone, const int *nnl, const double complex *tmpr, const double *rat, const int *nn,
+                    const double *ra, int *kc);
+/* Test hook: copy the P-SV solution g0 (nn values, at most cap) of the
+ * (l, m, imt) right-hand side of the next orc_psv_omega calls into buf;
+ * buf = NULL turns it off. */
+int orc_psv_capture(int32_t l, int32_t m, int32_t imt, double complex *buf, int32_t cap);
 /* (internal) record one call: l and the n complex values calamp reads */
 void orc_trace_calamp(int l, const double complex *g, int n);
 

@@ -1405,6 +1405,100 @@ static void p_callsuf(const double *omega, const int *nzone, const double *vrmax
     *lsuf = (int)(*omega * F1(vrmax, *nzone) / tvs - 0.5) + 1;
 }
 
+/* calcutd (P-SV) others.f90:1421-1499 -- defined but never called by the
+ * reference's SGTpsv.  |U| of every node from the solution tmpc (solid rows
+ * alternate U, V from kkdr; fluid rows are U; a node shared by two zones of
+ * different phase counted once), threshold rat * max, nc the LAST node above
+ * it (the loop's `cycle`), its zone, and kc the band row of node nc. */
+void orc_psv_calcutd(const int *nzone, const int *nnlayer, const int *nnl, const zc *tmpc, const double *rat,
+                     const int *nn, const int *iphase, const double *spo, const int *spn, const double *ra,
+                     const int *kkdr, int *kc)
+{
+    (void)nnlayer;
+    (void)spo;
+    (void)spn;
+    const int n = *nn, nz = *nzone;
+    double *cU = calloc((size_t)n + 1, sizeof(double)), *amp = calloc((size_t)n + 1, sizeof(double));
+    int *ml = calloc((size_t)nz + 1, sizeof(int));
+    int iz = 2, jz = 1;
+    for (int jj = 1; jj <= n; jj++) {
+        if (iz <= nz) {
+            if (jj == kkdr[iz - 1]) {
+                if (iphase[iz - 1] != iphase[iz - 2]) jz = jz - 1;
+                iz = iz + 1;
+            }
+        }
+        if (iphase[iz - 2] == 1) {
+            if ((jj - kkdr[iz - 2]) % 2 == 1) {
+                cU[jz - 1] = cabs(tmpc[jj - 1]);
+                jz = jz + 1;
+            }
+        } else {
+            cU[jz - 1] = cabs(tmpc[jj - 1]);
+            jz = jz + 1;
+        }
+    }
+    double maxamp = -1.0;
+    for (int i = 1; i <= jz - 1; i++) {
+        amp[i - 1] = cU[i - 1];
+        if (maxamp < amp[i - 1]) maxamp = amp[i - 1];
+    }
+    maxamp = maxamp * *rat;
+    int nc = 1;
+    for (int i = 1; i <= jz - 1; i++)
+        if (amp[i - 1] > maxamp) nc = i;
+    int i = 1, tzone = 0;
+    for (int jj = 1; jj <= nz; jj++) {
+        i = i + nnl[jj - 1];
+        ml[jj - 1] = i;
+    }
+    for (int jj = nz; jj >= 1; jj--)
+        if (ml[jj - 1] > nc) tzone = jj;
+    if (tzone == 0) {  // the reference reads an unassigned tzone here: kc unchanged
+        free(cU);
+        free(amp);
+        free(ml);
+        return;
+    }
+    const double rc = ra[nc - 1];
+    for (int ii = 1; ii <= jz - 1; ii++) {
+        if (ra[ii - 1] <= rc && rc < ra[ii]) {
+            nc = ii;
+            if (tzone == 1) {
+                if (iphase[tzone - 1] == 1) *kc = 1 + 2 * nc;
+                if (iphase[tzone - 1] == 2) *kc = 1 + nc;
+            } else {
+                if (iphase[tzone - 1] == 1) *kc = kkdr[tzone - 1] + 2 * (nc - ml[tzone - 2]);
+                if (iphase[tzone - 1] == 2) *kc = kkdr[tzone - 1] + nc - ml[tzone - 2];
+            }
+        }
+    }
+    free(cU);
+    free(amp);
+    free(ml);
+}
+
+/* Test hook: the solution g0 (nn values) of the (l, m, imt) right-hand side
+ * of the next orc_psv_omega calls, copied after its band solve. */
+static int CAP_L = -1, CAP_M = 0, CAP_IMT = 0, CAP_N = 0;
+static zc *CAP_BUF = NULL;
+
+int orc_psv_capture(int32_t l, int32_t m, int32_t imt, double complex *buf, int32_t cap)
+{
+    CAP_L = buf ? l : -1;
+    CAP_M = m;
+    CAP_IMT = imt;
+    CAP_BUF = buf;
+    CAP_N = cap;
+    return 0;
+}
+
+static void psv_capture(int l, int m, int imt, const zc *g0, int nn)
+{
+    if (l != CAP_L || m != CAP_M || imt != CAP_IMT || !CAP_BUF) return;
+    for (int k = 0; k < nn && k < CAP_N; k++) CAP_BUF[k] = g0[k];
+}
+
 /* calamp (P-SV) others.f90:1246-1264 */
 static void p_calamp(const zc *g, const int *l, const int *lsuf, double *maxamp, int *ismall, const double *ratl)
 {

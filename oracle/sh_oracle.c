@@ -785,6 +785,34 @@ void orc_trace_calamp(int l, const double complex *g, int n)
     TR_N++;
 }
 
+/* calcutd (SH) others.f90:928-982 -- defined but never called by the
+ * reference's SGTsh.  The real part of every value (cU = tmpr assigns the
+ * complex value's real part), threshold rat * max, nc the FIRST node above
+ * it (`exit`), kc = nc; kc = 1 when the threshold is zero. */
+void orc_sh_calcutd(const int *nzone, const int *nnl, const zc *tmpr, const double *rat, const int *nn,
+                    const double *ra, int *kc)
+{
+    (void)nnl;
+    (void)ra;
+    (void)nzone;
+    const int n = *nn;
+    double maxamp = -1.0;
+    for (int i = 1; i <= n; i++)
+        if (maxamp < creal(tmpr[i - 1])) maxamp = creal(tmpr[i - 1]);
+    maxamp = maxamp * *rat;
+    if (maxamp == 0.0) {
+        *kc = 1;
+        return;
+    }
+    int nc = 0;
+    for (int i = 1; i <= n; i++)
+        if (creal(tmpr[i - 1]) > maxamp) {
+            nc = i;
+            break;
+        }
+    *kc = nc;
+}
+
 /* calamp  others.f90:805-824 */
 static void o_calamp(const zc *g, const int *l, const int *lsuf, double *maxamp, int *ismall,
                      const double *ratl)
