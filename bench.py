@@ -318,6 +318,60 @@ def profile_traffic(mode, form, plan):
     return None
 
 
+def km_leg(eng, p, npoints=4096, reps=3):
+    """KernelMaker's catalog stage and transform on the resident catalog of
+    the timed engine (P-SV records only): npoints seeded scattering points at
+    the middle receiver radius, frequencies 0..imax, vertical component,
+    window 2 degrees (21 distances), reduction slowness 13.9 s / degree; then
+    the h3 spectra of every point and component to the time domain
+    of 1024 points (np1 = 4096, the reference's FFT length for this catalog).  Device time
+    from the engine's HIP events; the unique HBM bytes of the stage (one
+    radius's catalog read, the reduced catalog written and read once, the
+    outputs written) over it."""
+    import numpy as np
+    import dsm_sgt
+    rng = np.random.default_rng(0)
+    nf = p["imax"] + 1
+    ir = eng.info.r_n // 2
+    dr, ds = rng.uniform(0.0, 180.0, npoints), rng.uniform(0.0, 180.0, npoints)
+    a = rng.uniform(0.0, 2 * np.pi, (npoints, 3))
+    trig = np.stack([f(m * a[:, k]) for k in range(3) for m, f in ((1, np.cos), (1, np.sin), (2, np.cos),
+                                                                    (2, np.sin))], axis=1)
+    mt = np.array([1.0, -0.5, -0.5, 0.3, 0.2, 0.1])
+    args = (eng, None, ir, 0, p["imax"], 0, 13.9, 2.0, mt, dr, ds, trig)
+    dsm_sgt.km_points(*args)  # warm-up
+    wall, dev = [], []
+    for _ in range(reps):
+        eng.reset_stats()
+        t0 = time.perf_counter()
+        h3, h4 = dsm_sgt.km_points(*args)
+        wall.append(time.perf_counter() - t0)
+        dev.append(eng.stats().other_ms)
+    T = eng.info.theta_n
+    byts = (16.0 * nf * T * 30 * (eng.info.r_n > 0)  # catalog of one radius read
+            + 2 * 16.0 * nf * T * 30  # reduced catalog written and read
+            + 2 * 16.0 * npoints * nf * 6)  # outputs
+    dms = min(dev)
+    spec = h3[:1024].transpose(0, 2, 1).reshape(-1, nf)  # 1024 points x 6 components
+    np1 = 4096
+    dsm_sgt.km_vectorfft(eng, spec[:64], 0, np1, np.log(50.0) / p["tlen"], p["tlen"], 0, 2 * np1 - 1)
+    eng.reset_stats()
+    t0 = time.perf_counter()
+    dsm_sgt.km_vectorfft(eng, spec, 0, np1, np.log(50.0) / p["tlen"], p["tlen"], 0, 2 * np1 - 1)
+    fft_wall = time.perf_counter() - t0
+    fft_dev = eng.stats().other_ms
+    nvec = spec.shape[0]
+    return {"points": npoints, "frequencies": nf, "window_distances": 21, "receiver_radius_km": float(eng.radii[ir]),
+            "points_wall_ms": 1e3 * min(wall), "points_device_ms": dms,
+            "point_frequencies_per_s": npoints * nf / (dms * 1e-3),
+            "unique_hbm_bytes": byts, "hbm_gbs": byts / (dms * 1e-3) / 1e9,
+            "fft": {"spectra": nvec, "np1": np1, "wall_ms": 1e3 * fft_wall, "device_ms": fft_dev,
+                    "spectra_per_s": nvec / (fft_dev * 1e-3),
+                    "hbm_gbs": (16.0 * nvec * nf + 8.0 * nvec * 2 * np1) / (fft_dev * 1e-3) / 1e9},
+            "note": "P-SV records only; device time from HIP events on the engine stream; wall time includes "
+                    "the host phase tables and the PCIe copies of points and outputs"}
+
+
 def output_legs(inp, mode, prec, plan, imax, out_dir=None):
     """The catalog out of HBM, timed from engine creation as the timed step
     is: (1) into pinned host memory (dsm_sgt_copy_catalog_async per window
@@ -400,6 +454,7 @@ def main():
     ap.add_argument("--no-output", action="store_true",
                     help="skip the catalog output legs (host memory, files)")
     ap.add_argument("--out-dir", default=None, help="directory for the files leg's scratch catalog")
+    ap.add_argument("--no-km", action="store_true", help="skip the KernelMaker stage leg")
     ap.add_argument("--workload", choices=("catalog", "c5"), default="catalog",
                     help="catalog: the 5 s P-SV (or --mode sh 10 s SH) catalog; c5: 16 source depths at 2.5 s, "
                          "P-SV + SH")
@@ -463,6 +518,7 @@ def main():
     freqs = sorted(set(int(round(x)) for x in [1 + k * (p["imax"] - 1) / 16 for k in range(17)]))
     compare = args.form != "fp64" and not args.no_fp64
     got_all = {i: eng.fetch_all_fp64(i) for i in freqs} if compare else {}
+    km = None if (args.no_km or args.mode != "psv") else km_leg(eng, p)
     eng.close()
 
     # the plain FP64 form once, and the form's error against it
@@ -553,6 +609,8 @@ def main():
         "steps_detail": steps_detail,
         "band_solve": band_solve(args.mode, args.form, st, not args.imax),
     }
+    if km is not None:
+        out["kernelmaker_stage"] = km
     if not args.no_output:
         o = output_legs(inp, mode, prec, plan, p["imax"], args.out_dir)
         o["in_hbm_s"] = ms / 1e3
