@@ -552,7 +552,7 @@ def main():
     rows, p_s = (PSV_ROWS, 144.0) if args.mode == "psv" else (SH_ROWS, 70.0)
     sum_l = st.synth_flops / (4.0 * th_n * rows * r_n)
     alg = 4.0 * sum_l * p_s * r_n * th_n
-    mirror = args.form in ("bf16x6", "bf16x3")
+    mirror = args.form in ("bf16x6", "bf16x3", "fp64")  # the FP32 kernel has no distance symmetry
     useful = st.synth_flops / (2.0 if mirror else 1.0)
     if args.form in ("bf16x6", "bf16x3"):
         kernel, peak = "synth_b16_kernel<%d, 0>" % (3 if args.form == "bf16x6" else 2), BF16_PEAK

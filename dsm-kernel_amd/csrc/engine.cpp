@@ -356,14 +356,14 @@ static int split_rows(dsm_sgt_t *h, const DevBuf &E, int nslot, int nq, int r_ef
     return DSM_SGT_OK;
 }
 
-// Distance symmetry of the BF16 forms: the grid is symmetric about 90
-// degrees and every job's functions share one parity under
-// theta -> pi - theta.  par[b]: parity of table plane b without (-1)^l.
+// Distance symmetry of the BF16 and FP64 synthesis kernels: the grid is
+// symmetric about 90 degrees and every job's functions share one parity
+// under theta -> pi - theta.  par[b]: parity of table plane b without (-1)^l.
 static void set_mirror(dsm_sgt_t *h, const int *par)
 {
     h->mirror = 0;
     const int n = h->theta_n;
-    bool sym = h->ns > 0 && n >= 2;
+    bool sym = (h->ns > 0 || h->precision == DSM_SGT_FP64) && n >= 2;
     for (int i = 0; sym && i < n; i++)
         if (std::fabs(h->theta_deg[i] + h->theta_deg[n - 1 - i] - 180.0) > 1e-9) sym = false;
     for (auto &j : h->jobs) {
@@ -522,9 +522,10 @@ static void sh_synth_jobs(dsm_sgt_t *h)
         j.ncomp = ncomp[d.kind];
         j.out = (double2 *)outs[d.kind];
         // split point: the first plane whose parity differs from the first's
-        // (the planes of one parity are contiguous in SH_JOBS)
+        // (the planes of one parity are contiguous in SH_JOBS); the FP32
+        // kernel has no distance symmetry
         int cut = d.nb;
-        if (h->ns)
+        if (h->precision != DSM_SGT_FP32)
             for (int t = 1; t < d.nb; t++)
                 if (fpar[d.b[t]] != fpar[d.b[0]]) {
                     cut = t;
@@ -946,6 +947,8 @@ static int sh_run_batch(dsm_sgt_t *h, const ShPrep &p)
     ga.slot_out = h->d_slotout.as<int>();
     ga.jobs = h->d_jobs.as<SynthJob>();
     ga.njobs = h->njobs_main;
+    ga.mirror = h->mirror;
+    ga.nh = h->nh;
     TimedLaunch tg;
     ev_begin(h, tg, 2);
     launch_synth(h, ga, h->d_eb, h->d_nlr.as<int>());
@@ -1697,6 +1700,8 @@ static int psv_run_batch(dsm_sgt_t *h, const PsvPrep &p, bool full_range)
     ga.slot_out = h->d_slotout.as<int>();
     ga.jobs = h->d_jobs.as<SynthJob>();
     ga.njobs = h->njobs_main;
+    ga.mirror = h->mirror;
+    ga.nh = h->nh;
     TimedLaunch tg;
     if (gmode) {
         rc = begin_capture(h, cg);

@@ -76,6 +76,7 @@ struct SynthArgs {
     const int *slot_out;   // [slot] -> catalog frequency position
     const SynthJob *jobs;  // device array
     int njobs;
+    int mirror, nh;        // FP64: distance symmetry, nh = (theta_n + 1) / 2 columns computed
 };
 void launch_synth_f64(const SynthArgs &a, hipStream_t s);
 // Mixed form (synth32.hip): FP32 operands on FP32 MFMA, FP64 accumulation

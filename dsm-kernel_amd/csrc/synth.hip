@@ -15,6 +15,13 @@
 // register-staged double buffer: the global loads of chunk t+1 are in
 // flight while the MFMAs of chunk t run.  128x128 tiles keep the operand
 // stream at ~5 TB/s chip-wide at the FP64 MFMA rate (64x64 tiles needed ~10).
+//
+// Distance symmetry (MIR = 1, a grid symmetric about 90 degrees): every
+// job's theta functions satisfy F(l, pi - theta) = msign (-1)^l F(l, theta),
+// so the tiles cover the distances up to 90 degrees only, each K chunk is
+// staged with its 8 even degrees first and its 8 odd ones last, the waves
+// keep the even and the odd sums apart, and the epilogue writes distance c
+// as even + odd and distance theta_n - 1 - c as msign (even - odd).
 #include "dev_common.h"
 #include "kernels.h"
 
@@ -54,19 +61,24 @@ __device__ __forceinline__ void load_tile(const SynthArgs &a, const SynthJob &jo
     }
 }
 
+// K slot of degree l0 + kk in LDS: in order, or (MIR) the even degrees of the
+// chunk first (l0 is a multiple of BK, so kk has the parity of l)
+template <int MIR> __device__ __forceinline__ int kslot(int kk) { return MIR ? ((kk & 1) ? 8 + (kk >> 1) : (kk >> 1)) : kk; }
+
+template <int MIR>
 __device__ __forceinline__ void store_tile(double (*As)[BM], double (*Bs)[BN], int tid, const TileLoad &t)
 {
 #pragma unroll
     for (int u = 0; u < 2; u++) {
         const int idx = tid + NT * u;
-        const int pr = idx >> 4, kk = idx & 15;
+        const int pr = idx >> 4, kk = kslot<MIR>(idx & 15);
         As[kk][2 * pr] = t.a[u].x;
         As[kk][2 * pr + 1] = t.a[u].y;
     }
 #pragma unroll
     for (int u = 0; u < 2; u++) {
         const int idx = (tid + NT * u) * 2;
-        const int kk = idx >> 7, c = idx & 127;
+        const int kk = kslot<MIR>(idx >> 7), c = idx & 127;
         Bs[kk][c] = t.b[u].x;
         Bs[kk][c + 1] = t.b[u].y;
     }
@@ -91,7 +103,7 @@ __device__ __forceinline__ bool xcd_tile(int nx, int ny, int &tx, int &ty, int &
     return true;
 }
 
-template <int KIND>  // 0: receiver rows, 1: surface rows (separate symbol for profiling)
+template <int KIND, int MIR>  // KIND 0: receiver rows, 1: surface rows (separate symbol for profiling)
 __global__ __launch_bounds__(512) void synth_f64_kernel(SynthArgs a, int nx, int ny)
 {
     __shared__ double As[2][BK][BM];
@@ -114,16 +126,19 @@ __global__ __launch_bounds__(512) void synth_f64_kernel(SynthArgs a, int nx, int
     const size_t plane = (size_t)a.lcap * a.theta_pad;
     const int wr = wave >> 2, wc = wave & 3;
 
-    v4d acc[4][2];
+    // acc[0]: all degrees (MIR = 0) or the even ones; acc[1]: the odd ones
+    v4d acc[1 + MIR][4][2];
 #pragma unroll
-    for (int i = 0; i < 4; i++)
+    for (int p = 0; p <= MIR; p++)
 #pragma unroll
-        for (int j = 0; j < 2; j++) acc[i][j] = (v4d){0.0, 0.0, 0.0, 0.0};
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+            for (int j = 0; j < 2; j++) acc[p][i][j] = (v4d){0.0, 0.0, 0.0, 0.0};
 
     TileLoad tl;
     if (niter > 0) {
         load_tile(a, job, plane, m0, n0, M, R, 0, nk, tid, tl);
-        store_tile(As[0], Bs[0], tid, tl);
+        store_tile<MIR>(As[0], Bs[0], tid, tl);
     }
     __syncthreads();
     for (int it = 0; it < niter; it++) {
@@ -131,6 +146,7 @@ __global__ __launch_bounds__(512) void synth_f64_kernel(SynthArgs a, int nx, int
         if (it + 1 < niter) load_tile(a, job, plane, m0, n0, M, R, it + 1, nk, tid, tl);
 #pragma unroll
         for (int k4 = 0; k4 < BK; k4 += 4) {
+            const int p = (MIR && k4 >= 8) ? 1 : 0;
             const int kk = k4 + (lane >> 4);
             double av[4], bv[2];
 #pragma unroll
@@ -141,9 +157,9 @@ __global__ __launch_bounds__(512) void synth_f64_kernel(SynthArgs a, int nx, int
             for (int i = 0; i < 4; i++)
 #pragma unroll
                 for (int j = 0; j < 2; j++)
-                    acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[i], bv[j], acc[i][j], 0, 0, 0);
+                    acc[p][i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[i], bv[j], acc[p][i][j], 0, 0, 0);
         }
-        if (it + 1 < niter) store_tile(As[st ^ 1], Bs[st ^ 1], tid, tl);
+        if (it + 1 < niter) store_tile<MIR>(As[st ^ 1], Bs[st ^ 1], tid, tl);
         __syncthreads();
     }
     // epilogue: row = (slot, r, re|im) -> out[pos][comp][r][theta].
@@ -162,11 +178,24 @@ __global__ __launch_bounds__(512) void synth_f64_kernel(SynthArgs a, int nx, int
 #pragma unroll
             for (int j = 0; j < 2; j++) {
                 const int col = n0 + wc * 32 + j * 16 + (lane & 15);
-                if (col >= a.theta_n) continue;
+                if (col >= (MIR ? a.nh : a.theta_n)) continue;
+                const int mcol = a.theta_n - 1 - col;  // (MIR) the mirrored distance
                 for (int t = 0; t < ntgt; t++) {
-                    double *o = out + 2 * (((pos * job.ncomp + job.comp[t]) * R + r) * a.theta_n + col) + reim;
-                    const double v = job.sign[t] * acc[i][j][q];
-                    *o = job.accumulate ? *o + v : v;
+                    const size_t base = ((pos * job.ncomp + job.comp[t]) * R + r) * a.theta_n;
+                    double *o = out + 2 * (base + col) + reim;
+                    if (!MIR) {
+                        const double v = job.sign[t] * acc[0][i][j][q];
+                        *o = job.accumulate ? *o + v : v;
+                    } else {
+                        const double e = acc[0][i][j][q], od = acc[MIR][i][j][q];
+                        const double v = job.sign[t] * (e + od);
+                        *o = job.accumulate ? *o + v : v;
+                        if (mcol != col) {
+                            double *om = out + 2 * (base + mcol) + reim;
+                            const double w = job.sign[t] * (job.msign * (e - od));
+                            *om = job.accumulate ? *om + w : w;
+                        }
+                    }
                 }
             }
         }
@@ -175,10 +204,15 @@ __global__ __launch_bounds__(512) void synth_f64_kernel(SynthArgs a, int nx, int
 void launch_synth_f64(const SynthArgs &a, hipStream_t s)
 {
     const int M = a.nslot * a.r_eff * 2;
-    const int nx = a.theta_pad / BN, ny = (M + BM - 1) / BM;
+    const int nx = a.mirror ? (a.nh + BN - 1) / BN : a.theta_pad / BN, ny = (M + BM - 1) / BM;
     const dim3 grid(8 * ((nx * ny + 7) / 8) * a.njobs);
-    if (a.r_eff > 1) hipLaunchKernelGGL(synth_f64_kernel<0>, grid, dim3(NT), 0, s, a, nx, ny);
-    else hipLaunchKernelGGL(synth_f64_kernel<1>, grid, dim3(NT), 0, s, a, nx, ny);
+    if (a.mirror) {
+        if (a.r_eff > 1) hipLaunchKernelGGL((synth_f64_kernel<0, 1>), grid, dim3(NT), 0, s, a, nx, ny);
+        else hipLaunchKernelGGL((synth_f64_kernel<1, 1>), grid, dim3(NT), 0, s, a, nx, ny);
+    } else {
+        if (a.r_eff > 1) hipLaunchKernelGGL((synth_f64_kernel<0, 0>), grid, dim3(NT), 0, s, a, nx, ny);
+        else hipLaunchKernelGGL((synth_f64_kernel<1, 0>), grid, dim3(NT), 0, s, a, nx, ny);
+    }
 }
 
 }  // namespace dsm
