@@ -246,6 +246,7 @@ struct dsm_sgt {
     DevBuf d_km_omt, d_km_omr, d_km_ph, d_km_pos, d_km_thd, d_km_pts, d_km_php, d_km_out;
     DevBuf d_km_fin, d_km_fout, d_km_tw;  // dsm_km_vectorfft: spectra, windows, twiddles
     int km_tw_n = 0;
+    double2 *km_static_tw = nullptr;  // the device's static twiddle storage (dsm_km_vectorfft_noalloc)
     // timing
     bool timing = true;
     std::vector<TimedLaunch> pending;
@@ -1833,6 +1834,10 @@ int dsm_sgt_create_tuned(const dsm_sgt_input_t *in, int32_t mode, int32_t precis
             rc = fail(h, DSM_SGT_EHIP, "hipStreamCreate failed");
             break;
         }
+        if (const hipError_t e = km_static_twiddles(device, &h->km_static_tw); e != hipSuccess) {
+            rc = fail(h, DSM_SGT_EHIP, std::string("static twiddle storage: ") + hipGetErrorString(e));
+            break;
+        }
         rc = (mode == DSM_SGT_SH) ? sh_create(h) : psv_create(h);
         // the cache gives its blocks back when the engine leaves less than
         // the reserve free (kernel scratch is allocated at launch)
@@ -2658,13 +2663,9 @@ int dsm_km_vectorfft(dsm_sgt_t *h, int32_t nvec, int32_t imin, int32_t imax, int
     HIPCHK(h, h->d_km_fin.ensure(sizeof(double2) * nin));
     HIPCHK(h, h->d_km_fout.ensure(sizeof(double) * nout));
     if (h->km_tw_n != N) {  // e^{2 pi i k / N}, k < N / 2
-        std::vector<double2> tw(N / 2);
-        for (int k = 0; k < N / 2; k++) {
-            const double a = 2.0 * 3.141592653589793 * (double)k / (double)N;
-            tw[k] = make_double2(std::cos(a), std::sin(a));
-        }
-        HIPCHK(h, h->d_km_tw.ensure(sizeof(double2) * tw.size()));
-        HIPCHK(h, hipMemcpy(h->d_km_tw.p, tw.data(), sizeof(double2) * tw.size(), hipMemcpyHostToDevice));
+        HIPCHK(h, h->d_km_tw.ensure(sizeof(double2) * (N / 2)));
+        launch_km_twiddles(h->d_km_tw.as<double2>(), N, h->stream);
+        HIPCHK(h, hipGetLastError());
         h->km_tw_n = N;
     }
     HIPCHK(h, hipMemcpyAsync(h->d_km_fin.p, ccvec, sizeof(double2) * nin, hipMemcpyHostToDevice, h->stream));
@@ -2687,6 +2688,73 @@ int dsm_km_vectorfft(dsm_sgt_t *h, int32_t nvec, int32_t imin, int32_t imax, int
     HIPCHK(h, hipMemcpyAsync(rvec, fa.rvec, sizeof(double) * nout, hipMemcpyDeviceToHost, h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));
     ev_collect(h);
+    return DSM_SGT_OK;
+}
+
+// A device array of `bytes` bytes at p on `device`: within one allocation
+// of that device (not host memory, not running past the allocation's end).
+static bool device_span(const void *p, size_t bytes, int device, std::string &why)
+{
+    hipPointerAttribute_t at{};
+    if (hipPointerGetAttributes(&at, p) != hipSuccess || at.type != hipMemoryTypeDevice || at.device != device) {
+        (void)hipGetLastError();
+        why = "is not device memory of the engine's GPU";
+        return false;
+    }
+    hipDeviceptr_t base = nullptr;
+    size_t size = 0;
+    if (hipMemGetAddressRange(&base, &size, (hipDeviceptr_t)p) != hipSuccess) {
+        (void)hipGetLastError();
+        why = "is not device memory of the engine's GPU";
+        return false;
+    }
+    const uintptr_t b = (uintptr_t)base, q = (uintptr_t)p;
+    if (q < b || q - b > size || bytes > size - (q - b)) {
+        why = "extends past the end of its device allocation";
+        return false;
+    }
+    return true;
+}
+
+int dsm_km_vectorfft_noalloc(dsm_sgt_t *h, void *stream, int32_t nvec, int32_t imin, int32_t imax, int32_t np1,
+                             const void *ccvec, double omegai, double tlen, int32_t iws, int32_t iwe, double *rvec)
+{
+    if (!h) return DSM_SGT_EINVAL;
+    if (!h->ready) return fail(h, DSM_SGT_ESTATE, "engine not initialised");
+    if (np1 > KM_FFT_MAX_NP1)
+        return fail(h, DSM_SGT_EINVAL, "np1 above " + std::to_string(KM_FFT_MAX_NP1) +
+                                           ", the size of the static twiddle storage");
+    if (np1 < 2 || (np1 & (np1 - 1))) return fail(h, DSM_SGT_EINVAL, "np1 must be a power of two");
+    const int N = 2 * np1;
+    if (imin < 0 || imax < imin || imax >= N || iws < 0 || iwe < iws || iwe >= N)
+        return fail(h, DSM_SGT_EINVAL, "spectrum or window outside 0..2 np1 - 1");
+    if (nvec < 0) return fail(h, DSM_SGT_EINVAL, "bad arrays");
+    if (!(tlen > 0.0)) return fail(h, DSM_SGT_EINVAL, "tlen must be positive");
+    if (nvec == 0) return DSM_SGT_OK;
+    if (!h->km_static_tw) return fail(h, DSM_SGT_ESTATE, "no static twiddle storage on the engine's device");
+    if (hipSetDevice(h->device) != hipSuccess) return fail(h, DSM_SGT_EHIP, "hipSetDevice failed");
+    const size_t nin = (size_t)nvec * (imax - imin + 1), nout = (size_t)nvec * (iwe - iws + 1);
+    std::string why;
+    if (!ccvec || !device_span(ccvec, sizeof(double2) * nin, h->device, why))
+        return fail(h, DSM_SGT_EINVAL, "ccvec " + (ccvec ? why : std::string("is NULL")));
+    if (!rvec || !device_span(rvec, sizeof(double) * nout, h->device, why))
+        return fail(h, DSM_SGT_EINVAL, "rvec " + (rvec ? why : std::string("is NULL")));
+    hipStream_t s = stream ? (hipStream_t)stream : h->stream;
+    launch_km_twiddles(h->km_static_tw, N, s);
+    KmFftArgs fa{};
+    fa.ccvec = (const double2 *)ccvec;
+    fa.imin = imin;
+    fa.imax = imax;
+    fa.np1 = np1;
+    fa.iws = iws;
+    fa.iwe = iwe;
+    fa.omegai = omegai;
+    fa.tlen = tlen;
+    fa.tw2 = h->km_static_tw;
+    fa.rvec = rvec;
+    launch_km_fft(fa, nvec, s);
+    HIPCHK(h, hipGetLastError());
+    if (!stream) HIPCHK(h, hipStreamSynchronize(h->stream));
     return DSM_SGT_OK;
 }
 

@@ -46,5 +46,10 @@ struct KmFftArgs {
     double *rvec;
 };
 void launch_km_fft(const KmFftArgs &a, int nvec, hipStream_t s);
+// tw[k] = e^{2 pi i k / N}, k < N / 2 (the table KmFftArgs::tw2 reads)
+void launch_km_twiddles(double2 *tw, int N, hipStream_t s);
+// Device address of the static twiddle storage (KM_FFT_MAX_NP1 values) of
+// `device`, which must be current; looked up once per device.
+hipError_t km_static_twiddles(int device, double2 **tw);
 
 }  // namespace dsm

@@ -13,6 +13,9 @@
 // tables built on the host.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+#include <vector>
+
 #include "km_stage.h"
 
 namespace dsm {
@@ -299,6 +302,46 @@ void launch_km_fft(const KmFftArgs &a, int nvec, hipStream_t s)
     const int lds = (int)sizeof(double2) * a.np1;  // up to 128 KiB of the CU's 160
     (void)hipFuncSetAttribute((const void *)km_fft_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
     hipLaunchKernelGGL(km_fft_kernel, dim3(nvec), dim3(256), lds, s, a);
+}
+
+// Twiddle table of the transform: tw[k] = e^{2 pi i k / N}, k < N / 2.
+__global__ __launch_bounds__(256) void km_twiddle_kernel(double2 *tw, int N)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= N / 2) return;
+    double sn, cs;
+    sincos(2.0 * 3.141592653589793 * (double)k / (double)N, &sn, &cs);
+    tw[k] = make_double2(cs, sn);
+}
+
+void launch_km_twiddles(double2 *tw, int N, hipStream_t s)
+{
+    if (N < 2) return;
+    hipLaunchKernelGGL(km_twiddle_kernel, dim3((N / 2 + 255) / 256), dim3(256), 0, s, tw, N);
+}
+
+// Twiddle storage of the allocation-free transform: KM_FFT_MAX_NP1 values
+// (128 KiB) per device, held until the process ends.  Host code reaches it
+// only through the address km_static_twiddles returns for the device.
+static __device__ __align__(256) double2 scratch[KM_FFT_MAX_NP1];
+
+hipError_t km_static_twiddles(int device, double2 **tw)
+{
+    static std::mutex mu;
+    static std::vector<double2 *> addr;  // per device
+    *tw = nullptr;
+    int cur = -1;
+    if (hipGetDevice(&cur) != hipSuccess || cur != device) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(mu);
+    if ((int)addr.size() <= device) addr.resize(device + 1, nullptr);
+    if (!addr[device]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        addr[device] = (double2 *)p;
+    }
+    *tw = addr[device];
+    return hipSuccess;
 }
 
 void launch_km_rd(const KmRdArgs &a, hipStream_t s)

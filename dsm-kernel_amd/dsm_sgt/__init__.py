@@ -68,7 +68,7 @@ EXPORTS = ("dsm_sgt_create", "dsm_sgt_get_info", "dsm_sgt_get_grid", "dsm_sgt_ru
            "dsm_sgt_reset_stats", "dsm_sgt_clear", "dsm_sgt_last_error", "dsm_sgt_destroy", "dsm_sgt_psv_band",
            "dsm_sgt_create_tuned", "dsm_sgt_get_tuning", "dsm_sgt_autotune", "dsm_sgt_run_list",
            "dsm_sgt_write_catalogs_async", "dsm_sgt_copy_catalog_async", "dsm_sgt_wait", "dsm_sgt_host_alloc",
-           "dsm_sgt_host_free", "dsm_km_points", "dsm_km_vectorfft")
+           "dsm_sgt_host_free", "dsm_km_points", "dsm_km_vectorfft", "dsm_km_vectorfft_noalloc")
 
 
 def lib():
@@ -99,6 +99,8 @@ def lib():
                                     C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         L.dsm_km_vectorfft.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_void_p,
                                        C.c_double, C.c_double, C.c_int32, C.c_int32, C.c_void_p]
+        L.dsm_km_vectorfft_noalloc.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
+                                               C.c_void_p, C.c_double, C.c_double, C.c_int32, C.c_int32, C.c_void_p]
         L.dsm_sgt_fetch.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_void_p]
         L.dsm_sgt_fetch_fp64.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_void_p]
         L.dsm_sgt_lcut.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_int32)]

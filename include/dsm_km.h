@@ -65,6 +65,22 @@ int dsm_km_points(dsm_sgt_t *psv, dsm_sgt_t *sh, int32_t ir, const dsm_km_params
 int dsm_km_vectorfft(dsm_sgt_t *h, int32_t nvec, int32_t imin, int32_t imax, int32_t np1, const void *ccvec,
                      double omegai, double tlen, int32_t iws, int32_t iwe, double *rvec);
 
+/* dsm_km_vectorfft on device arrays, without allocating: no counterpart in
+ * the reference.  For callers that cannot allocate device memory (code under
+ * HIP graph capture, serving loops, harnesses that forbid hipMalloc).
+ * ccvec and rvec as in dsm_km_vectorfft, but device memory of the engine's
+ * GPU (each checked to lie within one device allocation); stream: a HIP
+ * stream of that GPU, on which the call returns at once, or NULL for the
+ * engine's stream, on which it returns when the transform is done.  The
+ * twiddle table lives in static device storage of 8192
+ * complex(8) values (128 KiB, held by every GPU the library runs on until
+ * the process ends): np1 above 8192 is refused.  Every call rewrites that
+ * GPU's table, so calls that overlap on one GPU (other streams, other
+ * threads) race; no call may run while another on the same GPU is in
+ * flight. */
+int dsm_km_vectorfft_noalloc(dsm_sgt_t *h, void *stream, int32_t nvec, int32_t imin, int32_t imax, int32_t np1,
+                             const void *ccvec, double omegai, double tlen, int32_t iws, int32_t iwe, double *rvec);
+
 #ifdef __cplusplus
 }
 #endif
