@@ -548,7 +548,8 @@ def main():
     #          theta) with 63 (P-SV) / 36 (SH) rows per receiver, halved where the
     #          distance symmetry computes theta and pi - theta together;
     #  issued: the MFMA flops the bf16 kernels issue (all split cross products
-    #          over the tiles' degree extents).
+    #          over the tiles' degree extents); FP64: the useful count over the
+    #          distance columns the tiles cover.
     rows, p_s = (PSV_ROWS, 144.0) if args.mode == "psv" else (SH_ROWS, 70.0)
     sum_l = st.synth_flops / (4.0 * th_n * rows * r_n)
     alg = 4.0 * sum_l * p_s * r_n * th_n
@@ -560,7 +561,9 @@ def main():
     elif args.form == "fp32":
         kernel, peak, issued = "synth_f32_kernel<0>", FP32_MFMA_PEAK, st.synth_flops
     else:
-        kernel, peak, issued = "synth_f64_kernel<0>", FP64_MFMA_PEAK, st.synth_flops
+        # the distance tiles (128 wide) cover the first (theta_n + 1) / 2 distances
+        cols = 128 * -(-((th_n + 1) // 2) // 128)
+        kernel, peak, issued = "synth_f64_kernel<0, 1>", FP64_MFMA_PEAK, st.synth_flops * cols / th_n
     flops = alg / launches
     achieved = alg / (st.synth_ms * 1e-3) if st.synth_ms > 0 else 0.0
     issued_rate = issued / (st.synth_ms * 1e-3) if st.synth_ms > 0 else 0.0
@@ -607,6 +610,7 @@ def main():
         "stages_ms": {"solve+coef": st.solve_ms, "legendre": st.legendre_ms, "synth": st.synth_ms,
                       "other": st.other_ms},
         "steps_detail": steps_detail,
+        "plan": eng_plan,
         "band_solve": band_solve(args.mode, args.form, st, not args.imax),
     }
     if km is not None:
