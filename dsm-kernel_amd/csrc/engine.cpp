@@ -419,6 +419,8 @@ static int b16_tile_order(dsm_sgt_t *h, const std::vector<int> &nlr, int S, int 
     // hold -1 (no tile).
     constexpr int NXCD = 8;
     const int XG = h->xg > 0 ? h->xg : std::max(1, 32 / nx);
+    // column tiles of a group in full-tile work (the last one may be narrow)
+    const double wx = nx - (h->nh - (nx - 1) * 128 <= SYNTH_B16_NARROW ? 0.75 : 0.0);
     auto order = [&](int j0, int nj, std::vector<int> &out) {
         std::vector<int> groups;
         std::vector<double> w(nj * ny);
@@ -436,7 +438,7 @@ static int b16_tile_order(dsm_sgt_t *h, const std::vector<int> &nlr, int S, int 
             for (int y = 1; y < NXCD; y++)
                 if (load[y] < load[x]) x = y;
             for (size_t g = c; g < std::min(groups.size(), c + XG); g++) {
-                load[x] += w[groups[g]] * nx;
+                load[x] += w[groups[g]] * wx;
                 for (int tx = 0; tx < nx; tx++) q[x].push_back(groups[g] * nx + tx);
             }
         }
@@ -460,11 +462,13 @@ static int b16_tile_order(dsm_sgt_t *h, const std::vector<int> &nlr, int S, int 
 
 // MFMA flops the bf16 synthesis issued for the batch, from the rows' l
 // extents the kernels used (nlr, device values): per tile and K step of 32
-// degrees 2 x 128 x 128 x 32 per cross product.
+// degrees 2 x 128 x 128 x 32 per cross product (2 x 128 x 32 x 32 for a
+// narrow last distance tile).
 static double b16_issued_flops(const dsm_sgt_t *h, const std::vector<int> &nlr, int S, int R)
 {
     int nx, ny;
     synth_b16_tiles(S, R, h->nhp, nx, ny);
+    const double wx = nx - (h->nh - (nx - 1) * 128 <= SYNTH_B16_NARROW ? 0.75 : 0.0);
     const int M = S * R * 2;
     const double per = 2.0 * 128 * 128 * 32 * (h->ns == 3 ? 6 : 3);
     double fl = 0.0;
@@ -472,7 +476,7 @@ static double b16_issued_flops(const dsm_sgt_t *h, const std::vector<int> &nlr, 
         int kt = 0;
         for (int pr = (ty * 128) >> 1; pr <= (std::min(M - 1, ty * 128 + 127) >> 1); pr++) kt = std::max(kt, nlr[pr]);
         const int ce = ((kt + 1) / 2 + 31) / 32, co = (kt / 2 + 31) / 32;
-        for (int j = 0; j < h->njobs_main + h->njobs_acc; j++) fl += per * nx * h->jobs[j].nb * (ce + co);
+        for (int j = 0; j < h->njobs_main + h->njobs_acc; j++) fl += per * wx * h->jobs[j].nb * (ce + co);
     }
     return fl;
 }

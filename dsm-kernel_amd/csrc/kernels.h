@@ -112,6 +112,9 @@ void launch_zero_pad_b16(unsigned short *Eb, size_t nrows, int lcap, int lfrom, 
 void launch_split_table(const double *tab, int nplanes, int lcap, int theta_pad, int nh, int nhp, int ns,
                         unsigned short *Tb, hipStream_t s);
 size_t synth_b16_grid(int nslot, int r_eff, int nhp, int njobs);
+// a distance tile with at most this many columns below nh (the last one)
+// runs a quarter of the MFMAs of a full tile
+constexpr int SYNTH_B16_NARROW = 32;
 // tile geometry of launch_synth_b16: nx distance tiles, ny row tiles
 void synth_b16_tiles(int nslot, int r_eff, int nhp, int &nx, int &ny);
 void launch_synth_b16(const SynthB16Args &a, int ns, hipStream_t s);

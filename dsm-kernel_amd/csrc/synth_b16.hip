@@ -189,58 +189,52 @@ __global__ void split_table_kernel(const double *tab, int nplanes, int lcap, int
 // the table columns), group g by wave g % 8, 1 KiB per group.  loff[u]:
 // the lane's row offset (degree segment included) in group wave + 8 u;
 // ea, tb: the job's coefficient-row and table-plane bases at degree lp0.
+// with_b false: the wave loads its A groups only (NS loads instead of 2 NS).
 // Nothing here may wait on a vector load: the loads of the later stages
 // are in flight.
 template <int NS>
 __device__ __forceinline__ void issue_stage(const u16 *ea, const u16 *tb, size_t a_split, size_t t_split,
-                                            const unsigned *loff, int wave, u16 *dst)
+                                            const unsigned *loff, int wave, bool with_b, u16 *dst)
 {
 #pragma unroll
     for (int u = 0; u < NS * 2; u++) {
         const int g = wave + 8 * u;
+        if (g >= NS * 8 && !with_b) continue;
         const int s = (g >> 3) % NS;
         const u16 *src = (g < NS * 8) ? ea + s * a_split : tb + s * t_split;
         __builtin_amdgcn_global_load_lds((const void *)(src + loff[u]), LDS_PTR(dst + g * 512), 16, 0, 0);
     }
 }
 
-template <int NS, int KIND>  // KIND 0: receiver rows, 1: surface rows (separate symbol for profiling)
-__global__ __launch_bounds__(512) void synth_b16_kernel(SynthB16Args a, int nx, int ny)
+// One 128 x 128 tile (rows m0.., distance columns n0..) of job `job`.
+// Wave tiles: 64 x 32 (4 x 2 blocks of 16 x 16; waves in 2 x 4), or, for
+// the last distance tile when it holds at most 32 columns (NARROW), 16 x 32
+// (1 x 2 blocks; waves stacked along the rows): the tile's MFMAs cover the
+// 32 columns only, and only waves 0 and 1 load table rows (the 32 columns).
+template <int NS, bool NARROW>
+__device__ __forceinline__ void synth_b16_tile(const SynthB16Args &a, const SynthJob *job, int tile, int m0, int n0,
+                                               u16 *lds)
 {
     constexpr int STAGE = NS * (BM + BN) * BK;  // u16
-    __shared__ __attribute__((aligned(1024))) u16 lds[NSTAGE * STAGE];
-    int tx, ty, tz, tile = blockIdx.x;
-    if (a.tiles) {
-        // host order: groups of tiles that share operands on one XCD,
-        // longest l series first; -1 marks an empty slot
-        if ((int)blockIdx.x >= a.ntiles) return;
-        const int t = a.tiles[blockIdx.x];
-        if (t < 0) return;
-        tile = t;
-        tx = t % nx;
-        ty = (t / nx) % ny;
-        tz = t / (nx * ny);
-    } else if (!xcd_tile_b16(nx, ny, tx, ty, tz)) {
-        return;
-    }
-    const SynthJob *job = a.jobs + tz;
+    constexpr int WI = NARROW ? 1 : 4;          // 16-row blocks per wave
     const int nb = job->nb;
     const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int R = a.r_eff;
     const int M = a.nslot * R * 2;
-    const int m0 = ty * BM, n0 = tx * BN;
     int kmax = 0;  // longest l series among the tile's (receiver, slot) rows
     for (int pr = m0 >> 1; pr <= (min(M - 1, m0 + BM - 1) >> 1); pr++) kmax = max(kmax, a.nlr[pr]);
     const int ce = ((kmax + 1) / 2 + BK - 1) / BK;  // chunks of even degrees l < kmax
     const int co = (kmax / 2 + BK - 1) / BK;        // odd
     const int LH = a.lcap >> 1;
     const int nit_e = nb * ce, nit = nit_e + nb * co;
-    const int wr = wave >> 2, wc = wave & 3;
+    const int rbase = NARROW ? wave * 16 : (wave >> 2) * 64;  // the wave's first row, column in the tile
+    const int cbase = NARROW ? 0 : (wave & 3) * 32;
+    const bool with_b = !NARROW || wave < 2;
     const int fr = lane & 15;
 
-    v4d acc64[4][2];
+    v4d acc64[WI][2];
 #pragma unroll
-    for (int i = 0; i < 4; i++)
+    for (int i = 0; i < WI; i++)
 #pragma unroll
         for (int j = 0; j < 2; j++) acc64[i][j] = (v4d){0.0, 0.0, 0.0, 0.0};
     // stash of tile t (< nx ny njobs, within synth_b16_grid)
@@ -279,37 +273,39 @@ __global__ __launch_bounds__(512) void synth_b16_kernel(SynthB16Args a, int nx, 
         const int plane = jb == 0 ? pl0 : jb == 1 ? pl1 : jb == 2 ? pl2 : pl3;
         const u16 *ea = a.Eb + (size_t)(q0 + jb) * NS * a_split + lp0;
         const u16 *tb = a.Tb + (size_t)plane * a.nhp * a.lcap + lp0;
-        issue_stage<NS>(ea, tb, a_split, t_split, loff, wave, lds + (it2 % NSTAGE) * STAGE);
+        issue_stage<NS>(ea, tb, a_split, t_split, loff, wave, with_b, lds + (it2 % NSTAGE) * STAGE);
     };
     if (nit > 0) issue(0);
     if (nit > 1) issue(1);
     for (int it = 0; it < nit; it++) {
-        // this stage's loads landed (each wave its own 2 NS), and every wave's
-        // fragment reads of the stage refilled next are done
-        if (it + 1 < nit) wait_vm_lgkm0<2 * NS>();
-        else wait_vm_lgkm0<0>();
+        // this stage's loads landed (each wave its own 2 NS, or NS without
+        // table rows), and every wave's fragment reads of the stage refilled
+        // next are done
+        if (it + 1 >= nit) wait_vm_lgkm0<0>();
+        else if (with_b) wait_vm_lgkm0<2 * NS>();
+        else wait_vm_lgkm0<NS>();
         asm volatile("" ::: "memory");
         __builtin_amdgcn_s_barrier();
         asm volatile("" ::: "memory");
         if (it + 2 < nit) issue(it + 2);
         const u16 *As = lds + (it % NSTAGE) * STAGE, *Bs = As + NS * BM * BK;
-        v8bf af[NS][4], bfg[NS][2];
+        v8bf af[NS][WI], bfg[NS][2];
 #pragma unroll
         for (int s = 0; s < NS; s++) {
 #pragma unroll
-            for (int i = 0; i < 4; i++) {
-                const int row = wr * 64 + i * 16 + fr;
+            for (int i = 0; i < WI; i++) {
+                const int row = rbase + i * 16 + fr;
                 af[s][i] = *(const v8bf *)&As[(s * BM + row) * BK + 8 * ((lane >> 4) ^ sw_seg(row))];
             }
 #pragma unroll
             for (int j = 0; j < 2; j++) {
-                const int row = wc * 32 + j * 16 + fr;
+                const int row = cbase + j * 16 + fr;
                 bfg[s][j] = *(const v8bf *)&Bs[(s * BN + row) * BK + 8 * ((lane >> 4) ^ sw_seg(row))];
             }
         }
-        v4f acc[4][2];
+        v4f acc[WI][2];
 #pragma unroll
-        for (int i = 0; i < 4; i++)
+        for (int i = 0; i < WI; i++)
 #pragma unroll
             for (int j = 0; j < 2; j++) acc[i][j] = (v4f){0.f, 0.f, 0.f, 0.f};
         // cross products of order < NS, smallest first
@@ -319,20 +315,20 @@ __global__ __launch_bounds__(512) void synth_b16_kernel(SynthB16Args a, int nx, 
             for (int sa = ord; sa >= 0; sa--) {
                 const int sb = ord - sa;
 #pragma unroll
-                for (int i = 0; i < 4; i++)
+                for (int i = 0; i < WI; i++)
 #pragma unroll
                     for (int j = 0; j < 2; j++)
                         acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[sa][i], bfg[sb][j], acc[i][j], 0, 0, 0);
             }
 #pragma unroll
-        for (int i = 0; i < 4; i++)
+        for (int i = 0; i < WI; i++)
 #pragma unroll
             for (int j = 0; j < 2; j++)
 #pragma unroll
                 for (int q = 0; q < 4; q++) acc64[i][j][q] += (double)acc[i][j][q];
         if (a.mirror && it == nit_e - 1) {
 #pragma unroll
-            for (int i = 0; i < 4; i++)
+            for (int i = 0; i < WI; i++)
 #pragma unroll
                 for (int j = 0; j < 2; j++)
 #pragma unroll
@@ -352,17 +348,17 @@ __global__ __launch_bounds__(512) void synth_b16_kernel(SynthB16Args a, int nx, 
     const int comp0 = job->comp[0], comp1 = job->comp[1];
     const double sign0 = job->sign[0], sign1 = job->sign[1], msign = job->msign;
 #pragma unroll
-    for (int i = 0; i < 4; i++)
+    for (int i = 0; i < WI; i++)
 #pragma unroll
         for (int h2 = 0; h2 < 2; h2++) {
-            const int row = m0 + wr * 64 + i * 16 + 4 * (lane >> 4) + 2 * h2;  // even; M is even
+            const int row = m0 + rbase + i * 16 + 4 * (lane >> 4) + 2 * h2;  // even; M is even
             if (row >= M) continue;
             const int pr = row >> 1;
             const int r = pr / a.nslot, sl = pr - r * a.nslot;
             const size_t pos = a.slot_out[sl];
 #pragma unroll
             for (int j = 0; j < 2; j++) {
-                const int col = n0 + wc * 32 + j * 16 + fr;
+                const int col = n0 + cbase + j * 16 + fr;
                 if (col >= a.nh) continue;
                 const int q0 = 2 * h2, q1 = q0 + 1;
                 const double o0 = acc64[i][j][q0], o1 = acc64[i][j][q1];
@@ -385,6 +381,29 @@ __global__ __launch_bounds__(512) void synth_b16_kernel(SynthB16Args a, int nx, 
                 }
             }
         }
+}
+
+template <int NS, int KIND>  // KIND 0: receiver rows, 1: surface rows (separate symbol for profiling)
+__global__ __launch_bounds__(512) void synth_b16_kernel(SynthB16Args a, int nx, int ny)
+{
+    __shared__ __attribute__((aligned(1024))) u16 lds[NSTAGE * NS * (BM + BN) * BK];
+    int tx, ty, tz, tile = blockIdx.x;
+    if (a.tiles) {
+        // host order: groups of tiles that share operands on one XCD,
+        // longest l series first; -1 marks an empty slot
+        if ((int)blockIdx.x >= a.ntiles) return;
+        const int t = a.tiles[blockIdx.x];
+        if (t < 0) return;
+        tile = t;
+        tx = t % nx;
+        ty = (t / nx) % ny;
+        tz = t / (nx * ny);
+    } else if (!xcd_tile_b16(nx, ny, tx, ty, tz)) {
+        return;
+    }
+    const int m0 = ty * BM, n0 = tx * BN;
+    if (a.nh - n0 <= SYNTH_B16_NARROW) synth_b16_tile<NS, true>(a, a.jobs + tz, tile, m0, n0, lds);
+    else synth_b16_tile<NS, false>(a, a.jobs + tz, tile, m0, n0, lds);
 }
 
 void launch_split_rows(const double2 *E, int nslot, int nq, int r_eff, int lcap, int ns, unsigned short *Eb,

@@ -94,6 +94,23 @@ def test_psv_mixed_asymmetric_grid():
     _vs_oracle(eng, inp, range(1, 7), lambda x, i: ol.psv_omega(x, i), SMALL_TOL["psv", "BF16X6"])
 
 
+# Distance grids whose last 128-column tile of the bf16 synthesis holds at
+# most 32 columns (its narrow path): 301 distances with the symmetry (151
+# columns computed, a last tile of 23) and 131 without (a last tile of 3).
+NARROW = [dict(thetamin=0., thetamax=180., thetadelta=0.6), dict(thetamin=0., thetamax=13., thetadelta=0.1)]
+
+
+@pytest.mark.parametrize("grid", NARROW)
+def test_psv_mixed_narrow_last_tile(grid):
+    import dsm_sgt
+    p = dict(tlen=1638.4, imax=16, rmin_=5371., rmax_=6371., rdelta_=250., maxlmax=400, **grid)
+    inp = di.synthetic_input(**p)
+    eng = _engine(inp, dsm_sgt.PSV, dsm_sgt.BF16X6)
+    eng.run(1, 9)
+    w = _vs_oracle(eng, inp, (2, 9), lambda x, i: ol.psv_omega(x, i, fast=True), SMALL_TOL["psv", "BF16X6"])
+    print(f"P-SV BF16X6, {eng.info.theta_n} distances: max rel L2 {w:.2e}")
+
+
 def test_psv_mixed_uncut_small_vs_oracle():
     """The batched mixed form with the depth cut switched off
     (DSM_SGT_DEPTH_CUT=0): the solve stores the factors of the full
