@@ -71,6 +71,14 @@ __host__ __device__ inline void bf16_terms(double x, int ns, unsigned short *t)
         r -= bf16_value(t[s]);
     }
 }
+#ifdef __HIPCC__  // device code only (the host set-up includes this header too)
+// catalog stores of the synthesis: a non-finite value counts 1
+__device__ __forceinline__ int nonfinite2(double2 v) { return !(__builtin_isfinite(v.x) && __builtin_isfinite(v.y)); }
+__device__ __forceinline__ void add_nonfinite(unsigned long long *count, int n)
+{
+    if (n && count) atomicAdd(count, (unsigned long long)n);
+}
+#endif
 // degree order of the bf16 operands: even l first, then odd l
 __host__ __device__ inline int lperm(int l, int lcap) { return (l & 1) ? (lcap >> 1) + (l >> 1) : (l >> 1); }
 // Destination of the bf16 coefficient rows: Eb [q][ns][m][lcap] with

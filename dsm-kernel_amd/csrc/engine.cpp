@@ -201,6 +201,7 @@ struct dsm_sgt {
     int ns = 0;                // bf16 terms per operand (0: not a BF16 form)
     int mirror = 0, nh = 0, nhp = 0;
     DevBuf d_tb, d_eb, d_stash, d_nlr, d_tiles;
+    DevBuf d_diag;  // [0] small pivots, [1] non-finite catalog values of the current run
     std::vector<int> tiles_main, tiles_acc;  // host copies of the tile orders (launch sizes)
     int tiles_main_n = 0, tiles_acc_n = 0;
     DevBuf d_bfe, d_bzbase, d_bzsolid, d_bznl, d_bzkkdr, d_bcolz, d_bovrii, d_bovr, d_bscal;  // device band assembly
@@ -337,6 +338,7 @@ static void launch_synth(dsm_sgt_t *h, const SynthArgs &ga, const DevBuf &Eb, co
         b.stash = h->d_stash.as<double>();
         b.tiles = tiles;
         b.ntiles = ntiles;
+        b.nonfinite = ga.nonfinite;
         launch_synth_b16(b, h->ns, h->stream);
     } else if (h->precision == DSM_SGT_FP32) {
         launch_synth_f32(ga, h->d_tab32.as<float>(), h->stream);
@@ -821,6 +823,7 @@ static int sh_run_batch(dsm_sgt_t *h, const ShPrep &p)
         HIPCHK(h, hipMemcpyAsync(h->d_slots.p, active.data(), sizeof(int) * active.size(), hipMemcpyHostToDevice,
                                  h->stream));
         ShSolveArgs sa{};
+        sa.small_pivots = h->d_diag.as<unsigned long long>();
         sa.band = h->d_band.as<double2>();
         sa.slot_list = h->d_slots.as<int>();
         sa.nlb = nlb;
@@ -940,6 +943,7 @@ static int sh_run_batch(dsm_sgt_t *h, const ShPrep &p)
     ev_end(h, tz);
 
     SynthArgs ga{};
+    ga.nonfinite = h->d_diag.as<unsigned long long>() + 1;
     ga.E = h->d_E.as<double2>();
     ga.nq = 36;
     ga.r_eff = R;
@@ -1693,6 +1697,7 @@ static int psv_run_batch(dsm_sgt_t *h, const PsvPrep &p, bool full_range)
     const bool tiled = h->ns && h->xg >= 0;
 
     SynthArgs ga{};
+    ga.nonfinite = h->d_diag.as<unsigned long long>() + 1;
     ga.E = h->d_E.as<double2>();
     ga.nq = PSV_NQ;
     ga.r_eff = R;
@@ -1793,6 +1798,33 @@ int dsm_sgt_create(const dsm_sgt_input_t *in, int32_t mode, int32_t precision, i
     return dsm_sgt_create_tuned(in, mode, precision, device, nullptr, handle);
 }
 
+// The inputs the set-up reads, checked before anything is computed: zone
+// count, frequency and degree ranges, and every model and grid value finite
+// (the reference reads them unchecked, and a non-finite value would reach
+// the catalogs as NaN).  Empty string: valid.
+static std::string check_input(const dsm_sgt_input_t &in)
+{
+    if (in.nzone < 1 || in.nzone > DSM_SGT_MAXZONE)
+        return "nzone " + std::to_string(in.nzone) + " outside 1.." + std::to_string(DSM_SGT_MAXZONE);
+    if (in.maxlmax < 0) return "maxlmax is negative";
+    if (in.imin < 0 || in.imax < in.imin) return "frequency range imin..imax is empty or negative";
+    auto finite = [](double v) { return std::isfinite(v); };
+    for (int z = 0; z < in.nzone; z++) {
+        bool ok = finite(in.vrmin[z]) && finite(in.vrmax[z]) && finite(in.qmu[z]) && finite(in.qkappa[z]);
+        for (int c = 0; c < 4; c++)
+            ok = ok && finite(in.rho[z][c]) && finite(in.vpv[z][c]) && finite(in.vph[z][c]) && finite(in.vsv[z][c]) &&
+                 finite(in.vsh[z][c]) && finite(in.eta[z][c]);
+        if (!ok) return "zone " + std::to_string(z + 1) + " of the model holds a non-finite value";
+    }
+    const double grid[] = {in.re, in.ratc, in.ratl, in.omegai_seed, in.tlen, in.rmin_, in.rmax_,
+                           in.rdelta_, in.r0min, in.thetamin, in.thetamax, in.thetadelta};
+    for (double v : grid)
+        if (!finite(v)) return "a configuration or grid value is not finite";
+    if (!(in.tlen > 0.0)) return "tlen must be positive";
+    if (!(in.omegai_seed > 0.0)) return "the omegai seed must be positive (omegai = -log(seed) / tlen)";
+    return "";
+}
+
 int dsm_sgt_create_tuned(const dsm_sgt_input_t *in, int32_t mode, int32_t precision, int32_t device,
                          const dsm_sgt_tuning_t *tuning, dsm_sgt_t **handle)
 {
@@ -1820,6 +1852,10 @@ int dsm_sgt_create_tuned(const dsm_sgt_input_t *in, int32_t mode, int32_t precis
             break;
         }
         h->ns = (precision == DSM_SGT_BF16X3) ? 2 : (precision == DSM_SGT_BF16X6) ? 3 : 0;
+        if (const std::string bad = check_input(*in); !bad.empty()) {
+            rc = fail(h, DSM_SGT_EINVAL, bad);
+            break;
+        }
         int ndev = 0;
         if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev) {
             rc = fail(h, DSM_SGT_EHIP, "no HIP device " + std::to_string(device));
@@ -1840,6 +1876,11 @@ int dsm_sgt_create_tuned(const dsm_sgt_input_t *in, int32_t mode, int32_t precis
         }
         if (const hipError_t e = km_static_twiddles(device, &h->km_static_tw); e != hipSuccess) {
             rc = fail(h, DSM_SGT_EHIP, std::string("static twiddle storage: ") + hipGetErrorString(e));
+            break;
+        }
+        if (h->d_diag.ensure(2 * sizeof(unsigned long long)) != hipSuccess ||
+            hipMemset(h->d_diag.p, 0, h->d_diag.n) != hipSuccess) {
+            rc = fail(h, DSM_SGT_ENOMEM, "diagnostic counters");
             break;
         }
         rc = (mode == DSM_SGT_SH) ? sh_create(h) : psv_create(h);
@@ -2022,9 +2063,8 @@ int dsm_sgt_get_grid(dsm_sgt_t *h, double *radii, double *theta_deg)
 
 // Run the frequencies `all` (ascending, distinct, no 0) in batches of
 // consecutive entries.
-static int run_frequencies(dsm_sgt_t *h, const std::vector<int> &all)
+static int run_batches(dsm_sgt_t *h, const std::vector<int> &all)
 {
-    if (hipSetDevice(h->device) != hipSuccess) return fail(h, DSM_SGT_EHIP, "hipSetDevice failed");
     // (the l bound keeps the last batch's cut-off across runs: a run of the
     // next catalog window continues from it; a wrong guess only costs a
     // full-range re-run of the batch)
@@ -2086,6 +2126,27 @@ static int run_frequencies(dsm_sgt_t *h, const std::vector<int> &all)
             return rc;
         }
     }
+    return DSM_SGT_OK;
+}
+
+// A run of the frequency set `all`: the batches, then the counts of small
+// pivots (stats) and of non-finite catalog values, which fail the run with
+// DSM_SGT_ENUMERIC (the catalog is kept, as the reference writes it).
+static int run_frequencies(dsm_sgt_t *h, const std::vector<int> &all)
+{
+    if (hipSetDevice(h->device) != hipSuccess) return fail(h, DSM_SGT_EHIP, "hipSetDevice failed");
+    HIPCHK(h, hipMemsetAsync(h->d_diag.p, 0, h->d_diag.n, h->stream));
+    const int rc = run_batches(h, all);
+    if (rc) return rc;
+    unsigned long long diag[2] = {0, 0};
+    HIPCHK(h, hipMemcpyAsync(diag, h->d_diag.p, sizeof(diag), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    h->stats.small_pivots += (int64_t)diag[0];
+    h->stats.nonfinite += (int64_t)diag[1];
+    if (diag[1])
+        return fail(h, DSM_SGT_ENUMERIC,
+                    std::to_string(diag[1]) + " non-finite catalog values in frequencies " + std::to_string(all.front()) +
+                        ".." + std::to_string(all.back()) + " (a zero or non-finite pivot, or non-finite input)");
     return DSM_SGT_OK;
 }
 

@@ -77,6 +77,7 @@ struct SynthArgs {
     const SynthJob *jobs;  // device array
     int njobs;
     int mirror, nh;        // FP64: distance symmetry, nh = (theta_n + 1) / 2 columns computed
+    unsigned long long *nonfinite;  // count of non-finite values stored (may be null)
 };
 void launch_synth_f64(const SynthArgs &a, hipStream_t s);
 // Mixed form (synth32.hip): FP32 operands on FP32 MFMA, FP64 accumulation
@@ -99,6 +100,7 @@ struct SynthB16Args {
     double *stash;             // [workgroup][8 waves][32][64 lanes]
     const int *tiles;          // optional [grid]: tile (job ny + ty) nx + tx per workgroup, -1 none
     int ntiles;
+    unsigned long long *nonfinite;  // count of non-finite values stored (may be null)
 };
 // zero degrees l in [nl(slot), lend] of every bf16 row
 void launch_zero_tail_b16(unsigned short *Eb, int nq, int ns, int nslot, int r_eff, int lcap, const int *nl, int lend,

@@ -5,7 +5,11 @@
 
 namespace dsm {
 
-__global__ __launch_bounds__(64) void sh_solve_kernel(ShSolveArgs a) { sh_solve_lane(a, blockIdx.x, threadIdx.x); }
+__global__ __launch_bounds__(64) void sh_solve_kernel(ShSolveArgs a)
+{
+    const int small = sh_solve_lane(a, blockIdx.x, threadIdx.x);
+    if (small && a.small_pivots) atomicAdd(a.small_pivots, (unsigned long long)small);
+}
 
 void launch_sh_solve(const ShSolveArgs &a, int nblocks, hipStream_t s)
 {

@@ -38,6 +38,7 @@ struct ShSolveArgs {
     const double2 *om;     // [slot][2]: comega2, coef(spn)
     const ShSrcConst *src;
     const int *jstart;     // mixed forms, depth cut: first row solved per block of the launch (null: row 1)
+    unsigned long long *small_pivots;  // count of pivots with |d| <= 1e-14 (may be null)
 };
 // Depth cut of the SH systems (mixed forms): per block of 64 degrees the
 // first row whose decay below the source, integrated from the source down,
@@ -167,11 +168,13 @@ __host__ __device__ inline void sh_excitation(int l, double2 comega2, double2 co
 // values go to a scratch buffer laid out [row][lane] (coalesced), read back
 // once by the backward pass.
 // ---------------------------------------------------------------------------
-__host__ __device__ inline void sh_solve_lane(const ShSolveArgs &a, int b, int tx)
+// Returns the lane's number of pivots with |d| <= 1e-14 (or not finite),
+// the test of dclisb0 (dclib.f90:37-86), which reports and continues.
+__host__ __device__ inline int sh_solve_lane(const ShSolveArgs &a, int b, int tx)
 {
     const int sl = a.slot_list[b / a.nlb];
     const int l = a.l0 + (b % a.nlb) * 64 + tx;
-    if (l > a.lend) return;
+    if (l > a.lend) return 0;
     const long gid = (long)b * 64 + tx;
     const double l2 = (double)l * (double)(l + 1);
     const int nn = a.nn, k = a.k;
@@ -186,7 +189,7 @@ __host__ __device__ inline void sh_solve_lane(const ShSolveArgs &a, int b, int t
     if (l == 0) {  // no m with 0 < |m| <= l: nothing is solved (SGTsh-MPI.f90:453)
         for (int p = 0; p < a.nneed; p++)
             for (int q = 0; q < SH_NSOL; q++) xo[(size_t)(SH_NSOL * p + q) * a.lcap] = cz();
-        return;
+        return 0;
     }
     Exc ex[10];
     double2 gam;
@@ -199,17 +202,20 @@ __host__ __device__ inline void sh_solve_lane(const ShSolveArgs &a, int b, int t
     const int jlo = a.jstart ? a.jstart[b] : 1;
     double2 drp = cz();
     double2 z[10];
+    int small = 0;
     for (int j = jlo; j <= nn; j++) {
         const double2 dg = caxpy(a0d[j - 1], l2, a2d[j - 1]);
-        double2 Lj, drj;
+        double2 Lj, pv;
         if (j == jlo) {
-            drj = crecip(dg);
+            pv = dg;
             Lj = cz();
         } else {
             const double2 xx = caxpy(a0o[j - 1], l2, a2o[j - 1]);
             Lj = cmul(xx, drp);
-            drj = crecip(csub(dg, cmul(xx, Lj)));
+            pv = csub(dg, cmul(xx, Lj));
         }
+        const double2 drj = crecip(pv);
+        small += !(pv.x * pv.x + pv.y * pv.y > 1e-28);
         Ls[(size_t)(j - 1) * ss + gid] = Lj;
         if (j >= k) {
             for (int s = 0; s < 10; s++) {
@@ -244,6 +250,7 @@ __host__ __device__ inline void sh_solve_lane(const ShSolveArgs &a, int b, int t
     }
     for (; p < a.nneed; p++)  // stencil rows below the cut
         for (int q = 0; q < SH_NSOL; q++) xo[(size_t)(SH_NSOL * p + q) * a.lcap] = cz();
+    return small;
 }
 
 // ---------------------------------------------------------------------------

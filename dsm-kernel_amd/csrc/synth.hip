@@ -166,6 +166,7 @@ __global__ __launch_bounds__(512) void synth_f64_kernel(SynthArgs a, int nx, int
     // v_mfma_f64_16x16x4f64 leaves D[i][j] in lane j + 16*(i%4), register i/4.
     double *out = (double *)job.out;
     const int ntgt = job.comp[1] >= 0 ? 2 : 1;
+    int bad = 0;
 #pragma unroll
     for (int i = 0; i < 4; i++)
 #pragma unroll
@@ -185,20 +186,27 @@ __global__ __launch_bounds__(512) void synth_f64_kernel(SynthArgs a, int nx, int
                     double *o = out + 2 * (base + col) + reim;
                     if (!MIR) {
                         const double v = job.sign[t] * acc[0][i][j][q];
-                        *o = job.accumulate ? *o + v : v;
+                        const double s = job.accumulate ? *o + v : v;
+                        *o = s;
+                        bad += !__builtin_isfinite(s);
                     } else {
                         const double e = acc[0][i][j][q], od = acc[MIR][i][j][q];
                         const double v = job.sign[t] * (e + od);
-                        *o = job.accumulate ? *o + v : v;
+                        const double s = job.accumulate ? *o + v : v;
+                        *o = s;
+                        bad += !__builtin_isfinite(s);
                         if (mcol != col) {
                             double *om = out + 2 * (base + mcol) + reim;
                             const double w = job.sign[t] * (job.msign * (e - od));
-                            *om = job.accumulate ? *om + w : w;
+                            const double sw = job.accumulate ? *om + w : w;
+                            *om = sw;
+                            bad += !__builtin_isfinite(sw);
                         }
                     }
                 }
             }
         }
+    add_nonfinite(a.nonfinite, bad);
 }
 
 void launch_synth_f64(const SynthArgs &a, hipStream_t s)

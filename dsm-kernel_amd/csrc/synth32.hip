@@ -154,6 +154,7 @@ __global__ __launch_bounds__(512) void synth_f32_kernel(SynthArgs a, const float
     // one (slot, r) row, 16 lanes write 16 consecutive distances
     double2 *out = job.out;
     const int ntgt = job.comp[1] >= 0 ? 2 : 1;
+    int bad = 0;
 #pragma unroll
     for (int i = 0; i < 4; i++)
 #pragma unroll
@@ -170,10 +171,13 @@ __global__ __launch_bounds__(512) void synth_f32_kernel(SynthArgs a, const float
                 for (int t = 0; t < ntgt; t++) {
                     double2 *o = out + ((pos * job.ncomp + job.comp[t]) * R + r) * a.theta_n + col;
                     const double2 v = make_double2(job.sign[t] * acc64[i][j][2 * h2], job.sign[t] * acc64[i][j][2 * h2 + 1]);
-                    *o = job.accumulate ? make_double2(o->x + v.x, o->y + v.y) : v;
+                    const double2 w = job.accumulate ? make_double2(o->x + v.x, o->y + v.y) : v;
+                    *o = w;
+                    bad += nonfinite2(w);
                 }
             }
         }
+    add_nonfinite(a.nonfinite, bad);
 }
 
 __global__ void tab_to_f32_kernel(const double *src, float *dst, size_t n)

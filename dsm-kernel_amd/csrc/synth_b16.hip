@@ -347,6 +347,7 @@ __device__ __forceinline__ void synth_b16_tile(const SynthB16Args &a, const Synt
     const int ncomp = job->ncomp, accum = job->accumulate;
     const int comp0 = job->comp[0], comp1 = job->comp[1];
     const double sign0 = job->sign[0], sign1 = job->sign[1], msign = job->msign;
+    int bad = 0;
 #pragma unroll
     for (int i = 0; i < WI; i++)
 #pragma unroll
@@ -373,14 +374,19 @@ __device__ __forceinline__ void synth_b16_tile(const SynthB16Args &a, const Synt
                     const double sg = t ? sign1 : sign0;
                     double2 *od = out + ((pos * ncomp + comp) * R + r) * a.theta_n;
                     const double2 v = make_double2(sg * (e0 + o0), sg * (e1 + o1));
-                    od[col] = accum ? make_double2(od[col].x + v.x, od[col].y + v.y) : v;
+                    const double2 sv = accum ? make_double2(od[col].x + v.x, od[col].y + v.y) : v;
+                    od[col] = sv;
+                    bad += nonfinite2(sv);
                     if (mir) {
                         const double2 w = make_double2(sg * msign * (e0 - o0), sg * msign * (e1 - o1));
-                        od[mc] = accum ? make_double2(od[mc].x + w.x, od[mc].y + w.y) : w;
+                        const double2 sw = accum ? make_double2(od[mc].x + w.x, od[mc].y + w.y) : w;
+                        od[mc] = sw;
+                        bad += nonfinite2(sw);
                     }
                 }
             }
         }
+    add_nonfinite(a.nonfinite, bad);
 }
 
 template <int NS, int KIND>  // KIND 0: receiver rows, 1: surface rows (separate symbol for profiling)

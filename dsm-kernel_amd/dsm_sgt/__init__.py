@@ -20,7 +20,7 @@ SH, PSV = 1, 2
 FP64, FP32, BF16X3, BF16X6 = 0, 1, 2, 3
 TSGT, RSGT, SYNN = 0, 1, 2
 
-ERRORS = {1: "EINVAL", 2: "ESTATE", 3: "EHIP", 4: "ENOMEM", 5: "EIO", 6: "EUNSUPPORTED"}
+ERRORS = {1: "EINVAL", 2: "ESTATE", 3: "EHIP", 4: "ENOMEM", 5: "EIO", 6: "EUNSUPPORTED", 7: "ENUMERIC"}
 
 
 class Info(C.Structure):
@@ -35,7 +35,7 @@ class Stats(C.Structure):
                 ("systems", C.c_int64), ("solve_bytes", C.c_double), ("synth_flops", C.c_double),
                 ("synth_bytes", C.c_double), ("synth_mfma_flops", C.c_double),
                 ("prep_wait_ms", C.c_double), ("device_wait_ms", C.c_double), ("solve_retries", C.c_int64),
-                ("solve_alg_bytes", C.c_double)]
+                ("solve_alg_bytes", C.c_double), ("small_pivots", C.c_int64), ("nonfinite", C.c_int64)]
 
 
 class Tuning(C.Structure):

@@ -70,7 +70,9 @@ enum {
     DSM_SGT_EHIP = 3,      /* HIP runtime error (no device, launch failure) */
     DSM_SGT_ENOMEM = 4,    /* device or host allocation failed */
     DSM_SGT_EIO = 5,       /* catalog file could not be written */
-    DSM_SGT_EUNSUPPORTED = 6
+    DSM_SGT_EUNSUPPORTED = 6,
+    DSM_SGT_ENUMERIC = 7   /* the run finished but the catalog holds non-finite values
+                              (stats.nonfinite); the catalog is kept */
 };
 
 typedef struct dsm_sgt dsm_sgt_t;
@@ -104,6 +106,10 @@ typedef struct dsm_sgt_stats {
     double solve_alg_bytes;   /* algorithmic bytes of the reference's (omega, l >= 1) systems below the
                                  cut-off: 16 (w n + 4 R r_n) each, w = 4 / 2 band entries per row, n the rows
                                  solved (after the depth cut), R = 18 / 14 right-hand sides */
+    int64_t small_pivots;     /* SH: pivots with |d| <= 1e-14 in the band factorizations, the test of
+                                 dclisb0 (src/SGTsh/dclib.f90:37-86), which reports and continues; P-SV's
+                                 dcsymbdl0 receives eps = 0 from its driver and tests nothing */
+    int64_t nonfinite;        /* non-finite values the synthesis stored into the catalogs */
 } dsm_sgt_stats_t;
 
 /* Create an engine on HIP device `device` for `mode` and `precision`.

@@ -46,6 +46,27 @@ def test_no_fallback_without_gpu():
     assert e.value.code == 3  # EHIP
 
 
+@pytest.mark.parametrize("field", ["rho", "vsv", "qmu", "thetadelta", "omegai_seed"])
+def test_create_refuses_non_finite_input(field):
+    """dsm_sgt_create checks the inputs before any device call: a NaN in the
+    model or the grid is EINVAL naming the bad value (GPU or not)."""
+    import math
+    import dsm_sgt
+    inp = dsm_sgt.DsmSgtInput.from_buffer_copy(di.synthetic_input(
+        tlen=3276.8, imax=8, rmin_=4421., rmax_=6371., rdelta_=250., thetamin=0., thetamax=180., thetadelta=15.,
+        maxlmax=300))
+    if field in ("rho", "vsv"):
+        getattr(inp, field)[1][2] = math.nan
+    elif field == "qmu":
+        inp.qmu[0] = math.inf
+    else:
+        setattr(inp, field, math.nan)
+    with pytest.raises(dsm_sgt.DsmSgtError) as e:
+        dsm_sgt.Engine(inp)
+    assert e.value.code == 1, e.value  # EINVAL
+    assert "finite" in str(e.value) or "zone" in str(e.value)
+
+
 PARAM = """# parameter file in the reference's grammar
 ../configs/DSM.sample.config
 out/
@@ -97,8 +118,8 @@ def test_struct_layouts_match_the_header(tmp_path):
 int main(void) {
   printf("%zu %zu %zu %zu\\n", sizeof(dsm_sgt_tuning_t), sizeof(dsm_sgt_stats_t), sizeof(dsm_sgt_info_t),
          sizeof(dsm_sgt_input_t));
-  printf("%zu %zu %zu\\n", offsetof(dsm_sgt_tuning_t, catalog_window), offsetof(dsm_sgt_stats_t, solve_alg_bytes),
-         offsetof(dsm_sgt_info_t, r0));
+  printf("%zu %zu %zu %zu\\n", offsetof(dsm_sgt_tuning_t, catalog_window), offsetof(dsm_sgt_stats_t, solve_alg_bytes),
+         offsetof(dsm_sgt_info_t, r0), offsetof(dsm_sgt_stats_t, nonfinite));
   return 0;
 }
 """)
@@ -108,4 +129,5 @@ int main(void) {
                                                                         check=True).stdout.split("\n")[:2]]
     assert sizes == [C.sizeof(dsm_sgt.Tuning), C.sizeof(dsm_sgt.Stats), C.sizeof(dsm_sgt.Info),
                      C.sizeof(dsm_sgt.DsmSgtInput)]
-    assert offs == [dsm_sgt.Tuning.catalog_window.offset, dsm_sgt.Stats.solve_alg_bytes.offset, dsm_sgt.Info.r0.offset]
+    assert offs == [dsm_sgt.Tuning.catalog_window.offset, dsm_sgt.Stats.solve_alg_bytes.offset, dsm_sgt.Info.r0.offset,
+                    dsm_sgt.Stats.nonfinite.offset]
