@@ -25,6 +25,13 @@ module dsm_sgt_api
      integer(c_int32_t) :: reserved(3) = 0
   end type dsm_sgt_tuning
 
+  ! dsm_km_params_t (include/dsm_km.h): KernelMaker's parameters of the catalog stage
+  type, bind(c) :: dsm_km_params
+     integer(c_int32_t) :: fmin, fmax, compo, reserved = 0
+     real(c_double) :: c_red_reci, ipdistance
+     real(c_double) :: mt(6)
+  end type dsm_km_params
+
   interface
      integer(c_int32_t) function dsm_sgt_create(inp, mode, precision, device, handle) bind(c)
        import :: c_int32_t, c_ptr, dsm_sgt_input
@@ -68,6 +75,26 @@ module dsm_sgt_api
        import :: c_int32_t, c_ptr
        type(c_ptr), value :: handle
      end function dsm_sgt_wait
+     ! KernelMaker's catalog stage for npoints scattering points at receiver
+     ! radius index ir (0-based): h3, h4 complex(8) (6, fmin:fmax, npoints)
+     integer(c_int32_t) function dsm_km_points(psv, sh, ir, params, npoints, deltar, deltas, trig, h3, h4) bind(c)
+       import :: c_int32_t, c_ptr, c_double, c_double_complex, dsm_km_params
+       type(c_ptr), value :: psv, sh
+       integer(c_int32_t), value :: ir, npoints
+       type(dsm_km_params), intent(in) :: params
+       real(c_double), intent(in) :: deltar(*), deltas(*), trig(12, *)
+       complex(c_double_complex), intent(out) :: h3(6, *), h4(6, *)
+     end function dsm_km_points
+     ! vectorFFT_double for nvec spectra ccvec(imin:imax, nvec) -> rvec(iws:iwe, nvec)
+     integer(c_int32_t) function dsm_km_vectorfft(h, nvec, imin, imax, np1, ccvec, omegai, tlen, iws, iwe, rvec) &
+          bind(c)
+       import :: c_int32_t, c_ptr, c_double, c_double_complex
+       type(c_ptr), value :: h
+       integer(c_int32_t), value :: nvec, imin, imax, np1, iws, iwe
+       complex(c_double_complex), intent(in) :: ccvec(*)
+       real(c_double), value :: omegai, tlen
+       real(c_double), intent(out) :: rvec(*)
+     end function dsm_km_vectorfft
      integer(c_int32_t) function dsm_sgt_lcut(handle, i, llog) bind(c)
        import :: c_int32_t, c_ptr
        type(c_ptr), value :: handle
