@@ -13,6 +13,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cstddef>
 #include <cmath>
 #include <complex>
 #include <cstdio>
@@ -1822,6 +1823,17 @@ static std::string check_input(const dsm_sgt_input_t &in)
         if (!finite(v)) return "a configuration or grid value is not finite";
     if (!(in.tlen > 0.0)) return "tlen must be positive";
     if (!(in.omegai_seed > 0.0)) return "the omegai seed must be positive (omegai = -log(seed) / tlen)";
+    // receiver lists: present when counted, finite, strictly ascending
+    auto list_ok = [&](int32_t n, const double *v, double lo, double hi) {
+        if (n < 0 || n > (1 << 20) || (n > 0 && !v)) return false;
+        for (int32_t i = 0; i < n; i++)
+            if (!finite(v[i]) || v[i] < lo || v[i] > hi || (i > 0 && !(v[i] > v[i - 1]))) return false;
+        return true;
+    };
+    if (!list_ok(in.r_list_n, in.r_list, 0.0, 1.0e9))
+        return "the receiver radius list must be finite, positive and strictly ascending";
+    if (!list_ok(in.theta_list_n, in.theta_list, 0.0, 180.0))
+        return "the distance list must be finite, within 0..180 degrees and strictly ascending";
     return "";
 }
 
@@ -1918,7 +1930,10 @@ int dsm_sgt_autotune(const dsm_sgt_input_t *in, int32_t mode, int32_t precision,
     auto mix = [&hv](const void *p, size_t n) {
         for (size_t i = 0; i < n; i++) hv = (hv ^ ((const unsigned char *)p)[i]) * 1099511628211ULL;
     };
-    mix(in, sizeof(*in));
+    // the record up to the list pointers, then the lists' values (not their addresses)
+    mix(in, offsetof(dsm_sgt_input_t, r_list));
+    if (in->r_list_n > 0 && in->r_list) mix(in->r_list, sizeof(double) * (size_t)in->r_list_n);
+    if (in->theta_list_n > 0 && in->theta_list) mix(in->theta_list, sizeof(double) * (size_t)in->theta_list_n);
     mix(&mode, sizeof(mode));
     mix(&precision, sizeof(precision));
     mix(prop.gcnArchName, std::strlen(prop.gcnArchName));
@@ -2589,6 +2604,8 @@ int dsm_km_points(dsm_sgt_t *h, dsm_sgt_t *sh, int32_t ir, const dsm_km_params_t
     if (prm->compo < 0 || prm->compo > 2) return fail(h, DSM_SGT_EINVAL, "compo must be 0 (Z), 1 (R) or 2 (T)");
     if (!(prm->ipdistance > 0.0)) return fail(h, DSM_SGT_EINVAL, "ipdistance must be positive");
     if (prm->fmin < 0 || prm->fmax < prm->fmin) return fail(h, DSM_SGT_EINVAL, "bad frequency range");
+    if (h->in.theta_list_n > 0 || (sh && sh->in.theta_list_n > 0))
+        return fail(h, DSM_SGT_EINVAL, "the sinc interpolation in distance needs the uniform distance grid, not a list");
     const int nf = prm->fmax - prm->fmin + 1, T = h->theta_n;
     std::vector<int> pos(nf), pos_sh(nf, -1);
     for (int j = 0; j < nf; j++) {

@@ -187,19 +187,20 @@ int psv_setup(const dsm_sgt_input_t &in, PsvSetup &s, std::string &err)
     s.ratl = in.ratl;
     s.maxlmax = in.maxlmax;
     s.imax = in.imax;
-    if (in.tlen <= 0 || in.rdelta_ <= 0 || in.thetadelta <= 0 || in.maxlmax < 1) {
+    const bool rl = in.r_list_n > 0, tl = in.theta_list_n > 0;
+    if (in.tlen <= 0 || (!rl && in.rdelta_ <= 0) || (!tl && in.thetadelta <= 0) || in.maxlmax < 1) {
         err = "tlen, rdelta, thetadelta and maxlmax must be positive";
         return DSM_SGT_EINVAL;
     }
-    // receivers (SGTpsv-MPI.f90:127-157): no trim in P-SV
-    const int r_n = (int)((in.rmax_ - in.rmin_) / in.rdelta_) + 1;
+    // receivers (SGTpsv-MPI.f90:127-157): no trim in P-SV; or a radius list
+    const int r_n = rl ? in.r_list_n : (int)((in.rmax_ - in.rmin_) / in.rdelta_) + 1;
     if (r_n < 1) {
         err = "no receiver radius";
         return DSM_SGT_EINVAL;
     }
     s.r_n = r_n;
     s.r_.resize(r_n);
-    for (int i = 1; i <= r_n; i++) s.r_[i - 1] = in.rmin_ + (double)(i - 1) * in.rdelta_;
+    for (int i = 1; i <= r_n; i++) s.r_[i - 1] = rl ? in.r_list[i - 1] : in.rmin_ + (double)(i - 1) * in.rdelta_;
     // calstg_for_stackpoints (others.f90:687-762): lambda = A, qkappa
     s.lambda.assign(r_n, 0.0);
     s.qkp.assign(r_n, 0.0);
@@ -212,9 +213,10 @@ int psv_setup(const dsm_sgt_input_t &in, PsvSetup &s, std::string &err)
                 s.qkp[ir] = s.qkappa[iz];
             }
     }
-    s.theta_n = (int)((in.thetamax - in.thetamin) / in.thetadelta) + 1;
+    s.theta_n = tl ? in.theta_list_n : (int)((in.thetamax - in.thetamin) / in.thetadelta) + 1;
     s.theta_deg.resize(s.theta_n);
-    for (int i = 1; i <= s.theta_n; i++) s.theta_deg[i - 1] = in.thetamin + (double)(i - 1) * in.thetadelta;
+    for (int i = 1; i <= s.theta_n; i++)
+        s.theta_deg[i - 1] = tl ? in.theta_list[i - 1] : in.thetamin + (double)(i - 1) * in.thetadelta;
     s.r0 = in.r0min;
     s.intir0 = (int)(s.r0 * 1.0e3);
 

@@ -128,8 +128,8 @@ int sh_setup(const dsm_sgt_input_t &in, ShSetup &s, std::string &err)
     s.rmin = s.vrmin[0];
     s.rmax = s.vrmax[nz - 1];
     s.tlen = in.tlen;
-    if (!(in.tlen > 0) || !(in.omegai_seed > 0) || !(in.rdelta_ > 0) || !(in.thetadelta > 0) ||
-        in.maxlmax < 0 || in.imax < 0) {
+    if (!(in.tlen > 0) || !(in.omegai_seed > 0) || (in.r_list_n <= 0 && !(in.rdelta_ > 0)) ||
+        (in.theta_list_n <= 0 && !(in.thetadelta > 0)) || in.maxlmax < 0 || in.imax < 0) {
         err = "invalid tlen / omegai / rdelta / thetadelta / maxlmax / imax";
         return DSM_SGT_EINVAL;
     }
@@ -138,12 +138,14 @@ int sh_setup(const dsm_sgt_input_t &in, ShSetup &s, std::string &err)
     s.maxlmax = in.maxlmax;
     s.imax = in.imax;
 
-    // Receivers strictly above the CMB (SGTsh-MPI.f90:152-179).
+    // Receivers strictly above the CMB (SGTsh-MPI.f90:152-179); a radius
+    // list (ascending) loses its entries at or below it the same way.
+    const bool rl = in.r_list_n > 0, tl = in.theta_list_n > 0;
     double rmin_ = in.rmin_;
-    int r_n = (int)((in.rmax_ - in.rmin_) / in.rdelta_) + 1;
+    int r_n = rl ? in.r_list_n : (int)((in.rmax_ - in.rmin_) / in.rdelta_) + 1;
     int ir_ = 0;
     for (int i = 1; i <= r_n; i++)
-        if (rmin_ + (double)(i - 1) * in.rdelta_ <= s.rmin) ir_ = i;
+        if ((rl ? in.r_list[i - 1] : rmin_ + (double)(i - 1) * in.rdelta_) <= s.rmin) ir_ = i;
     r_n -= ir_;
     rmin_ = rmin_ + (double)ir_ * in.rdelta_;
     if (r_n < 1) {
@@ -152,19 +154,20 @@ int sh_setup(const dsm_sgt_input_t &in, ShSetup &s, std::string &err)
     }
     s.r_n = r_n;
     s.r_.resize(r_n);
-    for (int i = 1; i <= r_n; i++) s.r_[i - 1] = rmin_ + (double)(i - 1) * in.rdelta_;
+    for (int i = 1; i <= r_n; i++) s.r_[i - 1] = rl ? in.r_list[ir_ + i - 1] : rmin_ + (double)(i - 1) * in.rdelta_;
     for (int i = 0; i < r_n; i++)
         if (s.r_[i] > s.rmax) {
             err = "receiver above the surface";
             return DSM_SGT_EINVAL;
         }
-    s.theta_n = (int)((in.thetamax - in.thetamin) / in.thetadelta) + 1;
+    s.theta_n = tl ? in.theta_list_n : (int)((in.thetamax - in.thetamin) / in.thetadelta) + 1;
     if (s.theta_n < 1) {
         err = "empty distance grid";
         return DSM_SGT_EINVAL;
     }
     s.theta_deg.resize(s.theta_n);
-    for (int i = 1; i <= s.theta_n; i++) s.theta_deg[i - 1] = in.thetamin + (double)(i - 1) * in.thetadelta;
+    for (int i = 1; i <= s.theta_n; i++)
+        s.theta_deg[i - 1] = tl ? in.theta_list[i - 1] : in.thetamin + (double)(i - 1) * in.thetadelta;
     s.r0 = in.r0min;
     s.intir0 = (int)(s.r0 * 1.0e3);
     if (s.r0 < s.rmin || s.r0 > s.rmax) {

@@ -32,6 +32,8 @@ class DsmSgtInput(C.Structure):
         ("rmin_", C.c_double), ("rmax_", C.c_double), ("rdelta_", C.c_double),
         ("r0min", C.c_double),
         ("thetamin", C.c_double), ("thetamax", C.c_double), ("thetadelta", C.c_double),
+        ("r_list_n", C.c_int32), ("theta_list_n", C.c_int32),
+        ("r_list", C.c_void_p), ("theta_list", C.c_void_p),
     ]
 
 

@@ -17,6 +17,9 @@ module dsm_sgt_api
      real(c_double) :: re, ratc, ratl, omegai_seed, tlen
      real(c_double) :: rmin_, rmax_, rdelta_, r0min
      real(c_double) :: thetamin, thetamax, thetadelta
+     ! receiver lists in place of the grids (DSM3D_forward's files); 0: the grids
+     integer(c_int32_t) :: r_list_n = 0, theta_list_n = 0
+     type(c_ptr) :: r_list = c_null_ptr, theta_list = c_null_ptr
   end type dsm_sgt_input
 
   ! dsm_sgt_tuning_t (launch plan; 0 = default)

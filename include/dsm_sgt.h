@@ -51,6 +51,17 @@ typedef struct dsm_sgt_input {
     double rmin_, rmax_, rdelta_;
     double r0min;
     double thetamin, thetamax, thetadelta;
+    /* Receiver geometry as lists instead of the uniform grids above, as
+     * DSM3D_forward reads it (src/DSM3D_forward/mpi-tipsv.f90:163-210: a
+     * file of receiver radii and a file of distances).  r_list_n > 0: the
+     * receiver radii are r_list[0..r_list_n) (km, ascending), and rmin_,
+     * rmax_, rdelta_ are not used; theta_list_n > 0: the distances are
+     * theta_list[0..theta_list_n) (degrees, ascending, within [0, 180]), and
+     * thetamin, thetamax, thetadelta are not used.  Zero counts (and null
+     * pointers) keep the grids.  The arrays are read by dsm_sgt_create only. */
+    int32_t r_list_n, theta_list_n;
+    const double *r_list;
+    const double *theta_list;
 } dsm_sgt_input_t;
 
 enum { DSM_SGT_SH = 1, DSM_SGT_PSV = 2 };

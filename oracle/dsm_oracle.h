@@ -39,6 +39,11 @@ typedef struct {
     double rmin_, rmax_, rdelta_;         /* receiver radii (km) */
     double r0min;                         /* source radius (km) */
     double thetamin, thetamax, thetadelta;/* epicentral distances (deg) */
+    /* receiver lists in place of the grids (counts > 0; DSM3D_forward's
+     * radius and distance files, src/DSM3D_forward/mpi-tipsv.f90:163-210) */
+    int32_t r_list_n, theta_list_n;
+    const double *r_list;
+    const double *theta_list;
 } orc_input_t;
 
 /* Sizes of one SH run (after the fluid/inner-core removal and the

@@ -1174,10 +1174,10 @@ static int sh_setup(const orc_input_t *in, const sh_fns_t *F, sh_setup_t *s)
     s->rsgtswitch = in->rsgtswitch;
     /* receivers above the CMB (SGTsh-MPI.f90:152-179) */
     double rmin_ = in->rmin_;
-    int r_n = (int)((in->rmax_ - in->rmin_) / in->rdelta_) + 1;
-    int ir_ = 0;
+    int r_n = in->r_list_n > 0 ? in->r_list_n : (int)((in->rmax_ - in->rmin_) / in->rdelta_) + 1;
+    int ir_ = 0;  /* (a radius list: its entries at or below the CMB, the list ascending) */
     for (int i = 1; i <= r_n; i++)
-        if (rmin_ + (double)(i - 1) * in->rdelta_ <= s->rmin) ir_ = i;
+        if ((in->r_list_n > 0 ? in->r_list[i - 1] : rmin_ + (double)(i - 1) * in->rdelta_) <= s->rmin) ir_ = i;
     r_n = r_n - ir_;
     rmin_ = rmin_ + (double)ir_ * in->rdelta_;
     if (r_n < 1) return -2;
@@ -1185,10 +1185,12 @@ static int sh_setup(const orc_input_t *in, const sh_fns_t *F, sh_setup_t *s)
     s->r_ = calloc(r_n, sizeof(double));
     s->rrsta = calloc(3 * r_n, sizeof(double));
     s->iista = calloc(3 * r_n, sizeof(int));
-    for (int i = 1; i <= r_n; i++) s->r_[i - 1] = rmin_ + (double)(i - 1) * in->rdelta_;
-    s->theta_n = (int)((in->thetamax - in->thetamin) / in->thetadelta) + 1;
+    for (int i = 1; i <= r_n; i++)
+        s->r_[i - 1] = in->r_list_n > 0 ? in->r_list[ir_ + i - 1] : rmin_ + (double)(i - 1) * in->rdelta_;
+    s->theta_n = in->theta_list_n > 0 ? in->theta_list_n : (int)((in->thetamax - in->thetamin) / in->thetadelta) + 1;
     s->theta = calloc(s->theta_n, sizeof(double));
-    for (int i = 1; i <= s->theta_n; i++) s->theta[i - 1] = in->thetamin + (double)(i - 1) * in->thetadelta;
+    for (int i = 1; i <= s->theta_n; i++)
+        s->theta[i - 1] = in->theta_list_n > 0 ? in->theta_list[i - 1] : in->thetamin + (double)(i - 1) * in->thetadelta;
     s->r0 = in->r0min;
     s->intir0 = (int)(s->r0 * 1.0e3);
     if (s->r0 < s->rmin || s->r0 > s->rmax) return -3;
