@@ -228,6 +228,7 @@ struct dsm_sgt {
     int lb_nl = 0, lb_f = 0;
     std::vector<SynthJob> jobs;
     int njobs_main = 0, njobs_syn = 0, njobs_acc = 0;  // launch order: main, accumulate, SYNN
+    int syn_rows = 0;  // coefficient rows per receiver the TSGT/RSGT jobs read (P-SV 63 / 42, SH 36 / 26)
     PsvPrep prep[2];
     ShPrep shprep[2];
     // catalog output: the records of a run packed on the device into
@@ -520,8 +521,13 @@ static void sh_synth_jobs(dsm_sgt_t *h)
     void *outs[3] = {h->d_cat_t.p, h->d_cat_r.p, h->d_cat_s.p};
     const int ncomp[3] = {10, 5, 5};
     std::vector<SynthJob> main, acc, syn;
+    h->syn_rows = 0;
     for (int k = 0; k < SH_NJOBS; k++) {
         const ShJobDef &d = SH_JOBS[k];
+        // the reference computes RSGT only with rsgtswitch, SYNN only with
+        // synnswitch (SGTsh-MPI.f90:517-550)
+        if ((d.kind == 1 && !h->in.rsgtswitch) || (d.kind == 2 && !h->in.synnswitch)) continue;
+        if (d.kind != 2) h->syn_rows += d.nb;
         SynthJob j{};
         j.comp[0] = d.comp;
         j.comp[1] = -1;
@@ -566,11 +572,16 @@ static void psv_synth_jobs(dsm_sgt_t *h)
     void *outs[3] = {h->d_cat_t.p, h->d_cat_r.p, h->d_cat_s.p};
     const int ncomp[3] = {20, 10, 10};
     int counts[3] = {0, 0, 0};
+    h->syn_rows = 0;
     for (int grp = 0; grp < 3; grp++)
         for (int k = 0; k < PSV_NJOBS; k++) {
             const PsvJobDef &d = PSV_JOBS[k];
             const int g = (d.kind == 2) ? 2 : d.pass;
             if (g != grp) continue;
+            // RSGT only with rsgtswitch, SYNN only with synnswitch
+            // (SGTpsv-MPI.f90:917-1084)
+            if ((d.kind == 1 && !h->in.rsgtswitch) || (d.kind == 2 && !h->in.synnswitch)) continue;
+            if (d.kind != 2) h->syn_rows += d.nb;
             SynthJob j{};
             j.q0 = d.q0;
             j.nb = d.nb;
@@ -692,8 +703,8 @@ static int sh_create(dsm_sgt_t *h)
     HIPCHK(h, h->d_amp.ensure(sizeof(double) * (size_t)S * h->lcap));
     const size_t np = (size_t)std::max(1, h->ncat);
     HIPCHK(h, h->d_cat_t.ensure(sizeof(double2) * np * R * s.theta_n * 10));
-    HIPCHK(h, h->d_cat_r.ensure(sizeof(double2) * np * R * s.theta_n * 5));
-    HIPCHK(h, h->d_cat_s.ensure(sizeof(double2) * np * s.theta_n * 5));
+    if (h->in.rsgtswitch) HIPCHK(h, h->d_cat_r.ensure(sizeof(double2) * np * R * s.theta_n * 5));
+    if (h->in.synnswitch) HIPCHK(h, h->d_cat_s.ensure(sizeof(double2) * np * s.theta_n * 5));
     // depth cut of the mixed forms (DSM_SGT_DEPTH_CUT overrides; 0 disables):
     // node radius and smaller shear speed per band row
     h->dcut = (h->precision == DSM_SGT_FP64) ? 0.0 : 40.0;
@@ -976,17 +987,19 @@ static int sh_run_batch(dsm_sgt_t *h, const ShPrep &p)
     ga.r_eff = 1;
     ga.jobs = h->d_jobs.as<SynthJob>() + h->njobs_main + h->njobs_acc;
     ga.njobs = h->njobs_syn;
-    TimedLaunch tg2;
-    ev_begin(h, tg2, 3);
-    launch_synth_f64(ga, h->stream);  // SYNN in FP64 in every form (as P-SV)
-    HIPCHK(h, hipGetLastError());
-    ev_end(h, tg2);
+    if (h->njobs_syn) {
+        TimedLaunch tg2;
+        ev_begin(h, tg2, 3);
+        launch_synth_f64(ga, h->stream);  // SYNN in FP64 in every form (as P-SV)
+        HIPCHK(h, hipGetLastError());
+        ev_end(h, tg2);
+    }
     h->stats.synth_launches += h->njobs_acc ? 2 : 1;
     double sumL = 0.0;
     for (int k = 0; k < S; k++) sumL += nl[k];
     // 4 flops per (complex coefficient x real basis) per (r,theta); 36 rows
-    // per receiver
-    h->stats.synth_flops += 4.0 * sumL * s.theta_n * 36.0 * R;
+    // per receiver (26 without RSGT)
+    h->stats.synth_flops += 4.0 * sumL * s.theta_n * (double)h->syn_rows * R;
     // algorithmic bytes of the reference's systems l = 1 .. nl-1: the band
     // (w = 2) of the rows solved and the (value, derivative) of 14
     // right-hand sides at every receiver
@@ -1152,8 +1165,8 @@ static int psv_create(dsm_sgt_t *h)
     HIPCHK(h, h->d_amp.ensure(sizeof(double2) * 2 * (size_t)S * LC));
     const size_t np = (size_t)std::max(1, h->ncat);
     HIPCHK(h, h->d_cat_t.ensure(sizeof(double2) * np * R * s.theta_n * 20));
-    HIPCHK(h, h->d_cat_r.ensure(sizeof(double2) * np * R * s.theta_n * 10));
-    HIPCHK(h, h->d_cat_s.ensure(sizeof(double2) * np * s.theta_n * 10));
+    if (h->in.rsgtswitch) HIPCHK(h, h->d_cat_r.ensure(sizeof(double2) * np * R * s.theta_n * 10));
+    if (h->in.synnswitch) HIPCHK(h, h->d_cat_s.ensure(sizeof(double2) * np * s.theta_n * 10));
     psv_synth_jobs(h);
     {
         // parity of F0..F9 (legendre_lane.h) times (-1)^mu
@@ -1742,11 +1755,13 @@ static int psv_run_batch(dsm_sgt_t *h, const PsvPrep &p, bool full_range)
     ga.r_eff = 1;
     ga.jobs = h->d_jobs.as<SynthJob>() + h->njobs_main + h->njobs_acc;
     ga.njobs = h->njobs_syn;
-    TimedLaunch tg2;
-    ev_begin(h, tg2, 3);
-    launch_synth_f64(ga, h->stream);
-    HIPCHK(h, hipGetLastError());
-    ev_end(h, tg2);
+    if (h->njobs_syn) {
+        TimedLaunch tg2;
+        ev_begin(h, tg2, 3);
+        launch_synth_f64(ga, h->stream);
+        HIPCHK(h, hipGetLastError());
+        ev_end(h, tg2);
+    }
     h->stats.synth_launches += 2;
     std::vector<int> nlr_dev;
     if (h->ns) {  // the l extents the synthesis used, for its issued-flop count
@@ -1770,7 +1785,7 @@ static int psv_run_batch(dsm_sgt_t *h, const PsvPrep &p, bool full_range)
     double sumL = 0.0;
     for (int k = 0; k < S; k++) sumL += nl[k];
     // 4 flops per (complex coefficient x real function) per (r,theta)
-    h->stats.synth_flops += 4.0 * sumL * s.theta_n * (double)PSV_NQ * R;
+    h->stats.synth_flops += 4.0 * sumL * s.theta_n * (double)h->syn_rows * R;
     // algorithmic bytes of the reference's systems l = 1 .. nl-1: the band
     // (w = 4 entries per row) of the rows solved -- from the depth cut's
     // first row of the system's block of 256 degrees down to the surface --
@@ -2210,10 +2225,12 @@ static int fetch_common(dsm_sgt_t *h, int32_t i, int32_t kind, int32_t ir, std::
         src = h->d_cat_t.as<double2>() + ((size_t)pos * num * R + ir) * T;
     } else if (kind == DSM_SGT_RSGT) {
         num = h->ncomp[1];
+        if (!h->in.rsgtswitch) return fail(h, DSM_SGT_ESTATE, "RSGT not computed (rsgtswitch = 0)");
         if (ir < 0 || ir >= R) return fail(h, DSM_SGT_EINVAL, "receiver index out of range");
         src = h->d_cat_r.as<double2>() + ((size_t)pos * num * R + ir) * T;
     } else if (kind == DSM_SGT_SYNN) {
         num = h->ncomp[2];
+        if (!h->in.synnswitch) return fail(h, DSM_SGT_ESTATE, "SYNN not computed (synnswitch = 0)");
         rstride = 1;
         src = h->d_cat_s.as<double2>() + (size_t)pos * num * T;
     } else {
@@ -2345,6 +2362,10 @@ static int pack_catalog(dsm_sgt_t *h, const std::vector<int> &freqs)
         a.src = src[kind]->as<double2>();
         a.ncomp = h->ncomp[kind];
         a.nrec = kind == DSM_SGT_SYNN ? 1 : R;
+        if (!a.src) {  // a kind the switches left out: its part of the block is not written
+            off += (size_t)a.nrec * T * a.ncomp;
+            continue;
+        }
         a.theta_n = T;
         a.pos_list = h->d_packpos.as<int>();
         a.dst = h->d_pack.as<float2>();
@@ -2601,6 +2622,8 @@ int dsm_km_points(dsm_sgt_t *h, dsm_sgt_t *sh, int32_t ir, const dsm_km_params_t
     if (npoints < 0 || (npoints > 0 && (!deltar || !deltas || !trig || !h3 || !h4)))
         return fail(h, DSM_SGT_EINVAL, "bad point arrays");
     if (ir < 0 || ir >= h->r_n) return fail(h, DSM_SGT_EINVAL, "receiver index out of range");
+    if (!h->in.rsgtswitch || (sh && !sh->in.rsgtswitch))
+        return fail(h, DSM_SGT_ESTATE, "the stage reads RSGT records: the engines need rsgtswitch = 1");
     if (prm->compo < 0 || prm->compo > 2) return fail(h, DSM_SGT_EINVAL, "compo must be 0 (Z), 1 (R) or 2 (T)");
     if (!(prm->ipdistance > 0.0)) return fail(h, DSM_SGT_EINVAL, "ipdistance must be positive");
     if (prm->fmin < 0 || prm->fmax < prm->fmin) return fail(h, DSM_SGT_EINVAL, "bad frequency range");

@@ -187,12 +187,13 @@ class Engine:
         self._check(lib().dsm_sgt_fetch_fp64(self._h, i, kind, ir, out.ctypes.data))
         return out
 
-    def fetch_all_fp64(self, i):
-        """(tsgt[theta][r][num_tsgt], rsgt[theta][r][num_rsgt], synn[theta][num_synn])"""
+    def fetch_all_fp64(self, i, kinds=(TSGT, RSGT, SYNN)):
+        """(tsgt[theta][r][num_tsgt], rsgt[theta][r][num_rsgt], synn[theta][num_synn]);
+        a kind not in `kinds` (e.g. switched off) is None."""
         R = self.info.r_n
-        t = np.stack([self.fetch_fp64(i, TSGT, ir) for ir in range(R)], axis=1)
-        r = np.stack([self.fetch_fp64(i, RSGT, ir) for ir in range(R)], axis=1)
-        s = self.fetch_fp64(i, SYNN)
+        t = np.stack([self.fetch_fp64(i, TSGT, ir) for ir in range(R)], axis=1) if TSGT in kinds else None
+        r = np.stack([self.fetch_fp64(i, RSGT, ir) for ir in range(R)], axis=1) if RSGT in kinds else None
+        s = self.fetch_fp64(i, SYNN) if SYNN in kinds else None
         return t, r, s
 
     def psv_band(self, i):

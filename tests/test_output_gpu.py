@@ -11,6 +11,8 @@ C-ABI.
 * dsm_sgt_copy_catalog_async puts the same records in host memory, one
   block per frequency.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -149,3 +151,36 @@ def test_copy_catalog_to_host_memory(mode_name):
         eng.copy_catalog_async(1, 2, small)
     buf.close()
     eng.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode_name", ["PSV", "SH"])
+@pytest.mark.parametrize("prec_name", ["FP64", "BF16X6"])
+def test_switches_skip_rsgt_and_synn(tmp_path, mode_name, prec_name):
+    """rsgtswitch = synnswitch = 0: as the reference, the engine computes
+    neither kind (no catalog, no jobs); TSGT equals a full run's bit for bit,
+    fetching a skipped kind is ESTATE, and only TSGT files are written."""
+    import dsm_sgt
+    p = dict(tlen=1638.4, imax=6, rmin_=5371., rmax_=6371., rdelta_=500., thetamin=0., thetamax=180.,
+             thetadelta=10., maxlmax=500, r0=6100.0)
+    mode, prec = getattr(dsm_sgt, mode_name), getattr(dsm_sgt, prec_name)
+    full = dsm_sgt.Engine(dsm_sgt.DsmSgtInput.from_buffer_copy(di.synthetic_input(**p)), mode=mode, precision=prec)
+    part_in = dsm_sgt.DsmSgtInput.from_buffer_copy(di.synthetic_input(**p, switches=(0, 1, 0, 0)))
+    part = dsm_sgt.Engine(part_in, mode=mode, precision=prec)
+    full.run(1, p["imax"])
+    part.run(1, p["imax"])
+    for i in range(1, p["imax"] + 1):
+        a = full.fetch_all_fp64(i)[0]
+        b = part.fetch_all_fp64(i, kinds=(dsm_sgt.TSGT,))[0]
+        assert np.array_equal(a, b), i
+        assert part.lcut(i) == full.lcut(i)
+    for kind in (dsm_sgt.RSGT, dsm_sgt.SYNN):
+        with pytest.raises(dsm_sgt.DsmSgtError) as e:
+            part.fetch(1, kind, 0)
+        assert e.value.code == 2  # ESTATE
+    part.write_catalogs_async(1, p["imax"], str(tmp_path), "m")
+    part.wait()
+    names = [f for _, _, fs in os.walk(tmp_path) for f in fs]
+    assert names and all("TSGT" in f or f.startswith(("calLog", "list")) for f in names), names[:5]
+    full.close()
+    part.close()
