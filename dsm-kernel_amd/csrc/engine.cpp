@@ -1558,6 +1558,7 @@ static int psv_run_batch(dsm_sgt_t *h, const PsvPrep &p, bool full_range)
         launch_psv_solve(sa, (int)active.size(), h->precision != DSM_SGT_FP64, h->stream);
         HIPCHK(h, hipGetLastError());
         PsvCoefArgs ca{};
+        ca.nq = h->in.rsgtswitch ? PSV_NQ : PSV_NQT;  // the RSGT rows only with rsgtswitch
         ca.xn = h->d_xn.as<double2>();
         ca.slot_list = h->d_slots.as<int>();
         ca.nlb = nlb;

@@ -895,11 +895,12 @@ __global__ __launch_bounds__(64) void psv_coef_b16_kernel(PsvCoefArgs a)
     }
     const size_t qstride = (size_t)a.eb.ns * a.eb.nslot * a.eb.r_n * 2 * a.eb.lcap;
     const int kst = a.kstart ? a.kstart[b / a.nlb * a.nblk + ((lw0 - a.l0) >> 8)] : 1;
+    const int nq = a.nq ? a.nq : PSV_NQ;  // (rows past nq are read by no job)
     for (int r = 0; r < R; r++) {
         if (rneed && !rneed[r]) continue;  // below the solved rows of every slot the tile holds
         unsigned short *base = wave_row_base(a.eb, r, sl, lw0, t);
         if (a.kstart && kst > a.rlow[r]) {  // below this block's solved rows: zero rows
-            for (int q = 0; q < PSV_NQ; q++) wave_put(a.eb, st, base, qstride, q, cz(), t);
+            for (int q = 0; q < nq; q++) wave_put(a.eb, st, base, qstride, q, cz(), t);
             continue;
         }
         const int *rr = a.rec_row + 6 * r;
@@ -913,8 +914,9 @@ __global__ __launch_bounds__(64) void psv_coef_b16_kernel(PsvCoefArgs a)
             z[s].Ud = cadd(cadd(cscale(g0, w1[0]), cscale(g2, w1[1])), cscale(g4, w1[2]));
             z[s].Vd = cadd(cadd(cscale(g1, w1[0]), cscale(g3, w1[1])), cscale(g5, w1[2]));
         }
-        psv_rows(z, a.fluid[r] != 0, a.fluidK[(size_t)sl * R + r], il, a.irf[r], mu1, mu2,
-                 [&](int q, double2 v) { wave_put(a.eb, st, base, qstride, q, live ? v : cz(), t); });
+        psv_rows(z, a.fluid[r] != 0, a.fluidK[(size_t)sl * R + r], il, a.irf[r], mu1, mu2, [&](int q, double2 v) {
+            if (q < nq) wave_put(a.eb, st, base, qstride, q, live ? v : cz(), t);  // (q is uniform)
+        });
     }
 }
 

@@ -26,6 +26,7 @@ static constexpr double DSM_PI = 3.1415926535897932;
 //         l(l+1)c/2 / sqrt(l(l+1)) (icomp=2: +-S, icomp=3: -iS)
 constexpr int PSV_NSOL = 6;
 constexpr int PSV_NQ = 63;   // coefficient rows per receiver (TSGT 42 + RSGT 21)
+constexpr int PSV_NQT = 42;  // the TSGT rows, first
 constexpr int PSV_NQS = 10;  // SYNN rows
 
 // per-slot excitation data: calabnum tables and source-zone moduli
@@ -145,6 +146,7 @@ struct PsvCoefArgs {
     // first row has zero solutions there (not stored by the back substitution)
     const int *kstart, *rlow;
     int nblk;
+    int nq;                // rows written: PSV_NQ, or PSV_NQT without RSGT (0: PSV_NQ)
 };
 
 struct PsvL0Args {
@@ -863,13 +865,14 @@ __host__ __device__ inline void psv_coef_lane(const PsvCoefArgs &a, int b, int t
     }
     double2 *E = a.E + (size_t)sl * PSV_NQ * R * LC + l;
     const int kst = a.kstart ? a.kstart[b / a.nlb * a.nblk + ((l - a.l0) >> 8)] : 1;
+    const int nq = a.nq ? a.nq : PSV_NQ;
     for (int r = 0; r < R; r++) {
         if (rneed && !rneed[r]) continue;  // below the solved rows of every slot the tile holds
         if (a.kstart && kst > a.rlow[r]) {  // below this block's solved rows: zero rows
             if (a.eb.Eb)
-                for (int q = 0; q < PSV_NQ; q++) a.eb.put(q, r, sl, l, cz());
+                for (int q = 0; q < nq; q++) a.eb.put(q, r, sl, l, cz());
             else
-                for (int q = 0; q < PSV_NQ; q++) E[(size_t)(q * R + r) * LC] = cz();
+                for (int q = 0; q < nq; q++) E[(size_t)(q * R + r) * LC] = cz();
             continue;
         }
         const int *rr = a.rec_row + 6 * r;
@@ -885,10 +888,14 @@ __host__ __device__ inline void psv_coef_lane(const PsvCoefArgs &a, int b, int t
         }
         if (a.eb.Eb)
             psv_rows(z, a.fluid[r] != 0, a.fluidK[(size_t)sl * R + r], il, a.irf[r], mu1, mu2,
-                     [&](int q, double2 v) { a.eb.put(q, r, sl, l, v); });
+                     [&](int q, double2 v) {
+                         if (q < nq) a.eb.put(q, r, sl, l, v);
+                     });
         else
             psv_rows(z, a.fluid[r] != 0, a.fluidK[(size_t)sl * R + r], il, a.irf[r], mu1, mu2,
-                     [&](int q, double2 v) { E[(size_t)(q * R + r) * LC] = v; });
+                     [&](int q, double2 v) {
+                         if (q < nq) E[(size_t)(q * R + r) * LC] = v;
+                     });
     }
 }
 
