@@ -946,11 +946,13 @@ static int sh_run_batch(dsm_sgt_t *h, const ShPrep &p)
     HIPCHK(h, hipGetLastError());
     int rc = split_rows(h, h->d_E, S, 36, R, h->d_eb);
     if (rc) return rc;
+    std::vector<int> nlr;  // [r][slot] l extents of the bf16 rows
     if (h->ns) {
-        std::vector<int> nlr((size_t)R * S);
+        nlr.resize((size_t)R * S);
         for (int r = 0; r < R; r++)
             for (int k = 0; k < S; k++) nlr[(size_t)r * S + k] = nl[k];
         HIPCHK(h, hipMemcpyAsync(h->d_nlr.p, nlr.data(), sizeof(int) * nlr.size(), hipMemcpyHostToDevice, h->stream));
+        h->stats.synth_mfma_flops += b16_issued_flops(h, nlr, S, R);
     }
     ev_end(h, tz);
 
