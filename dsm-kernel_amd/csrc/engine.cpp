@@ -601,6 +601,28 @@ static void psv_synth_jobs(dsm_sgt_t *h)
     h->njobs_syn = counts[2];
 }
 
+// The device catalog of ncat frequencies (FP64, the kinds the switches
+// keep) must fit in the device's free memory with the reserve: without a
+// window, say which window would fit rather than fail in an allocation.
+static int check_catalog_fits(dsm_sgt_t *h, int R, int T, const int ncomp[3])
+{
+    const double per = 16.0 * T * ((double)R * (ncomp[0] + (h->in.rsgtswitch ? ncomp[1] : 0)) +
+                                   (h->in.synnswitch ? ncomp[2] : 0));
+    size_t fr = 0, tot = 0;
+    if (hipMemGetInfo(&fr, &tot) != hipSuccess) return DSM_SGT_OK;
+    if (per * h->ncat <= (double)fr - (double)kHbmReserve) return DSM_SGT_OK;
+    buf_cache().flush(h->device);  // blocks kept from destroyed engines count as free
+    if (hipMemGetInfo(&fr, &tot) != hipSuccess) return DSM_SGT_OK;
+    const double avail = (double)fr - (double)kHbmReserve;
+    if (per * h->ncat <= avail) return DSM_SGT_OK;
+    const int w = avail > 0 ? (int)(0.5 * avail / per) : 0;  // (half: the batch buffers need the rest)
+    return fail(h, DSM_SGT_ENOMEM,
+                "the device catalog of " + std::to_string(h->ncat) + " frequencies needs " +
+                    std::to_string((long long)(per * h->ncat / 1e9)) + " GB of the " +
+                    std::to_string((long long)(fr / 1000000000ULL)) + " GB free: run in catalog windows (" +
+                    "dsm_sgt_tuning_t.catalog_window" + (w > 0 ? " <= " + std::to_string(w) : std::string()) + ")");
+}
+
 // Catalog positions without a window: frequency i at i - ipos0.
 static void init_catpos(dsm_sgt_t *h)
 {
@@ -650,6 +672,10 @@ static int sh_create(dsm_sgt_t *h)
     h->logged.assign(h->npos, 0);
     h->ncat = h->tune.catalog_window > 0 ? std::min(h->tune.catalog_window, std::max(1, h->npos)) : h->npos;
     init_catpos(h);
+    {
+        const int nc[3] = {10, 5, 5};
+        if (const int rc = check_catalog_fits(h, R, s.theta_n, nc)) return rc;
+    }
 
     std::vector<double> th(s.theta_n);
     for (int i = 0; i < s.theta_n; i++) th[i] = s.theta_deg[i] / 180.0 * PI;
@@ -1053,6 +1079,10 @@ static int psv_create(dsm_sgt_t *h)
     h->logged.assign(h->npos, 0);
     h->ncat = h->tune.catalog_window > 0 ? std::min(h->tune.catalog_window, std::max(1, h->npos)) : h->npos;
     init_catpos(h);
+    {
+        const int nc[3] = {20, 10, 10};
+        if (const int rc = check_catalog_fits(h, R, s.theta_n, nc)) return rc;
+    }
 
     std::vector<double> th(s.theta_n);
     for (int i = 0; i < s.theta_n; i++) th[i] = s.theta_deg[i] / 180.0 * PI;
@@ -2848,6 +2878,13 @@ int dsm_km_vectorfft_noalloc(dsm_sgt_t *h, void *stream, int32_t nvec, int32_t i
     if (!rvec || !device_span(rvec, sizeof(double) * nout, h->device, why))
         return fail(h, DSM_SGT_EINVAL, "rvec " + (rvec ? why : std::string("is NULL")));
     hipStream_t s = stream ? (hipStream_t)stream : h->stream;
+    if (stream) {  // the caller's stream must belong to the engine's GPU
+        hipDevice_t sd = -1;
+        if (hipStreamGetDevice(s, &sd) != hipSuccess || sd != h->device) {
+            (void)hipGetLastError();
+            return fail(h, DSM_SGT_EINVAL, "stream is not a stream of the engine's GPU");
+        }
+    }
     launch_km_twiddles(h->km_static_tw, N, s);
     KmFftArgs fa{};
     fa.ccvec = (const double2 *)ccvec;

@@ -27,3 +27,16 @@ def test_valid_runs_report_clean_status(mode, prec):
     st = eng.stats()
     assert st.nonfinite == 0 and st.small_pivots == 0, (st.nonfinite, st.small_pivots)
     eng.close()
+
+
+def test_catalog_larger_than_memory_asks_for_a_window():
+    """A catalog of ~400 GB (1311 frequencies x 178 radii x 3601 distances)
+    without a catalog window is refused at create, before the large
+    allocations, with ENOMEM naming the window that would fit."""
+    import dsm_sgt
+    p = dict(tlen=3276.8, imax=1311, rmin_=3716., rmax_=6371., rdelta_=15., thetamin=0., thetamax=180.,
+             thetadelta=0.05, maxlmax=600)
+    inp = dsm_sgt.DsmSgtInput.from_buffer_copy(di.synthetic_input(**p))
+    with pytest.raises(dsm_sgt.DsmSgtError) as e:
+        dsm_sgt.Engine(inp, mode=dsm_sgt.PSV, precision=dsm_sgt.BF16X6)
+    assert e.value.code == 4 and "catalog_window <=" in str(e.value), e.value
