@@ -136,6 +136,7 @@ struct DevBuf {
 struct TimedLaunch {
     hipEvent_t a, b;
     int kind;  // 0 solve, 1 legendre, 2 synth, 3 other
+    hipStream_t s;
 };
 
 }  // namespace
@@ -168,6 +169,11 @@ struct dsm_sgt {
     // (omega, l >= 1) solves (default priority, like the engine stream)
     hipStream_t stream2 = nullptr;
     hipEvent_t ev_l0in = nullptr, ev_l0out = nullptr;
+    // the Legendre tables are computed on stream2 beside the host set-up and
+    // the first batch's solves; the engine stream waits on ev_tab before its
+    // first synthesis (tab_wait: that wait is still to be enqueued)
+    hipEvent_t ev_tab = nullptr;
+    bool tab_wait = false;
     std::string err;
     dsm_sgt_input_t in{};
     ShSetup sh;
@@ -270,18 +276,19 @@ static int fail(dsm_sgt_t *h, int code, const std::string &msg)
                         std::string(#call) + ": " + hipGetErrorString(e_));                      \
     } while (0)
 
-static void ev_begin(dsm_sgt_t *h, TimedLaunch &t, int kind)
+static void ev_begin(dsm_sgt_t *h, TimedLaunch &t, int kind, hipStream_t s = nullptr)
 {
     t.kind = kind;
+    t.s = s ? s : h->stream;
     if (!h->timing) return;
     (void)hipEventCreate(&t.a);
     (void)hipEventCreate(&t.b);
-    (void)hipEventRecord(t.a, h->stream);
+    (void)hipEventRecord(t.a, t.s);
 }
 static void ev_end(dsm_sgt_t *h, TimedLaunch &t)
 {
     if (!h->timing) return;
-    (void)hipEventRecord(t.b, h->stream);
+    (void)hipEventRecord(t.b, t.s);
     h->pending.push_back(t);
 }
 // Host wall time since t0, in ms.
@@ -300,9 +307,16 @@ static hipError_t dev_sync(dsm_sgt_t *h)
     return e;
 }
 
+// Adds the finished timed launches to the stats (launches on another
+// stream that are still running stay pending).
 static void ev_collect(dsm_sgt_t *h)
 {
+    std::vector<TimedLaunch> left;
     for (auto &t : h->pending) {
+        if (hipEventQuery(t.b) == hipErrorNotReady) {
+            left.push_back(t);
+            continue;
+        }
         float ms = 0.f;
         (void)hipEventElapsedTime(&ms, t.a, t.b);
         if (t.kind == 0) h->stats.solve_ms += ms;
@@ -312,7 +326,27 @@ static void ev_collect(dsm_sgt_t *h)
         (void)hipEventDestroy(t.a);
         (void)hipEventDestroy(t.b);
     }
-    h->pending.clear();
+    h->pending.swap(left);
+}
+
+// End of a create: the table launches are on stream2; ev_tab marks their
+// end.  The engine's own uploads are complete on return.
+static int tables_enqueued(dsm_sgt_t *h)
+{
+    HIPCHK(h, hipEventRecord(h->ev_tab, h->stream2));
+    h->tab_wait = true;
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    ev_collect(h);
+    return DSM_SGT_OK;
+}
+
+// The engine stream's wait for the Legendre tables, enqueued once before
+// the first synthesis (outside any graph capture).
+static hipError_t wait_tables(dsm_sgt_t *h)
+{
+    if (!h->tab_wait) return hipSuccess;
+    h->tab_wait = false;
+    return hipStreamWaitEvent(h->stream, h->ev_tab, 0);
 }
 
 // Synthesis launch in the engine's precision; Eb is the bf16 split of
@@ -383,20 +417,21 @@ static void set_mirror(dsm_sgt_t *h, const int *par)
     h->nhp = ((h->nh + 127) / 128) * 128;
 }
 
-// Tables of the mixed forms: FP32 copy, or the bf16 split (after set_mirror).
-static int make_mixed_tables(dsm_sgt_t *h, size_t nplanes)
+// Tables of the mixed forms: FP32 copy, or the bf16 split (after set_mirror),
+// on stream st.
+static int make_mixed_tables(dsm_sgt_t *h, size_t nplanes, hipStream_t st)
 {
     if (h->precision == DSM_SGT_FP64) return DSM_SGT_OK;
     if (h->ns) {
         HIPCHK(h, h->d_tb.ensure(sizeof(unsigned short) * h->ns * nplanes * (size_t)h->nhp * h->lcap));
         launch_split_table(h->d_tab.as<double>(), (int)nplanes, h->lcap, h->theta_pad, h->nh, h->nhp, h->ns,
-                           h->d_tb.as<unsigned short>(), h->stream);
+                           h->d_tb.as<unsigned short>(), st);
         HIPCHK(h, hipGetLastError());
         return DSM_SGT_OK;
     }
     const size_t n = nplanes * (size_t)h->lcap * h->theta_pad;
     HIPCHK(h, h->d_tab32.ensure(sizeof(float) * n));
-    launch_tab_to_f32(h->d_tab.as<double>(), h->d_tab32.as<float>(), n, h->stream);
+    launch_tab_to_f32(h->d_tab.as<double>(), h->d_tab32.as<float>(), n, st);
     HIPCHK(h, hipGetLastError());
     return DSM_SGT_OK;
 }
@@ -714,6 +749,15 @@ static int sh_create(dsm_sgt_t *h)
     HIPCHK(h, hipMemcpy(h->d_w1.p, s.w1.data(), sizeof(double) * 3 * R, hipMemcpyHostToDevice));
     HIPCHK(h, hipMemcpy(h->d_rinv.p, rinv.data(), sizeof(double) * R, hipMemcpyHostToDevice));
     HIPCHK(h, hipMemcpy(h->d_src.p, &src, sizeof(src), hipMemcpyHostToDevice));
+    {
+        // the FP64 table on stream2, beside the rest of the set-up
+        TimedLaunch t;
+        ev_begin(h, t, 1, h->stream2);
+        launch_legendre_sh(h->d_theta.as<double>(), s.theta_n, h->theta_pad, s.maxlmax, h->lcap,
+                           h->d_tab.as<double>(), h->stream2);
+        HIPCHK(h, hipGetLastError());
+        ev_end(h, t);
+    }
 
     const int S = h->nslot, nn = s.nn;
     HIPCHK(h, h->d_band.ensure(sizeof(double2) * 4 * (size_t)nn * S));
@@ -773,15 +817,12 @@ static int sh_create(dsm_sgt_t *h)
     HIPCHK(h, hipMemcpy(h->d_jobs.p, h->jobs.data(), sizeof(SynthJob) * h->jobs.size(), hipMemcpyHostToDevice));
 
     TimedLaunch t;
-    ev_begin(h, t, 1);
-    launch_legendre_sh(h->d_theta.as<double>(), s.theta_n, h->theta_pad, s.maxlmax, h->lcap, h->d_tab.as<double>(),
-                       h->stream);
-    HIPCHK(h, hipGetLastError());
-    rc = make_mixed_tables(h, 12);
+    ev_begin(h, t, 1, h->stream2);
+    rc = make_mixed_tables(h, 12, h->stream2);
     if (rc) return rc;
     ev_end(h, t);
-    HIPCHK(h, hipStreamSynchronize(h->stream));
-    ev_collect(h);
+    rc = tables_enqueued(h);
+    if (rc) return rc;
     h->ready = true;
     return DSM_SGT_OK;
 }
@@ -998,6 +1039,7 @@ static int sh_run_batch(dsm_sgt_t *h, const ShPrep &p)
     ga.njobs = h->njobs_main;
     ga.mirror = h->mirror;
     ga.nh = h->nh;
+    HIPCHK(h, wait_tables(h));
     TimedLaunch tg;
     ev_begin(h, tg, 2);
     launch_synth(h, ga, h->d_eb, h->d_nlr.as<int>());
@@ -1112,6 +1154,16 @@ static int psv_create(dsm_sgt_t *h)
     HIPCHK(h, hipMemcpy(h->d_noderow.p, s.node_row.data(), sizeof(int) * N0, hipMemcpyHostToDevice));
     HIPCHK(h, hipMemcpy(h->d_nodec1.p, s.node_c1.data(), sizeof(int) * N0, hipMemcpyHostToDevice));
     HIPCHK(h, hipMemcpy(h->d_jsta0.p, s.jsta0.data(), sizeof(int) * R, hipMemcpyHostToDevice));
+    {
+        // the FP64 table on stream2, beside the rest of the set-up
+        TimedLaunch t;
+        ev_begin(h, t, 1, h->stream2);
+        HIPCHK(h, h->d_plm.ensure(sizeof(double) * 4 * (size_t)LC * h->theta_pad));
+        launch_legendre_psv(h->d_theta.as<double>(), s.theta_n, h->theta_pad, s.maxlmax, LC, h->d_plm.as<double>(),
+                            h->d_tab.as<double>(), h->stream2);
+        HIPCHK(h, hipGetLastError());
+        ev_end(h, t);
+    }
 
     HIPCHK(h, h->d_band.ensure(sizeof(double2) * 16 * (size_t)nn * S));
     {
@@ -1223,17 +1275,12 @@ static int psv_create(dsm_sgt_t *h)
     HIPCHK(h, hipMemcpy(h->d_jobs.p, h->jobs.data(), sizeof(SynthJob) * h->jobs.size(), hipMemcpyHostToDevice));
 
     TimedLaunch t;
-    ev_begin(h, t, 1);
-    HIPCHK(h, h->d_plm.ensure(sizeof(double) * 4 * (size_t)LC * h->theta_pad));
-    launch_legendre_psv(h->d_theta.as<double>(), s.theta_n, h->theta_pad, s.maxlmax, LC, h->d_plm.as<double>(),
-                        h->d_tab.as<double>(),
-                        h->stream);
-    HIPCHK(h, hipGetLastError());
-    rc = make_mixed_tables(h, PSV_NPLANES);
+    ev_begin(h, t, 1, h->stream2);
+    rc = make_mixed_tables(h, PSV_NPLANES, h->stream2);
     if (rc) return rc;
     ev_end(h, t);
-    HIPCHK(h, hipStreamSynchronize(h->stream));
-    ev_collect(h);
+    rc = tables_enqueued(h);
+    if (rc) return rc;
     h->ready = true;
     return DSM_SGT_OK;
 }
@@ -1759,6 +1806,7 @@ static int psv_run_batch(dsm_sgt_t *h, const PsvPrep &p, bool full_range)
     ga.njobs = h->njobs_main;
     ga.mirror = h->mirror;
     ga.nh = h->nh;
+    HIPCHK(h, wait_tables(h));
     TimedLaunch tg;
     if (gmode) {
         rc = begin_capture(h, cg);
@@ -1930,7 +1978,8 @@ int dsm_sgt_create_tuned(const dsm_sgt_input_t *in, int32_t mode, int32_t precis
             hipStreamCreateWithFlags(&h->stream_io, hipStreamNonBlocking) != hipSuccess ||
             hipEventCreateWithFlags(&h->ev_pack, hipEventDisableTiming) != hipSuccess ||
             hipEventCreateWithFlags(&h->ev_l0in, hipEventDisableTiming) != hipSuccess ||
-            hipEventCreateWithFlags(&h->ev_l0out, hipEventDisableTiming) != hipSuccess) {
+            hipEventCreateWithFlags(&h->ev_l0out, hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&h->ev_tab, hipEventDisableTiming) != hipSuccess) {
             rc = fail(h, DSM_SGT_EHIP, "hipStreamCreate failed");
             break;
         }
@@ -2937,6 +2986,7 @@ void dsm_sgt_destroy(dsm_sgt_t *h)
     if (h->stream) {
         (void)hipSetDevice(h->device);
         (void)hipStreamSynchronize(h->stream);
+        if (h->stream2) (void)hipStreamSynchronize(h->stream2);
         free_graphs(h);
         ev_collect(h);
         (void)hipStreamDestroy(h->stream);
@@ -2945,6 +2995,7 @@ void dsm_sgt_destroy(dsm_sgt_t *h)
         (void)hipStreamSynchronize(h->stream2);
         (void)hipStreamDestroy(h->stream2);
     }
+    if (h->ev_tab) (void)hipEventDestroy(h->ev_tab);
     if (h->ev_l0in) (void)hipEventDestroy(h->ev_l0in);
     if (h->ev_l0out) {
         (void)hipEventDestroy(h->ev_l0out);
