@@ -208,6 +208,7 @@ struct dsm_sgt {
     int ns = 0;                // bf16 terms per operand (0: not a BF16 form)
     int mirror = 0, nh = 0, nhp = 0;
     DevBuf d_tb, d_eb, d_stash, d_nlr, d_tiles;
+    DevBuf d_kpart;  // split-K partial sums of the SYNN synthesis
     DevBuf d_diag;  // [0] small pivots, [1] non-finite catalog values of the current run
     std::vector<int> tiles_main, tiles_acc;  // host copies of the tile orders (launch sizes)
     int tiles_main_n = 0, tiles_acc_n = 0;
@@ -381,6 +382,28 @@ static void launch_synth(dsm_sgt_t *h, const SynthArgs &ga, const DevBuf &Eb, co
     } else {
         launch_synth_f64(ga, h->stream);
     }
+}
+
+// SYNN synthesis in split K: its launch has a few workgroups per job (two
+// row tiles of 64 frequencies' surface rows), fewer than the device's CUs,
+// each with the whole l series.  Eight parts of the series per theta
+// function, fixed by lcap alone, so that every launch plan gives the same
+// bits.
+static hipError_t synn_split(dsm_sgt_t *h, SynthArgs &gs)
+{
+    const int nk = (h->lcap + 15) / 16;  // chunks of 16 degrees
+    gs.kchunk = (nk + 7) / 8;
+    gs.ksplit = (nk + gs.kchunk - 1) / gs.kchunk;
+    gs.knb = 0;
+    for (int j = 0; j < gs.njobs; j++) {
+        const SynthJob &d = h->jobs[(size_t)(gs.jobs - h->d_jobs.as<SynthJob>()) + j];
+        gs.knb = std::max(gs.knb, d.nb);
+    }
+    gs.kpart = nullptr;
+    if (gs.ksplit <= 1) return hipSuccess;
+    const hipError_t e = h->d_kpart.ensure(sizeof(double) * synth_f64_kpart_size(gs));
+    if (e == hipSuccess) gs.kpart = h->d_kpart.as<double>();
+    return e;
 }
 
 // bf16 split of the coefficient rows E [slot][nq][r_eff][lcap] into Eb:
@@ -1040,6 +1063,15 @@ static int sh_run_batch(dsm_sgt_t *h, const ShPrep &p)
     ga.mirror = h->mirror;
     ga.nh = h->nh;
     HIPCHK(h, wait_tables(h));
+    // SYNN rows (surface node, no receiver dimension): FP64 in every form
+    // (as P-SV)
+    SynthArgs gs = ga;
+    gs.E = h->d_Es.as<double2>();
+    gs.nq = SH_NQS;
+    gs.r_eff = 1;
+    gs.jobs = h->d_jobs.as<SynthJob>() + h->njobs_main + h->njobs_acc;
+    gs.njobs = h->njobs_syn;
+    if (h->njobs_syn) HIPCHK(h, synn_split(h, gs));
     TimedLaunch tg;
     ev_begin(h, tg, 2);
     launch_synth(h, ga, h->d_eb, h->d_nlr.as<int>());
@@ -1051,16 +1083,10 @@ static int sh_run_batch(dsm_sgt_t *h, const ShPrep &p)
         HIPCHK(h, hipGetLastError());
     }
     ev_end(h, tg);
-    // SYNN rows (surface node, no receiver dimension)
-    ga.E = h->d_Es.as<double2>();
-    ga.nq = SH_NQS;
-    ga.r_eff = 1;
-    ga.jobs = h->d_jobs.as<SynthJob>() + h->njobs_main + h->njobs_acc;
-    ga.njobs = h->njobs_syn;
     if (h->njobs_syn) {
         TimedLaunch tg2;
         ev_begin(h, tg2, 3);
-        launch_synth_f64(ga, h->stream);  // SYNN in FP64 in every form (as P-SV)
+        launch_synth_f64(gs, h->stream);
         HIPCHK(h, hipGetLastError());
         ev_end(h, tg2);
     }
@@ -1807,6 +1833,17 @@ static int psv_run_batch(dsm_sgt_t *h, const PsvPrep &p, bool full_range)
     ga.mirror = h->mirror;
     ga.nh = h->nh;
     HIPCHK(h, wait_tables(h));
+    // SYNN rows (10 per frequency, a few percent of the synthesis): FP64
+    // MFMA in every form.  The surface displacement of a shallow source at
+    // far distances is a near-field series that cancels over l, where the
+    // FP32 sums of the mixed forms lose up to 1e-4 of it.
+    SynthArgs gs = ga;
+    gs.E = h->d_Es.as<double2>();
+    gs.nq = PSV_NQS;
+    gs.r_eff = 1;
+    gs.jobs = h->d_jobs.as<SynthJob>() + h->njobs_main + h->njobs_acc;
+    gs.njobs = h->njobs_syn;
+    if (h->njobs_syn) HIPCHK(h, synn_split(h, gs));
     TimedLaunch tg;
     if (gmode) {
         rc = begin_capture(h, cg);
@@ -1827,19 +1864,10 @@ static int psv_run_batch(dsm_sgt_t *h, const PsvPrep &p, bool full_range)
     } else {
         ev_end(h, tg);
     }
-    // SYNN rows (10 per frequency, a few percent of the synthesis): FP64
-    // MFMA in every form.  The surface displacement of a shallow source at
-    // far distances is a near-field series that cancels over l, where the
-    // FP32 sums of the mixed forms lose up to 1e-4 of it.
-    ga.E = h->d_Es.as<double2>();
-    ga.nq = PSV_NQS;
-    ga.r_eff = 1;
-    ga.jobs = h->d_jobs.as<SynthJob>() + h->njobs_main + h->njobs_acc;
-    ga.njobs = h->njobs_syn;
     if (h->njobs_syn) {
         TimedLaunch tg2;
         ev_begin(h, tg2, 3);
-        launch_synth_f64(ga, h->stream);
+        launch_synth_f64(gs, h->stream);
         HIPCHK(h, hipGetLastError());
         ev_end(h, tg2);
     }

@@ -78,8 +78,21 @@ struct SynthArgs {
     int njobs;
     int mirror, nh;        // FP64: distance symmetry, nh = (theta_n + 1) / 2 columns computed
     unsigned long long *nonfinite;  // count of non-finite values stored (may be null)
+    // FP64 kernel, split K: ksplit > 1 cuts the l series of every theta
+    // function into ksplit parts of kchunk chunks of 16 degrees, fixed by
+    // the degree (the same parts for every tile, batch size and device),
+    // run by separate workgroups; a second launch adds their partial sums
+    // (kpart, synth_f64_kpart_size doubles) in part order.  knb: the most
+    // theta functions of a job of the launch.
+    int ksplit, kchunk, knb;
+    double *kpart;
+    int kphase;            // set by launch_synth_f64: 0 whole K, 1 partial sums, 2 their sum
 };
 void launch_synth_f64(const SynthArgs &a, hipStream_t s);
+// doubles of SynthArgs::kpart for a launch of a.njobs jobs with a.ksplit parts
+size_t synth_f64_kpart_size(const SynthArgs &a);
+// workgroups of one launch of launch_synth_f64 without split K
+size_t synth_f64_tiles(const SynthArgs &a);
 // Mixed form (synth32.hip): FP32 operands on FP32 MFMA, FP64 accumulation
 // across 32-wide K chunks; tab32 is the theta table rounded to FP32.
 void launch_synth_f32(const SynthArgs &a, const float *tab32, hipStream_t s);

@@ -103,6 +103,16 @@ __device__ __forceinline__ bool xcd_tile(int nx, int ny, int &tx, int &ty, int &
     return true;
 }
 
+// accumulator registers per lane: [1 + MIR][4][2] v4d
+template <int MIR> constexpr int kpart_regs() { return (1 + MIR) * 4 * 2 * 4; }
+
+// Split K (a.kphase 1): workgroup (tile, part) sums the chunks
+// [pp kchunk, (pp + 1) kchunk) of theta function jb = part / ksplit (pp =
+// part % ksplit) and stores its accumulators in kpart; a.kphase 2: one
+// workgroup per tile adds the parts in order and runs the epilogue.  The
+// parts are fixed degree ranges and their sums are added in a fixed order,
+// so the result depends neither on the scheduling nor on the tile's other
+// rows (chunks past a row's series add exact zeros).
 template <int KIND, int MIR>  // KIND 0: receiver rows, 1: surface rows (separate symbol for profiling)
 __global__ __launch_bounds__(512) void synth_f64_kernel(SynthArgs a, int nx, int ny)
 {
@@ -122,7 +132,16 @@ __global__ __launch_bounds__(512) void synth_f64_kernel(SynthArgs a, int nx, int
         for (int sl = s0; sl <= s1; sl++) kmax = max(kmax, a.nl[sl]);
     }
     const int nk = (kmax + BK - 1) / BK;
-    const int niter = job.nb * nk;
+    const int part = (a.kphase == 1) ? (int)blockIdx.y : 0;
+    int it0 = 0, niter = (a.kphase == 2) ? 0 : job.nb * nk;
+    if (a.kphase == 1) {
+        const int jbp = part / a.ksplit, c0 = (part - jbp * a.ksplit) * a.kchunk;
+        const int c1 = min(nk, c0 + a.kchunk);
+        it0 = jbp * nk + c0;
+        niter = (jbp < job.nb && c1 > c0) ? jbp * nk + c1 : it0;
+    }
+    const size_t tile_id = ((size_t)tz * ny + ty) * nx + tx;
+    const int nparts = a.knb * a.ksplit;
     const size_t plane = (size_t)a.lcap * a.theta_pad;
     const int wr = wave >> 2, wc = wave & 3;
 
@@ -136,13 +155,13 @@ __global__ __launch_bounds__(512) void synth_f64_kernel(SynthArgs a, int nx, int
             for (int j = 0; j < 2; j++) acc[p][i][j] = (v4d){0.0, 0.0, 0.0, 0.0};
 
     TileLoad tl;
-    if (niter > 0) {
-        load_tile(a, job, plane, m0, n0, M, R, 0, nk, tid, tl);
+    if (niter > it0) {
+        load_tile(a, job, plane, m0, n0, M, R, it0, nk, tid, tl);
         store_tile<MIR>(As[0], Bs[0], tid, tl);
     }
     __syncthreads();
-    for (int it = 0; it < niter; it++) {
-        const int st = it & 1;
+    for (int it = it0; it < niter; it++) {
+        const int st = (it - it0) & 1;
         if (it + 1 < niter) load_tile(a, job, plane, m0, n0, M, R, it + 1, nk, tid, tl);
 #pragma unroll
         for (int k4 = 0; k4 < BK; k4 += 4) {
@@ -161,6 +180,32 @@ __global__ __launch_bounds__(512) void synth_f64_kernel(SynthArgs a, int nx, int
         }
         if (it + 1 < niter) store_tile<MIR>(As[st ^ 1], Bs[st ^ 1], tid, tl);
         __syncthreads();
+    }
+    constexpr int NREG = kpart_regs<MIR>();
+    if (a.kphase == 1) {  // partial sums of this part, [tile][part][reg][thread]
+        double *kp = a.kpart + (tile_id * nparts + part) * NREG * NT + tid;
+#pragma unroll
+        for (int p = 0; p <= MIR; p++)
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+#pragma unroll
+                for (int j = 0; j < 2; j++)
+#pragma unroll
+                    for (int q = 0; q < 4; q++) kp[(size_t)(((p * 4 + i) * 2 + j) * 4 + q) * NT] = acc[p][i][j][q];
+        return;
+    }
+    if (a.kphase == 2) {  // the parts' sums, in part order
+        for (int pt = 0; pt < nparts; pt++) {
+            const double *kp = a.kpart + (tile_id * nparts + pt) * NREG * NT + tid;
+#pragma unroll
+            for (int p = 0; p <= MIR; p++)
+#pragma unroll
+                for (int i = 0; i < 4; i++)
+#pragma unroll
+                    for (int j = 0; j < 2; j++)
+#pragma unroll
+                        for (int q = 0; q < 4; q++) acc[p][i][j][q] += kp[(size_t)(((p * 4 + i) * 2 + j) * 4 + q) * NT];
+        }
     }
     // epilogue: row = (slot, r, re|im) -> out[pos][comp][r][theta].
     // v_mfma_f64_16x16x4f64 leaves D[i][j] in lane j + 16*(i%4), register i/4.
@@ -209,17 +254,50 @@ __global__ __launch_bounds__(512) void synth_f64_kernel(SynthArgs a, int nx, int
     add_nonfinite(a.nonfinite, bad);
 }
 
-void launch_synth_f64(const SynthArgs &a, hipStream_t s)
+static void synth_f64_geometry(const SynthArgs &a, int &nx, int &ny)
 {
     const int M = a.nslot * a.r_eff * 2;
-    const int nx = a.mirror ? (a.nh + BN - 1) / BN : a.theta_pad / BN, ny = (M + BM - 1) / BM;
-    const dim3 grid(8 * ((nx * ny + 7) / 8) * a.njobs);
+    nx = a.mirror ? (a.nh + BN - 1) / BN : a.theta_pad / BN;
+    ny = (M + BM - 1) / BM;
+}
+
+size_t synth_f64_tiles(const SynthArgs &a)
+{
+    int nx, ny;
+    synth_f64_geometry(a, nx, ny);
+    return (size_t)nx * ny * a.njobs;
+}
+
+size_t synth_f64_kpart_size(const SynthArgs &a)
+{
+    return synth_f64_tiles(a) * (size_t)(a.ksplit > 1 ? a.ksplit * a.knb : 0) * kpart_regs<1>() * NT;
+}
+
+static void synth_f64_launch(const SynthArgs &a, dim3 grid, int nx, int ny, hipStream_t s)
+{
     if (a.mirror) {
         if (a.r_eff > 1) hipLaunchKernelGGL((synth_f64_kernel<0, 1>), grid, dim3(NT), 0, s, a, nx, ny);
         else hipLaunchKernelGGL((synth_f64_kernel<1, 1>), grid, dim3(NT), 0, s, a, nx, ny);
     } else {
         if (a.r_eff > 1) hipLaunchKernelGGL((synth_f64_kernel<0, 0>), grid, dim3(NT), 0, s, a, nx, ny);
         else hipLaunchKernelGGL((synth_f64_kernel<1, 0>), grid, dim3(NT), 0, s, a, nx, ny);
+    }
+}
+
+void launch_synth_f64(const SynthArgs &a, hipStream_t s)
+{
+    int nx, ny;
+    synth_f64_geometry(a, nx, ny);
+    const unsigned g = 8 * ((nx * ny + 7) / 8) * a.njobs;
+    SynthArgs b = a;
+    if (a.ksplit > 1 && a.kpart && a.kchunk > 0 && a.knb > 0) {
+        b.kphase = 1;
+        synth_f64_launch(b, dim3(g, a.ksplit * a.knb), nx, ny, s);
+        b.kphase = 2;
+        synth_f64_launch(b, dim3(g), nx, ny, s);
+    } else {
+        b.kphase = 0;
+        synth_f64_launch(b, dim3(g), nx, ny, s);
     }
 }
 

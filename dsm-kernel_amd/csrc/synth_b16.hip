@@ -168,21 +168,38 @@ __global__ void zero_pad_b16_kernel(u16 *Eb, size_t nrows, int lcap, int lfrom)
 }
 
 // tab [plane][lcap][theta_pad] (FP64) -> Tb [s][plane][nhp][lcap], columns
-// c < nh taken from distance c, the others zero
-__global__ void split_table_kernel(const double *tab, int nplanes, int lcap, int theta_pad, int nh, int nhp, int ns,
-                                   u16 *Tb)
+// c < nh taken from distance c, the others zero.  A transpose through LDS:
+// a workgroup reads 64 degrees x 64 distances of one plane along the
+// distances and writes, per split term and distance, the tile's 32 even
+// and 32 odd degrees (two contiguous runs of the permuted row) along l.
+// lcap is a multiple of 64.
+constexpr int ST_L = 64, ST_C = 64, ST_NT = 256;
+__global__ __launch_bounds__(ST_NT) void split_table_kernel(const double *tab, int lcap, int theta_pad, int nh, int nhp,
+                                                            int ns, u16 *Tb, int nplanes)
 {
-    const size_t total = (size_t)nplanes * nhp * lcap;
-    for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
-         idx += (size_t)gridDim.x * blockDim.x) {
-        const int l = (int)(idx % lcap);
-        size_t rest = idx / lcap;
-        const int c = (int)(rest % nhp);
-        const int p = (int)(rest / nhp);
-        const double v = (c < nh) ? tab[((size_t)p * lcap + l) * theta_pad + c] : 0.0;
-        u16 t[3];
-        bf16_terms(v, ns, t);
-        for (int s = 0; s < ns; s++) Tb[(((size_t)s * nplanes + p) * nhp + c) * lcap + lperm(l, lcap)] = t[s];
+    __shared__ u16 t[3][ST_C][ST_L + 4];
+    const int l0 = blockIdx.x * ST_L, c0 = blockIdx.y * ST_C, p = blockIdx.z;
+    const double *src = tab + ((size_t)p * lcap + l0) * theta_pad + c0;
+    for (int idx = threadIdx.x; idx < ST_L * ST_C; idx += ST_NT) {
+        const int ll = idx / ST_C, cc = idx - ll * ST_C;
+        const double v = (c0 + cc < nh) ? src[(size_t)ll * theta_pad + cc] : 0.0;
+        u16 tr[3];
+        bf16_terms(v, ns, tr);
+        for (int sp = 0; sp < ns; sp++) t[sp][cc][ll] = tr[sp];
+    }
+    __syncthreads();
+    // per (term, distance, parity) 32 degrees = 8 lanes x 4 values
+    const int LH = lcap >> 1;
+    for (int idx = threadIdx.x; idx < ns * ST_C * 2 * 8; idx += ST_NT) {
+        const int q = idx & 7, par = (idx >> 3) & 1, rest = idx >> 4;
+        const int cc = rest % ST_C, sp = rest / ST_C;
+        u16 *dst = Tb + (((size_t)sp * nplanes + p) * nhp + c0 + cc) * lcap + (par ? LH : 0) + (l0 >> 1) + 4 * q;
+        ushort4 w;
+        w.x = t[sp][cc][2 * (4 * q + 0) + par];
+        w.y = t[sp][cc][2 * (4 * q + 1) + par];
+        w.z = t[sp][cc][2 * (4 * q + 2) + par];
+        w.w = t[sp][cc][2 * (4 * q + 3) + par];
+        *(ushort4 *)dst = w;
     }
 }
 
@@ -440,8 +457,10 @@ void launch_zero_pad_b16(unsigned short *Eb, size_t nrows, int lcap, int lfrom, 
 void launch_split_table(const double *tab, int nplanes, int lcap, int theta_pad, int nh, int nhp, int ns,
                         unsigned short *Tb, hipStream_t s)
 {
-    hipLaunchKernelGGL(split_table_kernel, dim3(8192), dim3(256), 0, s, tab, nplanes, lcap, theta_pad, nh, nhp, ns,
-                       Tb);
+    // the tiles read distances c0 .. c0 + 63 < nhp of rows of theta_pad >= nh
+    // values: columns past nh are not read
+    const dim3 g(lcap / ST_L, nhp / ST_C, nplanes);
+    hipLaunchKernelGGL(split_table_kernel, g, dim3(ST_NT), 0, s, tab, lcap, theta_pad, nh, nhp, ns, Tb, nplanes);
 }
 
 size_t synth_b16_grid(int nslot, int r_eff, int nhp, int njobs)
