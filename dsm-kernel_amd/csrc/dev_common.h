@@ -74,6 +74,45 @@ __host__ __device__ inline void bf16_terms(double x, int ns, unsigned short *t)
 #ifdef __HIPCC__  // device code only (the host set-up includes this header too)
 // catalog stores of the synthesis: a non-finite value counts 1
 __device__ __forceinline__ int nonfinite2(double2 v) { return !(__builtin_isfinite(v.x) && __builtin_isfinite(v.y)); }
+// Catalog element idx (complex): FP64 (c64 = 0) or complex64 (c64 = 1, the
+// files' precision: the mixed forms keep their catalog so).  accum adds v
+// into the stored value (the sum in FP64, then rounded).  Returns what was
+// stored.
+__device__ __forceinline__ double2 cat_store(void *out, size_t idx, double2 v, int accum, int c64)
+{
+    if (c64) {
+        float2 *o = (float2 *)out + idx;
+        if (accum) {
+            const float2 w = *o;
+            v = make_double2((double)w.x + v.x, (double)w.y + v.y);
+        }
+        const float2 f = make_float2((float)v.x, (float)v.y);
+        *o = f;
+        return make_double2((double)f.x, (double)f.y);
+    }
+    double2 *o = (double2 *)out + idx;
+    if (accum) {
+        const double2 w = *o;
+        v = make_double2(w.x + v.x, w.y + v.y);
+    }
+    *o = v;
+    return v;
+}
+// The real (reim 0) or imaginary part of catalog element idx, as cat_store.
+__device__ __forceinline__ double cat_store_part(void *out, size_t idx, int reim, double v, int accum, int c64)
+{
+    if (c64) {
+        float *o = (float *)out + 2 * idx + reim;
+        const double s = accum ? (double)*o + v : v;
+        const float f = (float)s;
+        *o = f;
+        return (double)f;
+    }
+    double *o = (double *)out + 2 * idx + reim;
+    const double s = accum ? *o + v : v;
+    *o = s;
+    return s;
+}
 __device__ __forceinline__ void add_nonfinite(unsigned long long *count, int n)
 {
     if (n && count) atomicAdd(count, (unsigned long long)n);

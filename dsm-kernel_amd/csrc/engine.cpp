@@ -183,6 +183,10 @@ struct dsm_sgt {
     bool ready = false;
     int r_n = 0, theta_n = 0, nn = 0, intir0 = 0;
     int ncomp[3] = {0, 0, 0};
+    // the device catalogs hold complex64 (the files' precision) in the
+    // mixed forms, complex FP64 in the plain form
+    int cat64 = 0;
+    size_t cat_el() const { return cat64 ? sizeof(float2) : sizeof(double2); }
     double r0 = 0;
     std::vector<double> radii, theta_deg;
     // geometry of the device arrays
@@ -592,7 +596,8 @@ static void sh_synth_jobs(dsm_sgt_t *h)
         j.sign[0] = 1.0;
         j.sign[1] = 0.0;
         j.ncomp = ncomp[d.kind];
-        j.out = (double2 *)outs[d.kind];
+        j.out = outs[d.kind];
+        j.c64 = h->cat64;
         // split point: the first plane whose parity differs from the first's
         // (the planes of one parity are contiguous in SH_JOBS); the FP32
         // kernel has no distance symmetry
@@ -650,7 +655,8 @@ static void psv_synth_jobs(dsm_sgt_t *h)
             j.sign[1] = (double)d.sign1;
             j.accumulate = d.pass;
             j.ncomp = ncomp[d.kind];
-            j.out = (double2 *)outs[d.kind];
+            j.out = outs[d.kind];
+        j.c64 = h->cat64;
             h->jobs.push_back(j);
             counts[grp]++;
         }
@@ -659,13 +665,13 @@ static void psv_synth_jobs(dsm_sgt_t *h)
     h->njobs_syn = counts[2];
 }
 
-// The device catalog of ncat frequencies (FP64, the kinds the switches
-// keep) must fit in the device's free memory with the reserve: without a
-// window, say which window would fit rather than fail in an allocation.
+// The device catalog of ncat frequencies (the kinds the switches keep) must
+// fit in the device's free memory with the reserve: without a window, say
+// which window would fit rather than fail in an allocation.
 static int check_catalog_fits(dsm_sgt_t *h, int R, int T, const int ncomp[3])
 {
-    const double per = 16.0 * T * ((double)R * (ncomp[0] + (h->in.rsgtswitch ? ncomp[1] : 0)) +
-                                   (h->in.synnswitch ? ncomp[2] : 0));
+    const double per = (double)h->cat_el() * T * ((double)R * (ncomp[0] + (h->in.rsgtswitch ? ncomp[1] : 0)) +
+                                                  (h->in.synnswitch ? ncomp[2] : 0));
     size_t fr = 0, tot = 0;
     if (hipMemGetInfo(&fr, &tot) != hipSuccess) return DSM_SGT_OK;
     if (per * h->ncat <= (double)fr - (double)kHbmReserve) return DSM_SGT_OK;
@@ -795,9 +801,9 @@ static int sh_create(dsm_sgt_t *h)
     HIPCHK(h, h->d_Es.ensure(sizeof(double2) * (size_t)S * 5 * h->lcap));
     HIPCHK(h, h->d_amp.ensure(sizeof(double) * (size_t)S * h->lcap));
     const size_t np = (size_t)std::max(1, h->ncat);
-    HIPCHK(h, h->d_cat_t.ensure(sizeof(double2) * np * R * s.theta_n * 10));
-    if (h->in.rsgtswitch) HIPCHK(h, h->d_cat_r.ensure(sizeof(double2) * np * R * s.theta_n * 5));
-    if (h->in.synnswitch) HIPCHK(h, h->d_cat_s.ensure(sizeof(double2) * np * s.theta_n * 5));
+    HIPCHK(h, h->d_cat_t.ensure(h->cat_el() * np * R * s.theta_n * 10));
+    if (h->in.rsgtswitch) HIPCHK(h, h->d_cat_r.ensure(h->cat_el() * np * R * s.theta_n * 5));
+    if (h->in.synnswitch) HIPCHK(h, h->d_cat_s.ensure(h->cat_el() * np * s.theta_n * 5));
     // depth cut of the mixed forms (DSM_SGT_DEPTH_CUT overrides; 0 disables):
     // node radius and smaller shear speed per band row
     h->dcut = (h->precision == DSM_SGT_FP64) ? 0.0 : 40.0;
@@ -1274,9 +1280,9 @@ static int psv_create(dsm_sgt_t *h)
     HIPCHK(h, h->d_Es.ensure(sizeof(double2) * (size_t)S * PSV_NQS * LC));
     HIPCHK(h, h->d_amp.ensure(sizeof(double2) * 2 * (size_t)S * LC));
     const size_t np = (size_t)std::max(1, h->ncat);
-    HIPCHK(h, h->d_cat_t.ensure(sizeof(double2) * np * R * s.theta_n * 20));
-    if (h->in.rsgtswitch) HIPCHK(h, h->d_cat_r.ensure(sizeof(double2) * np * R * s.theta_n * 10));
-    if (h->in.synnswitch) HIPCHK(h, h->d_cat_s.ensure(sizeof(double2) * np * s.theta_n * 10));
+    HIPCHK(h, h->d_cat_t.ensure(h->cat_el() * np * R * s.theta_n * 20));
+    if (h->in.rsgtswitch) HIPCHK(h, h->d_cat_r.ensure(h->cat_el() * np * R * s.theta_n * 10));
+    if (h->in.synnswitch) HIPCHK(h, h->d_cat_s.ensure(h->cat_el() * np * s.theta_n * 10));
     psv_synth_jobs(h);
     {
         // parity of F0..F9 (legendre_lane.h) times (-1)^mu
@@ -1976,6 +1982,7 @@ int dsm_sgt_create_tuned(const dsm_sgt_input_t *in, int32_t mode, int32_t precis
     h->in = *in;
     h->mode = mode;
     h->precision = precision;
+    h->cat64 = precision != DSM_SGT_FP64;
     h->device = device;
     int rc = DSM_SGT_OK;
     do {
@@ -2326,34 +2333,40 @@ static int fetch_common(dsm_sgt_t *h, int32_t i, int32_t kind, int32_t ir, std::
     const int pos = h->catpos[i - h->ipos0];  // device catalog position
     if (pos < 0 || pos >= h->ncat) return fail(h, DSM_SGT_ESTATE, "frequency " + std::to_string(i) + " not resident");
     const int R = h->r_n, T = h->theta_n;
-    // device catalogs: [pos][comp][r][theta] (r only for TSGT, RSGT)
-    const double2 *src;
+    // device catalogs: [pos][comp][r][theta] (r only for TSGT, RSGT), of
+    // complex64 or complex FP64 elements
+    const char *src;
+    const size_t el = h->cat_el();
     int rstride = R;
     if (kind == DSM_SGT_TSGT) {
         num = h->ncomp[0];
         if (ir < 0 || ir >= R) return fail(h, DSM_SGT_EINVAL, "receiver index out of range");
-        src = h->d_cat_t.as<double2>() + ((size_t)pos * num * R + ir) * T;
+        src = h->d_cat_t.as<char>() + el * (((size_t)pos * num * R + ir) * T);
     } else if (kind == DSM_SGT_RSGT) {
         num = h->ncomp[1];
         if (!h->in.rsgtswitch) return fail(h, DSM_SGT_ESTATE, "RSGT not computed (rsgtswitch = 0)");
         if (ir < 0 || ir >= R) return fail(h, DSM_SGT_EINVAL, "receiver index out of range");
-        src = h->d_cat_r.as<double2>() + ((size_t)pos * num * R + ir) * T;
+        src = h->d_cat_r.as<char>() + el * (((size_t)pos * num * R + ir) * T);
     } else if (kind == DSM_SGT_SYNN) {
         num = h->ncomp[2];
         if (!h->in.synnswitch) return fail(h, DSM_SGT_ESTATE, "SYNN not computed (synnswitch = 0)");
         rstride = 1;
-        src = h->d_cat_s.as<double2>() + (size_t)pos * num * T;
+        src = h->d_cat_s.as<char>() + el * ((size_t)pos * num * T);
     } else {
         return fail(h, DSM_SGT_EINVAL, "unknown catalog kind");
     }
-    std::vector<double2> cm((size_t)num * T);
+    std::vector<char> cm(el * num * T);
     if (hipSetDevice(h->device) != hipSuccess) return fail(h, DSM_SGT_EHIP, "hipSetDevice failed");
-    HIPCHK(h, hipMemcpy2D(cm.data(), sizeof(double2) * T, src, sizeof(double2) * T * rstride, sizeof(double2) * T, num,
-                          hipMemcpyDeviceToHost));
+    HIPCHK(h, hipMemcpy2D(cm.data(), el * T, src, el * T * rstride, el * T, num, hipMemcpyDeviceToHost));
     // records per distance: [theta][comp]
     buf.resize((size_t)num * T);
+    const float2 *c32 = (const float2 *)cm.data();
+    const double2 *c64 = (const double2 *)cm.data();
     for (int c = 0; c < num; c++)
-        for (int t = 0; t < T; t++) buf[(size_t)t * num + c] = cm[(size_t)c * T + t];
+        for (int t = 0; t < T; t++) {
+            const size_t k = (size_t)c * T + t;
+            buf[(size_t)t * num + c] = h->cat64 ? make_double2((double)c32[k].x, (double)c32[k].y) : c64[k];
+        }
     return DSM_SGT_OK;
 }
 
@@ -2469,7 +2482,8 @@ static int pack_catalog(dsm_sgt_t *h, const std::vector<int> &freqs)
     ev_begin(h, t, 3);
     for (int kind = 0; kind < 3; kind++) {
         PackArgs a{};
-        a.src = src[kind]->as<double2>();
+        a.src = src[kind]->p;
+        a.c64 = h->cat64;
         a.ncomp = h->ncomp[kind];
         a.nrec = kind == DSM_SGT_SYNN ? 1 : R;
         if (!a.src) {  // a kind the switches left out: its part of the block is not written
@@ -2807,13 +2821,15 @@ int dsm_km_points(dsm_sgt_t *h, dsm_sgt_t *sh, int32_t ir, const dsm_km_params_t
                                  h->stream));
     }
     KmRdArgs ra{};
-    ra.cat_t_psv = h->d_cat_t.as<double2>();
-    ra.cat_r_psv = h->d_cat_r.as<double2>();
+    ra.cat_t_psv = h->d_cat_t.p;
+    ra.cat_r_psv = h->d_cat_r.p;
+    ra.c64_psv = h->cat64;
     ra.r_psv = h->r_n;
     ra.ir_psv = ir;
     if (sh && ir_sh >= 0) {
-        ra.cat_t_sh = sh->d_cat_t.as<double2>();
-        ra.cat_r_sh = sh->d_cat_r.as<double2>();
+        ra.cat_t_sh = sh->d_cat_t.p;
+        ra.cat_r_sh = sh->d_cat_r.p;
+        ra.c64_sh = sh->cat64;
         ra.r_sh = sh->r_n;
         ra.ir_sh = ir_sh;
     }

@@ -63,7 +63,8 @@ struct SynthJob {
     double sign[2];        // +-1
     int accumulate;        // add into the targets instead of writing them
     int ncomp;             // record width
-    double2 *out;          // catalog of this kind: [pos][ncomp][r_eff][theta_n]
+    void *out;             // catalog of this kind: [pos][ncomp][r_eff][theta_n], complex FP64 or (c64) complex64
+    int c64;
     double msign;          // F(l, pi - theta) = msign (-1)^l F(l, theta) for all the job's functions
 };
 struct SynthArgs {
@@ -136,11 +137,12 @@ void launch_synth_b16(const SynthB16Args &a, int ns, hipStream_t s);
 
 // ---- catalog records (catalog.hip) ------------------------------------------
 // One kind of the component-major catalog src [pos][ncomp][nrec][theta_n]
-// cast to complex64 in the files' record order: for the k-th position of
+// (complex FP64, cast; or complex64) in the files' record order: for the k-th position of
 // pos_list, dst[k dst_stride + dst_off + (rec theta_n + t) ncomp + c].
 // ncomp <= 20.
 struct PackArgs {
-    const double2 *src;
+    const void *src;       // complex FP64, or complex64 with c64
+    int c64;
     int ncomp, nrec, theta_n;
     const int *pos_list;
     float2 *dst;

@@ -9,10 +9,10 @@ namespace dsm {
 // rdsgtomega of the RSGT and TSGT records of one receiver radius for nf
 // frequencies (fmin + j): om_t[(j T + t) 20 + c], om_r[(j T + t) 10 + c]
 struct KmRdArgs {
-    const double2 *cat_t_psv, *cat_r_psv;  // engine catalogs [pos][comp][rec][T]
-    int r_psv, ir_psv;
-    const double2 *cat_t_sh, *cat_r_sh;    // SH engine (null: none)
-    int r_sh, ir_sh;
+    const void *cat_t_psv, *cat_r_psv;     // engine catalogs [pos][comp][rec][T]
+    int r_psv, ir_psv, c64_psv;            // c64: complex64 catalog, else complex FP64
+    const void *cat_t_sh, *cat_r_sh;       // SH engine (null: none)
+    int r_sh, ir_sh, c64_sh;
     const int *pos_psv, *pos_sh;           // [j] catalog position, -1: not read (i = 0)
     const float2 *ph;                      // [j][T] the reference's exp(cmplx(0, -2 pi i / tlen redtime))
     int nf, theta_n;

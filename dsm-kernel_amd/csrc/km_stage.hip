@@ -37,6 +37,16 @@ __device__ __forceinline__ double2 rd_term(double2 v, double2 ph)
 }
 }  // namespace
 
+// Catalog element idx: complex64 (c64) or complex FP64
+__device__ __forceinline__ double2 cat_value(const void *cat, size_t idx, int c64)
+{
+    if (c64) {
+        const float2 f = ((const float2 *)cat)[idx];
+        return make_double2((double)f.x, (double)f.y);
+    }
+    return ((const double2 *)cat)[idx];
+}
+
 // One lane per (frequency j, distance t): TSGT into om_t[(j T + t) 20 + c],
 // RSGT into om_r[(j T + t) 10 + c]; SH records added into their P-SV slots
 // (TSGT_SH 1 -> 7, 2..10 -> 12..20; RSGT_SH 1..5 -> 6..10).
@@ -57,29 +67,29 @@ __global__ __launch_bounds__(256) void km_rd_kernel(KmRdArgs a)
     double2 vt[20], vr[10];
     {
         const size_t cs = (size_t)a.r_psv * T;
-        const double2 *ct = a.cat_t_psv + ((size_t)pos * 20 * a.r_psv + a.ir_psv) * T + t;
-        const double2 *cr = a.cat_r_psv + ((size_t)pos * 10 * a.r_psv + a.ir_psv) * T + t;
+        const size_t ct = ((size_t)pos * 20 * a.r_psv + a.ir_psv) * T + t;
+        const size_t cr = ((size_t)pos * 10 * a.r_psv + a.ir_psv) * T + t;
         for (int c = 0; c < 20; c++) {
-            const double2 v = rd_term(ct[c * cs], ph);
+            const double2 v = rd_term(cat_value(a.cat_t_psv, ct + c * cs, a.c64_psv), ph);
             vt[c] = make_double2(0.0 + v.x, 0.0 + v.y);
         }
         for (int c = 0; c < 10; c++) {
-            const double2 v = rd_term(cr[c * cs], ph);
+            const double2 v = rd_term(cat_value(a.cat_r_psv, cr + c * cs, a.c64_psv), ph);
             vr[c] = make_double2(0.0 + v.x, 0.0 + v.y);
         }
     }
     if (a.cat_t_sh && a.pos_sh[j] >= 0) {
         const int ps = a.pos_sh[j];
         const size_t cs = (size_t)a.r_sh * T;
-        const double2 *ct = a.cat_t_sh + ((size_t)ps * 10 * a.r_sh + a.ir_sh) * T + t;
-        const double2 *cr = a.cat_r_sh + ((size_t)ps * 5 * a.r_sh + a.ir_sh) * T + t;
+        const size_t ct = ((size_t)ps * 10 * a.r_sh + a.ir_sh) * T + t;
+        const size_t cr = ((size_t)ps * 5 * a.r_sh + a.ir_sh) * T + t;
         for (int c = 0; c < 10; c++) {
-            const double2 v = rd_term(ct[c * cs], ph);
+            const double2 v = rd_term(cat_value(a.cat_t_sh, ct + c * cs, a.c64_sh), ph);
             const int slot = c == 0 ? 6 : 10 + c;
             vt[slot] = make_double2(vt[slot].x + v.x, vt[slot].y + v.y);
         }
         for (int c = 0; c < 5; c++) {
-            const double2 v = rd_term(cr[c * cs], ph);
+            const double2 v = rd_term(cat_value(a.cat_r_sh, cr + c * cs, a.c64_sh), ph);
             vr[5 + c] = make_double2(vr[5 + c].x + v.x, vr[5 + c].y + v.y);
         }
     }

@@ -209,7 +209,7 @@ __global__ __launch_bounds__(512) void synth_f64_kernel(SynthArgs a, int nx, int
     }
     // epilogue: row = (slot, r, re|im) -> out[pos][comp][r][theta].
     // v_mfma_f64_16x16x4f64 leaves D[i][j] in lane j + 16*(i%4), register i/4.
-    double *out = (double *)job.out;
+    void *out = job.out;
     const int ntgt = job.comp[1] >= 0 ? 2 : 1;
     int bad = 0;
 #pragma unroll
@@ -228,24 +228,17 @@ __global__ __launch_bounds__(512) void synth_f64_kernel(SynthArgs a, int nx, int
                 const int mcol = a.theta_n - 1 - col;  // (MIR) the mirrored distance
                 for (int t = 0; t < ntgt; t++) {
                     const size_t base = ((pos * job.ncomp + job.comp[t]) * R + r) * a.theta_n;
-                    double *o = out + 2 * (base + col) + reim;
                     if (!MIR) {
                         const double v = job.sign[t] * acc[0][i][j][q];
-                        const double s = job.accumulate ? *o + v : v;
-                        *o = s;
-                        bad += !__builtin_isfinite(s);
+                        bad += !__builtin_isfinite(cat_store_part(out, base + col, reim, v, job.accumulate, job.c64));
                     } else {
                         const double e = acc[0][i][j][q], od = acc[MIR][i][j][q];
                         const double v = job.sign[t] * (e + od);
-                        const double s = job.accumulate ? *o + v : v;
-                        *o = s;
-                        bad += !__builtin_isfinite(s);
+                        bad += !__builtin_isfinite(cat_store_part(out, base + col, reim, v, job.accumulate, job.c64));
                         if (mcol != col) {
-                            double *om = out + 2 * (base + mcol) + reim;
                             const double w = job.sign[t] * (job.msign * (e - od));
-                            const double sw = job.accumulate ? *om + w : w;
-                            *om = sw;
-                            bad += !__builtin_isfinite(sw);
+                            bad += !__builtin_isfinite(
+                                cat_store_part(out, base + mcol, reim, w, job.accumulate, job.c64));
                         }
                     }
                 }

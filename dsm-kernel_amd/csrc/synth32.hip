@@ -152,7 +152,7 @@ __global__ __launch_bounds__(512) void synth_f32_kernel(SynthArgs a, const float
     // epilogue: v_mfma_f32_16x16x4f32 leaves D[i][j] in lane j + 16*(i/4), register i%4
     // out[pos][comp][r][theta] (complex): registers 2h, 2h+1 are re, im of
     // one (slot, r) row, 16 lanes write 16 consecutive distances
-    double2 *out = job.out;
+    void *out = job.out;
     const int ntgt = job.comp[1] >= 0 ? 2 : 1;
     int bad = 0;
 #pragma unroll
@@ -169,11 +169,9 @@ __global__ __launch_bounds__(512) void synth_f32_kernel(SynthArgs a, const float
                 const int col = n0 + wc * 32 + j * 16 + (lane & 15);
                 if (col >= a.theta_n) continue;
                 for (int t = 0; t < ntgt; t++) {
-                    double2 *o = out + ((pos * job.ncomp + job.comp[t]) * R + r) * a.theta_n + col;
+                    const size_t o = ((pos * job.ncomp + job.comp[t]) * R + r) * a.theta_n + col;
                     const double2 v = make_double2(job.sign[t] * acc64[i][j][2 * h2], job.sign[t] * acc64[i][j][2 * h2 + 1]);
-                    const double2 w = job.accumulate ? make_double2(o->x + v.x, o->y + v.y) : v;
-                    *o = w;
-                    bad += nonfinite2(w);
+                    bad += nonfinite2(cat_store(out, o, v, job.accumulate, job.c64));
                 }
             }
         }

@@ -362,8 +362,8 @@ __device__ __forceinline__ void synth_b16_tile(const SynthB16Args &a, const Synt
     // register i%4: registers 2h, 2h+1 are re, im of one (receiver, slot)
     // row, written as one complex to out[pos][comp][r][theta], 16 lanes on
     // 16 consecutive distances (the mirror columns likewise, descending)
-    double2 *out = job->out;
-    const int ncomp = job->ncomp, accum = job->accumulate;
+    void *out = job->out;
+    const int ncomp = job->ncomp, accum = job->accumulate, c64 = job->c64;
     const int comp0 = job->comp[0], comp1 = job->comp[1];
     const double sign0 = job->sign[0], sign1 = job->sign[1], msign = job->msign;
     int bad = 0;
@@ -391,16 +391,12 @@ __device__ __forceinline__ void synth_b16_tile(const SynthB16Args &a, const Synt
                     const int comp = t ? comp1 : comp0;
                     if (comp < 0) continue;
                     const double sg = t ? sign1 : sign0;
-                    double2 *od = out + ((pos * ncomp + comp) * R + r) * a.theta_n;
+                    const size_t od = ((pos * ncomp + comp) * R + r) * a.theta_n;
                     const double2 v = make_double2(sg * (e0 + o0), sg * (e1 + o1));
-                    const double2 sv = accum ? make_double2(od[col].x + v.x, od[col].y + v.y) : v;
-                    od[col] = sv;
-                    bad += nonfinite2(sv);
+                    bad += nonfinite2(cat_store(out, od + col, v, accum, c64));
                     if (mir) {
                         const double2 w = make_double2(sg * msign * (e0 - o0), sg * msign * (e1 - o1));
-                        const double2 sw = accum ? make_double2(od[mc].x + w.x, od[mc].y + w.y) : w;
-                        od[mc] = sw;
-                        bad += nonfinite2(sw);
+                        bad += nonfinite2(cat_store(out, od + mc, w, accum, c64));
                     }
                 }
             }

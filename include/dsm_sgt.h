@@ -188,7 +188,9 @@ int dsm_sgt_run_list(dsm_sgt_t *h, const int32_t *idx, int32_t n);
  * array complex(4) out(num, theta_n), exactly the record the reference
  * writes (SGTsh-MPI.f90:577-611, SGTpsv-MPI.f90:1113-1149). */
 int dsm_sgt_fetch(dsm_sgt_t *h, int32_t i, int32_t kind, int32_t ir, void *out);
-/* Same record before the complex64 cast: complex(8) out(num, theta_n). */
+/* Same record as complex(8) out(num, theta_n): the plain FP64 form's
+ * accumulators before the complex64 cast; the mixed forms keep their
+ * device catalog in complex64 (the files' precision), widened here. */
 int dsm_sgt_fetch_fp64(dsm_sgt_t *h, int32_t i, int32_t kind, int32_t ir, void *out);
 /* The l cut-off logged by the reference for frequency i (its llog-1,
  * SGTsh-MPI.f90:559, SGTpsv-MPI.f90:1098). */
@@ -208,7 +210,8 @@ int dsm_sgt_write_catalog(dsm_sgt_t *h, int32_t i, const char *output_dir, const
 /* Catalog output of a whole run (the write loop SGTpsv-MPI.f90:1103-1150,
  * SGTsh-MPI.f90:567-612, for every frequency of [i_begin, i_end] except
  * i = 0; each must be computed and resident).  The records are cast to
- * complex64 and put in the files' order on the device; the call returns
+ * complex64 (plain form; the mixed forms' catalog is complex64) and put in
+ * the files' order on the device; the call returns
  * once that is queued, and a copy stream and a writer thread move the
  * records to the host and write the files beside the engine's next
  * dsm_sgt_run.  Files, names and bytes are those of

@@ -30,11 +30,11 @@ def test_valid_runs_report_clean_status(mode, prec):
 
 
 def test_catalog_larger_than_memory_asks_for_a_window():
-    """A catalog of ~400 GB (1311 frequencies x 178 radii x 3601 distances)
-    without a catalog window is refused at create, before the large
-    allocations, with ENOMEM naming the window that would fit."""
+    """A complex64 catalog of ~400 GB (1311 frequencies x 355 radii x 3601
+    distances) without a catalog window is refused at create, before the
+    large allocations, with ENOMEM naming the window that would fit."""
     import dsm_sgt
-    p = dict(tlen=3276.8, imax=1311, rmin_=3716., rmax_=6371., rdelta_=15., thetamin=0., thetamax=180.,
+    p = dict(tlen=3276.8, imax=1311, rmin_=3716., rmax_=6371., rdelta_=7.5, thetamin=0., thetamax=180.,
              thetadelta=0.05, maxlmax=600)
     inp = dsm_sgt.DsmSgtInput.from_buffer_copy(di.synthetic_input(**p))
     with pytest.raises(dsm_sgt.DsmSgtError) as e:
