@@ -308,6 +308,33 @@ def band_solve(mode, form, st, full=True):
     return out
 
 
+def vendor_gemm(shapes, reps=10):
+    """bf16 GEMM rate of the vendor library (torch.matmul -> hipBLASLt) on
+    this device, fp32 accumulation, for (M, N, K) shapes: the practical MFMA
+    rate a plain bf16 GEMM of the synthesis's shape reaches here, beside the
+    spec-sheet peak."""
+    import torch
+    out = []
+    for (m, n, k) in shapes:
+        a = torch.randn(m, k, device="cuda", dtype=torch.bfloat16)
+        b = torch.randn(k, n, device="cuda", dtype=torch.bfloat16)
+        for _ in range(3):
+            c = a @ b
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            c = a @ b
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        out.append({"shape": [m, n, k], "ms": ms, "tflops": 2.0 * m * n * k / ms / 1e9})
+        del a, b, c
+    torch.cuda.empty_cache()
+    return out
+
+
 def profile_traffic(mode, form, plan):
     """HBM bytes per launch of the dominant kernel for this workload, form
     and launch plan, from the rocprof summaries under profiles/: a line
@@ -442,6 +469,9 @@ def output_legs(inp, mode, prec, plan, imax, out_dir=None):
 
 
 def main():
+    if len(sys.argv) == 3 and sys.argv[1] == "--vendor-gemm":
+        print(json.dumps(vendor_gemm([tuple(x) for x in json.loads(sys.argv[2])])), flush=True)
+        return 0
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
@@ -463,6 +493,8 @@ def main():
                     help="skip the catalog output legs (host memory, files)")
     ap.add_argument("--out-dir", default=None, help="directory for the files leg's scratch catalog")
     ap.add_argument("--no-km", action="store_true", help="skip the KernelMaker stage leg")
+    ap.add_argument("--no-vendor", action="store_true",
+                    help="skip the vendor bf16 GEMM (torch.matmul) calibration of the MFMA roofline")
     ap.add_argument("--workload", choices=("catalog", "c5"), default="catalog",
                     help="catalog: the 5 s P-SV (or --mode sh 10 s SH) catalog; c5: 16 source depths at 2.5 s, "
                          "P-SV + SH")
@@ -621,6 +653,22 @@ def main():
         "plan": eng_plan,
         "band_solve": band_solve(args.mode, args.form, st, not args.imax),
     }
+    if not args.no_vendor and args.form in ("bf16x6", "bf16x3"):
+        # the synthesis GEMM's rows (128 frequencies x receivers x re|im),
+        # its computed distance columns and a two-function l series; and a
+        # large square GEMM.  In a child process: torch's HIP runtime does
+        # not open the device beside this process's.
+        import subprocess
+        shapes = [(256 * r_n, 1024, 16384), (16384, 16384, 16384)]
+        res = subprocess.run([sys.executable, os.path.abspath(__file__), "--vendor-gemm", json.dumps(shapes)],
+                             capture_output=True, text=True, timeout=300)
+        if res.returncode != 0:
+            raise RuntimeError("vendor GEMM leg failed: " + res.stderr[-2000:])
+        g = json.loads(res.stdout.strip().splitlines()[-1])
+        out["roofline"]["vendor_bf16_gemm"] = {
+            "note": "torch.matmul (hipBLASLt) bf16 in, fp32 accumulation, on this device in the same run",
+            "runs": g, "frac_of_peak": [x["tflops"] * 1e12 / peak for x in g],
+            "issued_over_vendor_same_shape": issued_rate / (g[0]["tflops"] * 1e12)}
     if km is not None:
         out["kernelmaker_stage"] = km
     if not args.no_output:

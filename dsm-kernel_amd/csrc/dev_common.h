@@ -120,7 +120,19 @@ __device__ __forceinline__ void add_nonfinite(unsigned long long *count, int n)
 #endif
 // degree order of the bf16 operands: even l first, then odd l
 __host__ __device__ inline int lperm(int l, int lcap) { return (l & 1) ? (lcap >> 1) + (l >> 1) : (l >> 1); }
-// Destination of the bf16 coefficient rows: Eb [q][ns][m][lcap] with
+// Layout of the bf16 operands of the synthesis: planes of M rows of lcap
+// degree positions (lperm), stored chunk-major -- the 32 positions of one
+// chunk of every row of the plane together, [plane][lcap / 32][M][32] -- so
+// that one K step of a synthesis tile (16 rows x 32 degrees per wave load)
+// is 1 KiB of contiguous memory.  lcap is a multiple of 64 (the even and
+// the odd halves start on a chunk); a buffer of one-degree rows (lcap 1)
+// is a plain [plane][M] array.
+__host__ __device__ inline size_t b16_at(size_t plane, size_t M, size_t m, int p, int lcap)
+{
+    if (lcap < 32) return (plane * M + m) * lcap + p;
+    return ((plane * (size_t)(lcap >> 5) + (p >> 5)) * M + m) * 32 + (p & 31);
+}
+// Destination of the bf16 coefficient rows: Eb, planes q ns + term of
 // receiver-major rows m = (r nslot + slot) 2 + re|im.
 struct B16Rows {
     unsigned short *Eb;
@@ -134,9 +146,9 @@ struct B16Rows {
         bf16_terms(v.x, ns, tr);
         bf16_terms(v.y, ns, ti);
         for (int s = 0; s < ns; s++) {
-            unsigned short *row = Eb + (((size_t)q * ns + s) * M + m) * lcap + lp;
-            row[0] = tr[s];
-            row[lcap] = ti[s];
+            const size_t pl = (size_t)q * ns + s;
+            Eb[b16_at(pl, M, m, lp, lcap)] = tr[s];
+            Eb[b16_at(pl, M, m + 1, lp, lcap)] = ti[s];
         }
     }
 };

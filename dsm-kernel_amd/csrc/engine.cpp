@@ -560,7 +560,7 @@ static int alloc_b16(dsm_sgt_t *h, int nslot, int nq, int R, int lpad)
         return fail(h, DSM_SGT_EUNSUPPORTED, "too many receivers x frequencies per batch for the bf16 synthesis");
     HIPCHK(h, h->d_eb.ensure(sizeof(unsigned short) * nq * h->ns * M * h->lcap));
     if (lpad >= 0) {
-        launch_zero_pad_b16(h->d_eb.as<unsigned short>(), (size_t)nq * h->ns * M, h->lcap, lpad, h->stream);
+        launch_zero_pad_b16(h->d_eb.as<unsigned short>(), (size_t)nq * h->ns * M, M, h->lcap, lpad, h->stream);
         HIPCHK(h, hipGetLastError());
     } else {
         HIPCHK(h, hipMemset(h->d_eb.p, 0, h->d_eb.n));
@@ -1705,7 +1705,7 @@ static int psv_run_batch(dsm_sgt_t *h, const PsvPrep &p, bool full_range)
         if (h->ns && l0 == 0) {  // the l = 0 rows into degree 0 (the coefficient kernel staged zeros there)
             if (!gmode) HIPCHK(h, hipStreamWaitEvent(h->stream, h->ev_l0out, 0));
             launch_b16_put_l0(h->d_l0row.as<unsigned short>(), h->d_eb.as<unsigned short>(),
-                              (size_t)PSV_NQ * h->ns * S * R * 2, LC, h->stream);
+                              (size_t)PSV_NQ * h->ns * S * R * 2, (size_t)S * R * 2, LC, h->stream);
             HIPCHK(h, hipGetLastError());
         }
         if (dev_cut) {

@@ -103,8 +103,8 @@ void launch_tab_to_f32(const double *src, float *dst, size_t n, hipStream_t s);
 // 3 bf16 terms, degrees stored even-l first; with `mirror` the columns
 // c < nh = (theta_n+1)/2 give distances c and theta_n-1-c.
 struct SynthB16Args {
-    const unsigned short *Eb;  // [q][ns][m][lcap], m = (r nslot + slot) 2 + re|im
-    const unsigned short *Tb;  // [ns][plane][nhp][lcap]
+    const unsigned short *Eb;  // planes q ns + term of rows m = (r nslot + slot) 2 + re|im (b16_at)
+    const unsigned short *Tb;  // planes term nplanes + plane of nhp rows (b16_at)
     int nq, r_eff, nslot, lcap, nplanes;
     int theta_n, nh, nhp, mirror;
     const int *nlr;            // [r][slot]: l extent of every (receiver, slot) row
@@ -121,10 +121,12 @@ void launch_zero_tail_b16(unsigned short *Eb, int nq, int ns, int nslot, int r_e
                           hipStream_t s);
 void launch_split_rows(const double2 *E, int nslot, int nq, int r_eff, int lcap, int ns, unsigned short *Eb,
                        const int *nl, hipStream_t s);
-// degree 0 of nrows bf16 rows [row][lcap] from one-degree rows one[row]
-void launch_b16_put_l0(const unsigned short *one, unsigned short *Eb, size_t nrows, int lcap, hipStream_t s);
-// zero the degrees l in [lfrom, lcap) of nrows bf16 rows
-void launch_zero_pad_b16(unsigned short *Eb, size_t nrows, int lcap, int lfrom, hipStream_t s);
+// degree 0 of nrows bf16 rows (planes of M rows, b16_at) from one-degree
+// rows one[row]
+void launch_b16_put_l0(const unsigned short *one, unsigned short *Eb, size_t nrows, size_t M, int lcap,
+                       hipStream_t s);
+// zero the degrees l in [lfrom, lcap) of nrows bf16 rows (planes of M rows)
+void launch_zero_pad_b16(unsigned short *Eb, size_t nrows, size_t M, int lcap, int lfrom, hipStream_t s);
 void launch_split_table(const double *tab, int nplanes, int lcap, int theta_pad, int nh, int nhp, int ns,
                         unsigned short *Tb, hipStream_t s);
 size_t synth_b16_grid(int nslot, int r_eff, int nhp, int njobs);

@@ -835,7 +835,7 @@ __device__ __forceinline__ unsigned short *wave_row_base(const B16Rows &d, int r
     const int sg = (t >> 2) % (4 * d.ns), c = t & 3;
     const int pl = sg >> 1, hh = sg & 1, s = pl >> 1, reim = pl & 1;
     const size_t M = (size_t)d.nslot * d.r_n * 2, m = ((size_t)r * d.nslot + slot) * 2 + reim;
-    return d.Eb + ((size_t)s * M + m) * d.lcap + (hh ? (d.lcap >> 1) : 0) + (lw0 >> 1) + 8 * c;
+    return d.Eb + b16_at(s, M, m, (hh ? (d.lcap >> 1) : 0) + (lw0 >> 1) + 8 * c, d.lcap);
 }
 // stage row q's terms of lane t's value, then the wave writes the row
 // (lanes t < 16 NS, 16 bytes each at base + q NS M lcap)

@@ -88,8 +88,8 @@ __device__ __forceinline__ bool xcd_tile_b16(int nx, int ny, int &tx, int &ty, i
 
 }  // namespace
 
-// E [slot][q][r][lcap] (FP64 complex) -> Eb [q][s][m][lcap] (bf16 terms,
-// degrees permuted), receiver-major rows m = (r nslot + slot) 2 + re|im,
+// E [slot][q][r][lcap] (FP64 complex) -> Eb, planes q ns + s (bf16 terms,
+// degrees permuted, b16_at), receiver-major rows m = (r nslot + slot) 2 + re|im,
 // for the degrees a synthesis tile reads: l < nl(slot) split, l in
 // [nl(slot), lz) zero, lz = min(lcap, longest nl + 128) (a tile reads its
 // rows' degrees in chunks of 32 even and 32 odd ones up to its longest
@@ -142,33 +142,35 @@ __global__ __launch_bounds__(256) void zero_tail_b16_kernel(u16 *Eb, int nq, int
     if (smin > lz) return;  // every slot runs past what the tiles read
     const size_t rows = (size_t)nq * ns * nslot * R * 2, M = (size_t)nslot * R * 2;
     for (size_t row = blockIdx.x; row < rows; row += gridDim.x) {
-        const int sl = (int)(((row % M) >> 1) % nslot);
-        u16 *p = Eb + row * lcap;
-        for (int l = nl[sl] + (int)threadIdx.x; l <= lz; l += blockDim.x) p[lperm(l, lcap)] = 0;
+        const size_t m = row % M, pl = row / M;
+        const int sl = (int)((m >> 1) % nslot);
+        for (int l = nl[sl] + (int)threadIdx.x; l <= lz; l += blockDim.x) Eb[b16_at(pl, M, m, lperm(l, lcap), lcap)] = 0;
     }
 }
 
-// degree 0 of nrows bf16 rows from one-degree rows (lperm(0, lcap) = 0)
-__global__ void b16_put_l0_kernel(const u16 *one, u16 *Eb, size_t nrows, int lcap)
+// degree 0 of nrows bf16 rows (planes of M rows) from one-degree rows
+// (lperm(0, lcap) = 0)
+__global__ void b16_put_l0_kernel(const u16 *one, u16 *Eb, size_t nrows, size_t M, int lcap)
 {
     for (size_t r = (size_t)blockIdx.x * blockDim.x + threadIdx.x; r < nrows; r += (size_t)gridDim.x * blockDim.x)
-        Eb[r * lcap] = one[r];
+        Eb[b16_at(r / M, M, r % M, 0, lcap)] = one[r];
 }
 
-// zero the padding degrees l in [lfrom, lcap) of nrows rows (never written)
-__global__ void zero_pad_b16_kernel(u16 *Eb, size_t nrows, int lcap, int lfrom)
+// zero the padding degrees l in [lfrom, lcap) of nrows rows (planes of M
+// rows; never written)
+__global__ void zero_pad_b16_kernel(u16 *Eb, size_t nrows, size_t M, int lcap, int lfrom)
 {
     const int np = lcap - lfrom;
     const size_t total = nrows * (size_t)np;
     for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += (size_t)gridDim.x * blockDim.x) {
         const size_t row = idx / np;
         const int l = lfrom + (int)(idx - row * np);
-        Eb[row * lcap + lperm(l, lcap)] = 0;
+        Eb[b16_at(row / M, M, row % M, lperm(l, lcap), lcap)] = 0;
     }
 }
 
-// tab [plane][lcap][theta_pad] (FP64) -> Tb [s][plane][nhp][lcap], columns
-// c < nh taken from distance c, the others zero.  A transpose through LDS:
+// tab [plane][lcap][theta_pad] (FP64) -> Tb, planes s nplanes + plane of
+// nhp rows (b16_at), row c from distance c < nh, the others zero.  A transpose through LDS:
 // a workgroup reads 64 degrees x 64 distances of one plane along the
 // distances and writes, per split term and distance, the tile's 32 even
 // and 32 odd degrees (two contiguous runs of the permuted row) along l.
@@ -193,7 +195,7 @@ __global__ __launch_bounds__(ST_NT) void split_table_kernel(const double *tab, i
     for (int idx = threadIdx.x; idx < ns * ST_C * 2 * 8; idx += ST_NT) {
         const int q = idx & 7, par = (idx >> 3) & 1, rest = idx >> 4;
         const int cc = rest % ST_C, sp = rest / ST_C;
-        u16 *dst = Tb + (((size_t)sp * nplanes + p) * nhp + c0 + cc) * lcap + (par ? LH : 0) + (l0 >> 1) + 4 * q;
+        u16 *dst = Tb + b16_at((size_t)sp * nplanes + p, nhp, c0 + cc, (par ? LH : 0) + (l0 >> 1) + 4 * q, lcap);
         ushort4 w;
         w.x = t[sp][cc][2 * (4 * q + 0) + par];
         w.y = t[sp][cc][2 * (4 * q + 1) + par];
@@ -283,15 +285,16 @@ __device__ __forceinline__ void synth_b16_tile(const SynthB16Args &a, const Synt
         const int row = (g & 7) * 16 + (lane >> 2);
         const int seg = (lane & 3) ^ sw_seg(row);
         const int src_row = (g < NS * 8) ? min(m0 + row, M - 1) : n0 + row;
-        loff[u] = (unsigned)src_row * (unsigned)a.lcap + 8 * seg;
+        loff[u] = (unsigned)src_row * 32u + 8 * seg;  // within the chunk of the stage's 32 degrees
     }
     const size_t a_split = (size_t)M * a.lcap, t_split = (size_t)a.nplanes * a.nhp * a.lcap;
     auto issue = [&](int it2) {
         int jb, lp0;
         itmap(it2, jb, lp0);
         const int plane = jb == 0 ? pl0 : jb == 1 ? pl1 : jb == 2 ? pl2 : pl3;
-        const u16 *ea = a.Eb + (size_t)(q0 + jb) * NS * a_split + lp0;
-        const u16 *tb = a.Tb + (size_t)plane * a.nhp * a.lcap + lp0;
+        // the stage's chunk of degrees lp0 .. lp0 + 31 (b16_at)
+        const u16 *ea = a.Eb + (size_t)(q0 + jb) * NS * a_split + (size_t)lp0 * M;
+        const u16 *tb = a.Tb + (size_t)plane * a.nhp * a.lcap + (size_t)lp0 * a.nhp;
         issue_stage<NS>(ea, tb, a_split, t_split, loff, wave, with_b, lds + (it2 % NSTAGE) * STAGE);
     };
     if (nit > 0) issue(0);
@@ -439,15 +442,16 @@ void launch_zero_tail_b16(unsigned short *Eb, int nq, int ns, int nslot, int r_e
     hipLaunchKernelGGL(zero_tail_b16_kernel, dim3(8192), dim3(256), 0, s, Eb, nq, ns, nslot, r_eff, lcap, nl, lend);
 }
 
-void launch_b16_put_l0(const unsigned short *one, unsigned short *Eb, size_t nrows, int lcap, hipStream_t s)
+void launch_b16_put_l0(const unsigned short *one, unsigned short *Eb, size_t nrows, size_t M, int lcap,
+                       hipStream_t s)
 {
-    hipLaunchKernelGGL(b16_put_l0_kernel, dim3(1024), dim3(256), 0, s, one, Eb, nrows, lcap);
+    hipLaunchKernelGGL(b16_put_l0_kernel, dim3(1024), dim3(256), 0, s, one, Eb, nrows, M, lcap);
 }
 
-void launch_zero_pad_b16(unsigned short *Eb, size_t nrows, int lcap, int lfrom, hipStream_t s)
+void launch_zero_pad_b16(unsigned short *Eb, size_t nrows, size_t M, int lcap, int lfrom, hipStream_t s)
 {
     if (lfrom >= lcap || nrows == 0) return;
-    hipLaunchKernelGGL(zero_pad_b16_kernel, dim3(2048), dim3(256), 0, s, Eb, nrows, lcap, lfrom);
+    hipLaunchKernelGGL(zero_pad_b16_kernel, dim3(2048), dim3(256), 0, s, Eb, nrows, M, lcap, lfrom);
 }
 
 void launch_split_table(const double *tab, int nplanes, int lcap, int theta_pad, int nh, int nhp, int ns,
