@@ -1634,7 +1634,13 @@ static int psv_run_batch(dsm_sgt_t *h, const PsvPrep &p, bool full_range)
                 HIPCHK(h, hipMemcpyAsync(kst_host.data(), h->d_kstart.p, sizeof(int) * kst_host.size(),
                                          hipMemcpyDeviceToHost, h->stream));
                 HIPCHK(h, dev_sync(h));
-                HIPCHK(h, h->d_scratch.ensure(sizeof(double2) * (size_t)std::max(total, 1LL)));
+                // (a quarter more than this batch needs: later batches, of higher
+                // frequencies, store more rows, and each growth reallocates)
+                const size_t need = sizeof(double2) * (size_t)std::max(total, 1LL);
+                if (need > h->d_scratch.n && h->d_scratch.ensure(need + need / 4) != hipSuccess) {
+                    (void)hipGetLastError();
+                    HIPCHK(h, h->d_scratch.ensure(need));
+                }
                 sa.scratch = h->d_scratch.as<double2>();
                 sa.loff = loff;
             } else if (h->precision != DSM_SGT_FP64) {
@@ -1905,11 +1911,15 @@ static int psv_run_batch(dsm_sgt_t *h, const PsvPrep &p, bool full_range)
     // (w = 4 entries per row) of the rows solved -- from the depth cut's
     // first row of the system's block of 256 degrees down to the surface --
     // and (U, V, U', V') of 18 right-hand sides at every receiver
+    // (summed per block of 256 degrees, whose systems share the cut: this
+    // runs between two batches, with the device idle)
     for (int k = 0; k < S; k++)
-        for (int l = 1; l < nl[k]; l++) {
+        for (int l0b = 0; l0b < nl[k]; l0b += 256) {
+            const int la = std::max(1, l0b), lb = std::min(nl[k], l0b + 256);
+            if (lb <= la) continue;
             int ne = n;
-            if (!kst_host.empty() && (l >> 8) < kst_nblk) ne = n - kst_host[(size_t)k * kst_nblk + (l >> 8)] + 1;
-            h->stats.solve_alg_bytes += 16.0 * (4.0 * ne + 4.0 * 18 * R);
+            if (!kst_host.empty() && (l0b >> 8) < kst_nblk) ne = n - kst_host[(size_t)k * kst_nblk + (l0b >> 8)] + 1;
+            h->stats.solve_alg_bytes += (double)(lb - la) * 16.0 * (4.0 * ne + 4.0 * 18 * R);
         }
     HIPCHK(h, dev_sync(h));
     free_graphs(h);
