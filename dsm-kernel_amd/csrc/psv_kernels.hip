@@ -1,6 +1,8 @@
 // psv_kernels.hip -- P-SV (spheroidal) per-(omega,l) kernels: source
 // excitation, band solve, coefficient rows of the synthesis contraction and
 // the l = 0 system.  Built without FMA contraction (see psv_lane.h).
+#include <type_traits>
+
 #include "dev_common.h"
 #include "kernels.h"
 
@@ -907,18 +909,40 @@ __global__ __launch_bounds__(64) void psv_coef_b16_kernel(PsvCoefArgs a)
         }
         const int *rr = a.rec_row + 6 * r;
         const double *w0 = a.w0 + 3 * r, *w1 = a.w1 + 3 * r;
-        PsvVals z[PSV_NSOL];
-        for (int s = 0; s < PSV_NSOL; s++) {
-            const double2 g0 = xv(rr[0], s), g1 = xv(rr[1], s), g2 = xv(rr[2], s), g3 = xv(rr[3], s),
-                          g4 = xv(rr[4], s), g5 = xv(rr[5], s);
-            z[s].U = cadd(cadd(cscale(g0, w0[0]), cscale(g2, w0[1])), cscale(g4, w0[2]));
-            z[s].V = cadd(cadd(cscale(g1, w0[0]), cscale(g3, w0[1])), cscale(g5, w0[2]));
-            z[s].Ud = cadd(cadd(cscale(g0, w1[0]), cscale(g2, w1[1])), cscale(g4, w1[2]));
-            z[s].Vd = cadd(cadd(cscale(g1, w1[0]), cscale(g3, w1[1])), cscale(g5, w1[2]));
-        }
-        psv_rows(z, a.fluid[r] != 0, a.fluidK[(size_t)sl * R + r], il, a.irf[r], mu1, mu2, [&](int q, double2 v) {
+        const bool fl = a.fluid[r] != 0;
+        const double2 K = a.fluidK[(size_t)sl * R + r];
+        const double irf = a.irf[r];
+        auto put = [&](int q, double2 v) {
             if (q < nq) wave_put(a.eb, st, base, qstride, q, live ? v : cz(), t);  // (q is uniform)
-        });
+        };
+        // one solution at a time (psv_rows grouped by solution): its
+        // interpolated values, then its rows; the stencil values of the next
+        // solution are loaded before the rows of this one
+        auto ld = [&](int S, double2 *g) {
+            for (int k = 0; k < 6; k++) g[k] = xv(rr[k], S);
+        };
+        auto one = [&](auto sc, const double2 *g) {
+            constexpr int S = decltype(sc)::value;
+            PsvVals z;
+            z.U = cadd(cadd(cscale(g[0], w0[0]), cscale(g[2], w0[1])), cscale(g[4], w0[2]));
+            z.V = cadd(cadd(cscale(g[1], w0[0]), cscale(g[3], w0[1])), cscale(g[5], w0[2]));
+            z.Ud = cadd(cadd(cscale(g[0], w1[0]), cscale(g[2], w1[1])), cscale(g[4], w1[2]));
+            z.Vd = cadd(cadd(cscale(g[1], w1[0]), cscale(g[3], w1[1])), cscale(g[5], w1[2]));
+            psv_rows_one<S>(z, fl, K, il, irf, mu1, mu2, put);
+        };
+        double2 ga[6], gb[6];
+        ld(0, ga);
+        ld(1, gb);
+        one(std::integral_constant<int, 0>{}, ga);
+        ld(2, ga);
+        one(std::integral_constant<int, 1>{}, gb);
+        ld(3, gb);
+        one(std::integral_constant<int, 2>{}, ga);
+        ld(4, ga);
+        one(std::integral_constant<int, 3>{}, gb);
+        ld(5, gb);
+        one(std::integral_constant<int, 4>{}, ga);
+        one(std::integral_constant<int, 5>{}, gb);
     }
 }
 

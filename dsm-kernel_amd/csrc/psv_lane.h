@@ -684,39 +684,44 @@ struct PsvVals {
     double2 U, V, Ud, Vd;
 };
 
-template <class Put>
-__host__ __device__ inline void psv_rows(const PsvVals z[PSV_NSOL], bool fluid, double2 K, double il, double irf,
-                                         bool has_mu1, bool has_mu2, Put &&put)
+// The rows built from solution S alone (z[S] = a); in the fluid, S = 0
+// also emits the rows that are zero there.  psv_rows emits all of them.
+template <int S, class Put>
+__host__ __device__ inline void psv_rows_one(const PsvVals &a, bool fluid, double2 K, double il, double irf,
+                                             bool has_mu1, bool has_mu2, Put &&put)
 {
     const double2 Z = cz();
     if (fluid) {
-        const double2 u1 = cmul(K, z[0].U), u2 = cmul(K, z[1].U), ub = cmul(K, z[4].U);
-        const double2 ux = has_mu1 ? cmul(K, z[2].U) : Z, us = has_mu1 ? cmul(K, z[5].U) : Z;
-        const double2 u2x = has_mu2 ? cmul(K, z[3].U) : Z;
-        for (int q = 0; q < PSV_NQ; q++) {
-            double2 v = Z;
-            switch (q) {
-            case 0: v = u1; break;
-            case 1: v = cneg(u2); break;
-            case 2: v = cneg(u1); break;
-            case 5: v = u2; break;
-            case 14: v = cscale(ux, -2.0); break;
-            case 15: v = cscale(ux, 2.0); break;
-            case 29: v = cscale(u2x, 2.0); break;
-            case 30: v = cscale(u2x, -2.0); break;
-            case 42: v = ub; break;
-            case 46: v = cneg(ub); break;
-            case 49: v = cscale(us, -2.0); break;
-            case 50: v = cscale(us, 2.0); break;
-            default: break;
+        if (S == 0) {
+            const double2 u1 = cmul(K, a.U);
+            for (int q = 0; q < PSV_NQ; q++) {
+                switch (q) {
+                case 0: put(q, u1); break;
+                case 2: put(q, cneg(u1)); break;
+                case 1: case 5: case 14: case 15: case 29: case 30: case 42: case 46: case 49: case 50: break;
+                default: put(q, Z); break;
+                }
             }
-            put(q, v);
+        } else if (S == 1) {
+            const double2 u2 = cmul(K, a.U);
+            put(1, cneg(u2));
+            put(5, u2);
+        } else if (S == 4) {
+            const double2 ub = cmul(K, a.U);
+            put(42, ub);
+            put(46, cneg(ub));
+        } else if (S == 2 || S == 5) {
+            const double2 ux = has_mu1 ? cmul(K, a.U) : Z;
+            put(S == 2 ? 14 : 49, cscale(ux, -2.0));
+            put(S == 2 ? 15 : 50, cscale(ux, 2.0));
+        } else {  // S == 3
+            const double2 u2x = has_mu2 ? cmul(K, a.U) : Z;
+            put(29, cscale(u2x, 2.0));
+            put(30, cscale(u2x, -2.0));
         }
         return;
     }
-    // mu = 0: Y1 (imt 1)
-    {
-        const PsvVals &a = z[0];
+    if (S == 0) {  // mu = 0: Y1 (imt 1)
         const double2 Vdl = cscale(a.Vd, il);
         const double2 Ui = cscale(a.U, irf), Vli = cscale(cscale(a.V, il), irf);
         put(0, a.Ud);
@@ -726,10 +731,7 @@ __host__ __device__ inline void psv_rows(const PsvVals z[PSV_NSOL], bool fluid, 
         put(8, cscale(cadd(csub(Ui, Vli), Vdl), -0.5));
         put(9, cscale(Vli, 0.5));
         put(10, cscale(Vli, -0.5));
-    }
-    // Y2 (imt 2 = imt 3)
-    {
-        const PsvVals &a = z[1];
+    } else if (S == 1) {  // Y2 (imt 2 = imt 3)
         const double2 Vdl = cscale(a.Vd, il);
         const double2 Ui = cscale(a.U, irf), Vli = cscale(cscale(a.V, il), irf);
         put(1, cneg(a.Ud));
@@ -739,10 +741,7 @@ __host__ __device__ inline void psv_rows(const PsvVals z[PSV_NSOL], bool fluid, 
         put(11, cscale(Vli, -0.5));
         put(12, cscale(Vli, 0.5));
         put(13, cscale(cadd(csub(Ui, Vli), Vdl), 0.25));
-    }
-    // YB (back-propagated icomp 1)
-    {
-        const PsvVals &a = z[4];
+    } else if (S == 4) {  // YB (back-propagated icomp 1)
         const double2 Vdl = cscale(a.Vd, il);
         const double2 Ui = cscale(a.U, irf), Vli = cscale(cscale(a.V, il), irf);
         put(42, a.Ud);
@@ -752,22 +751,19 @@ __host__ __device__ inline void psv_rows(const PsvVals z[PSV_NSOL], bool fluid, 
         put(46, cneg(Ui));
         put(47, cscale(Vli, -0.5));
         put(48, cscale(Vli, -0.5));
-    }
-    // mu = 1: imt 4 / 5 (X1) and the back-propagated icomp 2 / 3 (S)
-    for (int w = 0; w < 2; w++) {
-        const PsvVals &a = z[w == 0 ? 2 : 5];
+    } else if (S == 2 || S == 5) {  // mu = 1: imt 4 / 5 (X1) and the back-propagated icomp 2 / 3 (S)
         const double2 Vdl = has_mu1 ? cscale(a.Vd, il) : Z;
         const double2 Ui = has_mu1 ? cscale(a.U, irf) : Z;
         const double2 Vli = has_mu1 ? cscale(cscale(a.V, il), irf) : Z;
         const double2 Ud = has_mu1 ? a.Ud : Z;
-        const int o = (w == 0) ? 14 : 49;
+        const int o = (S == 2) ? 14 : 49;
         put(o, cscale(Ud, -2.0));
         put(o + 1, cscale(Ui, 2.0));
         put(o + 2, Vli);
         put(o + 3, cneg(Vli));
         put(o + 4, Vli);
         const double2 ud = csub(csub(cscale(Vli, 2.0), Ui), cscale(Vdl, 2.0));  // (2Vl-U)/r - 2Vdl
-        if (w == 0) {
+        if (S == 2) {
             put(19, cscale(ud, 0.25));
             put(20, cscale(Vli, -0.5));
             put(21, cscale(Vli, -0.5));
@@ -788,10 +784,7 @@ __host__ __device__ inline void psv_rows(const PsvVals z[PSV_NSOL], bool fluid, 
             put(61, csub(Vdl, Vli));
             put(62, Ui);
         }
-    }
-    // mu = 2: imt 2 / 3 / 6 (X2)
-    {
-        const PsvVals &a = z[3];
+    } else {  // S == 3, mu = 2: imt 2 / 3 / 6 (X2)
         const double2 Vdl = has_mu2 ? cscale(a.Vd, il) : Z;
         const double2 Ui = has_mu2 ? cscale(a.U, irf) : Z;
         const double2 Vli = has_mu2 ? cscale(cscale(a.V, il), irf) : Z;
@@ -811,6 +804,18 @@ __host__ __device__ inline void psv_rows(const PsvVals z[PSV_NSOL], bool fluid, 
         put(40, cscale(Vli, 2.0));
         put(41, cscale(Vli, -2.0));
     }
+}
+
+template <class Put>
+__host__ __device__ inline void psv_rows(const PsvVals z[PSV_NSOL], bool fluid, double2 K, double il, double irf,
+                                         bool has_mu1, bool has_mu2, Put &&put)
+{
+    psv_rows_one<0>(z[0], fluid, K, il, irf, has_mu1, has_mu2, put);
+    psv_rows_one<1>(z[1], fluid, K, il, irf, has_mu1, has_mu2, put);
+    psv_rows_one<2>(z[2], fluid, K, il, irf, has_mu1, has_mu2, put);
+    psv_rows_one<3>(z[3], fluid, K, il, irf, has_mu1, has_mu2, put);
+    psv_rows_one<4>(z[4], fluid, K, il, irf, has_mu1, has_mu2, put);
+    psv_rows_one<5>(z[5], fluid, K, il, irf, has_mu1, has_mu2, put);
 }
 
 // SYNN rows from the surface values (Us, Vs) = g0(nn-1), g0(nn) of the
