@@ -254,6 +254,21 @@ struct dsm_sgt {
     int io_rc = 0;
     std::string io_err;
     float2 *pin = nullptr;  // pinned staging of the writer (NPIN blocks)
+    // pinned host copy of the amplitudes of the host cut-off replay (SH, the
+    // P-SV FP64 form): the strided read-back of every l chunk into pageable
+    // memory took milliseconds with the device idle
+    void *pin_amp = nullptr;
+    size_t pin_amp_n = 0;
+    hipError_t amp_buf(size_t bytes)
+    {
+        if (bytes <= pin_amp_n) return hipSuccess;
+        if (pin_amp) (void)hipHostFree(pin_amp);
+        pin_amp = nullptr;
+        pin_amp_n = 0;
+        const hipError_t e = hipHostMalloc(&pin_amp, bytes, hipHostMallocDefault);
+        if (e == hipSuccess) pin_amp_n = bytes;
+        return e;
+    }
     size_t pin_n = 0;
     // KernelMaker catalog stage (dsm_km_points): one radius's catalog, its
     // phase tables, the points and their outputs
@@ -920,7 +935,8 @@ static int sh_run_batch(dsm_sgt_t *h, const ShPrep &p)
 
     std::vector<int> active(S);
     for (int k = 0; k < S; k++) active[k] = k;
-    std::vector<double> amp((size_t)S * LC);
+    HIPCHK(h, h->amp_buf(sizeof(double) * (size_t)S * LC));
+    double *amp = (double *)h->pin_amp;
     // rows solved per (slot, l): nn, or from the depth cut's first row
     std::vector<int> neff((size_t)S * LC, nn), jst;
     int l0 = 0;
@@ -1560,7 +1576,8 @@ static int psv_run_batch(dsm_sgt_t *h, const PsvPrep &p, bool full_range)
     CaptureGuard cg;
     TimedLaunch t_solve;  // events of the current solve launch
     std::vector<int> nlr_est;
-    std::vector<double2> ampuv((size_t)S * LC * 2);
+    HIPCHK(h, h->amp_buf(sizeof(double2) * (size_t)S * LC * 2));
+    double2 *ampuv = (double2 *)h->pin_amp;
     std::vector<int> kst_host;  // depth cut per (slot, block of 256 l), single-launch case
     int kst_nblk = 0;
     int l0 = 0;
@@ -1634,12 +1651,15 @@ static int psv_run_batch(dsm_sgt_t *h, const PsvPrep &p, bool full_range)
                 HIPCHK(h, hipMemcpyAsync(kst_host.data(), h->d_kstart.p, sizeof(int) * kst_host.size(),
                                          hipMemcpyDeviceToHost, h->stream));
                 HIPCHK(h, dev_sync(h));
-                // (a quarter more than this batch needs: later batches, of higher
-                // frequencies, store more rows, and each growth reallocates)
+                // (a quarter more than this batch needs where memory is ample:
+                // later batches, of higher frequencies, store more rows, and
+                // each growth reallocates; with little free memory, exactly
+                // what is needed, so as not to evict the block cache)
                 const size_t need = sizeof(double2) * (size_t)std::max(total, 1LL);
-                if (need > h->d_scratch.n && h->d_scratch.ensure(need + need / 4) != hipSuccess) {
-                    (void)hipGetLastError();
-                    HIPCHK(h, h->d_scratch.ensure(need));
+                if (need > h->d_scratch.n) {
+                    size_t fr = 0, tot = 0, want = need;
+                    if (hipMemGetInfo(&fr, &tot) == hipSuccess && fr > 2 * kHbmReserve + 2 * need) want = need + need / 4;
+                    HIPCHK(h, h->d_scratch.ensure(want));
                 }
                 sa.scratch = h->d_scratch.as<double2>();
                 sa.loff = loff;
@@ -3037,6 +3057,7 @@ void dsm_sgt_destroy(dsm_sgt_t *h)
     if (h->stream_io) (void)hipStreamDestroy(h->stream_io);
     if (h->ev_pack) (void)hipEventDestroy(h->ev_pack);
     if (h->pin) (void)hipHostFree(h->pin);
+    if (h->pin_amp) (void)hipHostFree(h->pin_amp);
     if (h->stream) {
         (void)hipSetDevice(h->device);
         (void)hipStreamSynchronize(h->stream);
