@@ -45,11 +45,13 @@ constexpr double PI = 3.1415926535897932;
 // Device blocks released by destroyed engines, kept for the next engine of
 // the process: an engine holds ~100 GB, and freeing it and allocating it
 // again has stalled engine creation for seconds.  Per device; a released
-// block is reused for a request of at least half its size.  All cached
-// blocks of a device are freed when a newly built engine leaves less than
-// kHbmReserve free, and before an allocation that would leave less (freeing
-// them block by block as needed was measured slower: it repeats); an
-// allocation that would still leave less fails (ENOMEM).  The runtime allocates kernel scratch (register spills) at
+// block is reused for a request of at least half its size.  Cached blocks
+// of a device are freed, largest first, when a newly built engine leaves
+// less than kHbmReserve free and before an allocation that would leave
+// less, until that much is free (freeing every cached block there, as an
+// earlier version did, cost the next engines seconds of fresh allocations
+// after an autotuner's engines had filled the cache); an allocation that
+// would still leave less fails (ENOMEM).  The runtime allocates kernel scratch (register spills) at
 // launch from the same memory, and a queue whose scratch allocation fails
 // aborts.  Never torn down (the process exit returns the memory).
 class BufCache {
@@ -71,6 +73,22 @@ public:
     {
         std::lock_guard<std::mutex> g(m_);
         free_.insert({{dev, n}, p});
+    }
+    // free cached blocks of the device, largest first, until the device
+    // has `want` bytes free (or the cache is empty)
+    void trim(int dev, size_t want)
+    {
+        std::lock_guard<std::mutex> g(m_);
+        for (;;) {
+            size_t fr = 0, tot = 0;
+            if (hipMemGetInfo(&fr, &tot) != hipSuccess || fr >= want) return;
+            auto last = free_.end();
+            for (auto it = free_.begin(); it != free_.end(); ++it)
+                if (it->first.first == dev) last = it;  // the map is ordered by (device, size)
+            if (last == free_.end()) return;
+            (void)hipFree(last->second);
+            free_.erase(last);
+        }
     }
     void flush(int dev)
     {
@@ -115,7 +133,7 @@ struct DevBuf {
         }
         size_t fr = 0, tot = 0;
         if (hipMemGetInfo(&fr, &tot) == hipSuccess && fr < bytes + kHbmReserve) {
-            buf_cache().flush(dev);
+            buf_cache().trim(dev, bytes + kHbmReserve);
             // still short of the reserve: refuse here (ENOMEM) rather than
             // let a later kernel launch abort the queue
             if (hipMemGetInfo(&fr, &tot) == hipSuccess && fr < bytes + kHbmReserve) return hipErrorOutOfMemory;
@@ -671,7 +689,7 @@ static void psv_synth_jobs(dsm_sgt_t *h)
             j.accumulate = d.pass;
             j.ncomp = ncomp[d.kind];
             j.out = outs[d.kind];
-        j.c64 = h->cat64;
+            j.c64 = h->cat64;
             h->jobs.push_back(j);
             counts[grp]++;
         }
@@ -1651,16 +1669,7 @@ static int psv_run_batch(dsm_sgt_t *h, const PsvPrep &p, bool full_range)
                 HIPCHK(h, hipMemcpyAsync(kst_host.data(), h->d_kstart.p, sizeof(int) * kst_host.size(),
                                          hipMemcpyDeviceToHost, h->stream));
                 HIPCHK(h, dev_sync(h));
-                // (a quarter more than this batch needs where memory is ample:
-                // later batches, of higher frequencies, store more rows, and
-                // each growth reallocates; with little free memory, exactly
-                // what is needed, so as not to evict the block cache)
-                const size_t need = sizeof(double2) * (size_t)std::max(total, 1LL);
-                if (need > h->d_scratch.n) {
-                    size_t fr = 0, tot = 0, want = need;
-                    if (hipMemGetInfo(&fr, &tot) == hipSuccess && fr > 2 * kHbmReserve + 2 * need) want = need + need / 4;
-                    HIPCHK(h, h->d_scratch.ensure(want));
-                }
+                HIPCHK(h, h->d_scratch.ensure(sizeof(double2) * (size_t)std::max(total, 1LL)));
                 sa.scratch = h->d_scratch.as<double2>();
                 sa.loff = loff;
             } else if (h->precision != DSM_SGT_FP64) {
@@ -2061,7 +2070,7 @@ int dsm_sgt_create_tuned(const dsm_sgt_input_t *in, int32_t mode, int32_t precis
         // the cache gives its blocks back when the engine leaves less than
         // the reserve free (kernel scratch is allocated at launch)
         size_t fr = 0, tot = 0;
-        if (hipMemGetInfo(&fr, &tot) == hipSuccess && fr < kHbmReserve) buf_cache().flush(device);
+        if (hipMemGetInfo(&fr, &tot) == hipSuccess && fr < kHbmReserve) buf_cache().trim(device, kHbmReserve);
     } while (0);
     *handle = h;  // returned even on failure so dsm_sgt_last_error() can report
     return rc;
