@@ -283,9 +283,9 @@ def catalog_errors(got_all, ref, freqs):
 # HBM bytes of the solve stage (excitation, band factorization and
 # substitutions, coefficient rows, l = 0 systems, cut-off, depth cut) over
 # the whole 5 s P-SV catalog: rocprofv3 FETCH_SIZE + WRITE_SIZE summed over
-# the stage's dispatches, scaled by a plain copy's counters (x1.9906,
-# x0.9664); profiles/band_solve_psv_5s_catalog.md
-SOLVE_STAGE_BYTES = {("psv", "bf16x6"): 1.9994e11, ("psv", "fp64"): 1.6449e12}
+# the stage's dispatches, scaled by a plain copy's counters (x1.991,
+# x0.97); profiles/band_solve_psv_5s_catalog.md
+SOLVE_STAGE_BYTES = {("psv", "bf16x6"): 2.026e11, ("psv", "fp64"): 1.6449e12}
 
 
 def band_solve(mode, form, st, full=True):
@@ -615,7 +615,7 @@ def main():
     # frequencies per batch, by the synthesis tile grouping of the plan:
     # rocprofv3 FETCH_SIZE and WRITE_SIZE per dispatch of
     # synth_b16_kernel<3, 0>, scaled by the counters of a plain 1 GiB copy
-    # (x1.991, x0.967); profiles/synth_b16_psv_5s_catalog_mixed.md
+    # (x1.991, x0.973); profiles/synth_b16_psv_5s_catalog_mixed.md
     traffic = None if args.imax else profile_traffic(args.mode, args.form, eng_plan)
     form_txt = {"bf16x6": "batched mixed: bf16x6 synthesis, FP64 solve with the depth cut",
                 "bf16x3": "batched mixed: bf16x3 synthesis, FP64 solve with the depth cut",
