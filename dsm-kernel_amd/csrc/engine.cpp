@@ -13,6 +13,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <climits>
 #include <cstddef>
 #include <cmath>
 #include <complex>
@@ -192,6 +193,9 @@ struct dsm_sgt {
     // first synthesis (tab_wait: that wait is still to be enqueued)
     hipEvent_t ev_tab = nullptr;
     bool tab_wait = false;
+    // SH: the l chunks of a batch, recorded after each chunk's cut-off state
+    // is copied to the host (two in flight)
+    hipEvent_t ev_chunk[2] = {nullptr, nullptr};
     std::string err;
     dsm_sgt_input_t in{};
     ShSetup sh;
@@ -244,6 +248,9 @@ struct dsm_sgt {
     // per-slot angular frequency, first solved row per (slot, l block)
     double dcut = 0.0;
     DevBuf d_rrow, d_vrow, d_omg, d_kstart, d_rlow, d_rneed, d_zs, d_loff, d_lsuf, d_llog, d_wsave;
+    // SH l cut-off replayed on the device: [slot] running maximum (double),
+    // then [slot] count, lstop, lsuf (int)
+    DevBuf d_shcut;
     // fused FP64 family (mixed forms): compact band and its selector bits
     DevBuf d_cband, d_csel;
     // l = 0 receiver rows of the BF16 forms, written beside the solves and
@@ -272,9 +279,10 @@ struct dsm_sgt {
     int io_rc = 0;
     std::string io_err;
     float2 *pin = nullptr;  // pinned staging of the writer (NPIN blocks)
-    // pinned host copy of the amplitudes of the host cut-off replay (SH, the
-    // P-SV FP64 form): the strided read-back of every l chunk into pageable
-    // memory took milliseconds with the device idle
+    // pinned host buffer of the read-backs inside a batch: the amplitudes of
+    // the P-SV FP64 form's host cut-off replay, SH's per-chunk cut-off state
+    // and first solved rows (a strided read-back of every l chunk into
+    // pageable memory took milliseconds with the device idle)
     void *pin_amp = nullptr;
     size_t pin_amp_n = 0;
     hipError_t amp_buf(size_t bytes)
@@ -761,7 +769,8 @@ static int sh_create(dsm_sgt_t *h)
 
     // geometry of the batches
     h->nslot = h->tune.nslot > 0 ? h->tune.nslot : 32;
-    h->lchunk = h->tune.lchunk > 0 ? std::max(64, h->tune.lchunk) : 512;
+    // (whole blocks of 64 degrees: the solve's scratch is laid out per block)
+    h->lchunk = h->tune.lchunk > 0 ? (std::max(64, h->tune.lchunk) + 63) / 64 * 64 : 512;
     h->ipos0 = std::max(0, h->in.imin);
     h->npos = std::max(0, h->in.imax - h->ipos0 + 1);
     h->llog.assign(h->npos, 0);
@@ -833,6 +842,7 @@ static int sh_create(dsm_sgt_t *h)
     HIPCHK(h, h->d_E.ensure(sizeof(double2) * (size_t)S * 36 * R * h->lcap));
     HIPCHK(h, h->d_Es.ensure(sizeof(double2) * (size_t)S * 5 * h->lcap));
     HIPCHK(h, h->d_amp.ensure(sizeof(double) * (size_t)S * h->lcap));
+    HIPCHK(h, h->d_shcut.ensure((sizeof(double) + 3 * sizeof(int)) * (size_t)S));
     const size_t np = (size_t)std::max(1, h->ncat);
     HIPCHK(h, h->d_cat_t.ensure(h->cat_el() * np * R * s.theta_n * 10));
     if (h->in.rsgtswitch) HIPCHK(h, h->d_cat_r.ensure(h->cat_el() * np * R * s.theta_n * 5));
@@ -938,32 +948,50 @@ static int sh_run_batch(dsm_sgt_t *h, const ShPrep &p)
     const std::vector<int> &freqs = p.freqs;
     const std::vector<double2> &band = p.band, &om = p.om;
     const int S = (int)freqs.size(), nn = s.nn, R = s.r_n, LC = h->lcap;
-    std::vector<CutState> cut(S);
     std::vector<int> slotout(S);
-    for (int k = 0; k < S; k++) {
-        cut[k].lsuf = p.lsuf[k];
-        cut[k].llog = s.maxlmax;
-        slotout[k] = h->catpos[freqs[k] - h->ipos0];  // catalog position
-    }
+    for (int k = 0; k < S; k++) slotout[k] = h->catpos[freqs[k] - h->ipos0];  // catalog position
     HIPCHK(h, hipMemcpyAsync(h->d_band.p, band.data(), sizeof(double2) * band.size(), hipMemcpyHostToDevice, h->stream));
     HIPCHK(h, hipMemcpyAsync(h->d_om.p, om.data(), sizeof(double2) * om.size(), hipMemcpyHostToDevice, h->stream));
     HIPCHK(h, hipMemcpyAsync(h->d_slotout.p, slotout.data(), sizeof(int) * S, hipMemcpyHostToDevice, h->stream));
     if (h->dcut > 0.0)
         HIPCHK(h, hipMemcpyAsync(h->d_omg.p, p.omg.data(), sizeof(double) * S, hipMemcpyHostToDevice, h->stream));
 
-    std::vector<int> active(S);
-    for (int k = 0; k < S; k++) active[k] = k;
-    HIPCHK(h, h->amp_buf(sizeof(double) * (size_t)S * LC));
-    double *amp = (double *)h->pin_amp;
-    // rows solved per (slot, l): nn, or from the depth cut's first row
-    std::vector<int> neff((size_t)S * LC, nn), jst;
-    int l0 = 0;
-    while (!active.empty()) {
-        const int lend = std::min(l0 + h->lchunk - 1, s.maxlmax);
+    // every slot runs the l chunks until its cut-off, which the device
+    // replays after each chunk (sh_cut_kernel); later chunks skip the slots
+    // whose cut-off is known.  The host reads each chunk's state one chunk
+    // behind: chunk c is queued before the state after chunk c-1 is read, so
+    // the device does not wait for the host, and the chunk queued after the
+    // last slot finished does no work.
+    std::vector<int> all(S);
+    for (int k = 0; k < S; k++) all[k] = k;
+    HIPCHK(h, hipMemcpyAsync(h->d_slots.p, all.data(), sizeof(int) * S, hipMemcpyHostToDevice, h->stream));
+    std::vector<unsigned char> st((sizeof(double) + 3 * sizeof(int)) * (size_t)S, 0);  // carry 0, count 0
+    {
+        int *lst = (int *)(st.data() + sizeof(double) * S) + S;
+        for (int k = 0; k < S; k++) {
+            lst[k] = INT_MAX;
+            lst[S + k] = p.lsuf[k];
+        }
+    }
+    HIPCHK(h, hipMemcpyAsync(h->d_shcut.p, st.data(), st.size(), hipMemcpyHostToDevice, h->stream));
+    double *cut_carry = h->d_shcut.as<double>();
+    int *cut_count = (int *)(cut_carry + S), *cut_lstop = cut_count + S, *cut_lsuf = cut_lstop + S;
+    const int nlbmax = (h->lchunk + 63) / 64;
+    const int nchunk = s.maxlmax / h->lchunk + 1;
+    // pinned: per chunk [S] lstop, then [S][nlbmax] first solved rows
+    const size_t per_chunk = (size_t)S * (1 + nlbmax);
+    HIPCHK(h, h->amp_buf(sizeof(int) * per_chunk * nchunk));
+    int *pin = (int *)h->pin_amp;
+    auto all_cut = [&](int c) {
+        for (int k = 0; k < S; k++)
+            if (pin[per_chunk * c + k] == INT_MAX) return false;
+        return true;
+    };
+    int c = 0, last = -1;  // last: the chunk whose state holds every cut-off
+    for (; c < nchunk; c++) {
+        const int l0 = c * h->lchunk, lend = std::min(l0 + h->lchunk - 1, s.maxlmax);
         const int nlb = (lend - l0 + 64) / 64;
-        const int nblocks = (int)active.size() * nlb;
-        HIPCHK(h, hipMemcpyAsync(h->d_slots.p, active.data(), sizeof(int) * active.size(), hipMemcpyHostToDevice,
-                                 h->stream));
+        const int nblocks = S * nlb;
         ShSolveArgs sa{};
         sa.small_pivots = h->d_diag.as<unsigned long long>();
         sa.band = h->d_band.as<double2>();
@@ -981,12 +1009,13 @@ static int sh_run_batch(dsm_sgt_t *h, const ShPrep &p)
         sa.lcap = LC;
         sa.om = h->d_om.as<double2>();
         sa.src = h->d_src.as<ShSrcConst>();
+        sa.lstop = cut_lstop;
         TimedLaunch t0;
         ev_begin(h, t0, 0);
         if (h->dcut > 0.0) {  // mixed forms: first solved row per block of 64 degrees
             ShDepthArgs da{h->d_rrow.as<double>(), h->d_vrow.as<double>(), h->d_omg.as<double>(),
-                           h->d_slots.as<int>(), (int)active.size(), nlb, l0, s.ns, h->dcut, 0.35,
-                           h->d_kstart.as<int>()};
+                           h->d_slots.as<int>(), S, nlb, l0, s.ns, h->dcut, 0.35,
+                           h->d_kstart.as<int>(), cut_lstop};
             launch_sh_depth(da, h->stream);
             sa.jstart = h->d_kstart.as<int>();
         }
@@ -1008,64 +1037,65 @@ static int sh_run_batch(dsm_sgt_t *h, const ShPrep &p)
         ca.E = h->d_E.as<double2>();
         ca.Es = h->d_Es.as<double2>();
         ca.amp = h->d_amp.as<double>();
+        ca.lstop = cut_lstop;
         launch_sh_coef(ca, nblocks, h->stream);
+        HIPCHK(h, hipGetLastError());
+        ShCutArgs cu{h->d_amp.as<double>(), cut_lsuf, LC, l0, lend, s.maxlmax, s.ratl, cut_carry, cut_count, cut_lstop};
+        launch_sh_cut(cu, S, h->stream);
         HIPCHK(h, hipGetLastError());
         ev_end(h, t0);
         h->stats.solve_launches++;
-        h->stats.systems += (long)active.size() * (lend - l0 + 1);
-        // algorithmic bytes: band read once, 14 nonzero right-hand sides'
-        // (value, derivative) at every receiver written once (w = 2)
-        h->stats.solve_bytes += (double)active.size() * (lend - l0 + 1) * 16.0 * (2.0 * nn + 4.0 * 14 * R);
-        // |g0(nn)| of the chunk for every slot: one strided copy
-        HIPCHK(h, hipMemcpy2DAsync(&amp[l0], sizeof(double) * LC, h->d_amp.as<double>() + l0, sizeof(double) * LC,
-                                   sizeof(double) * (lend - l0 + 1), S, hipMemcpyDeviceToHost, h->stream));
-        if (h->dcut > 0.0) {
-            jst.resize((size_t)active.size() * nlb);
-            HIPCHK(h, hipMemcpyAsync(jst.data(), h->d_kstart.p, sizeof(int) * jst.size(), hipMemcpyDeviceToHost,
-                                     h->stream));
-        }
-        HIPCHK(h, hipStreamSynchronize(h->stream));
+        int *pc = pin + per_chunk * c;
         if (h->dcut > 0.0)
-            for (size_t a = 0; a < active.size(); a++)
-                for (int b = 0; b < nlb; b++)
-                    for (int l = l0 + 64 * b; l <= std::min(lend, l0 + 64 * b + 63); l++)
-                        neff[(size_t)active[a] * LC + l] = nn - jst[a * nlb + b] + 1;
-        // replay the cut-off in the reference's order (SGTsh-MPI.f90:423-478)
-        std::vector<int> still;
-        for (int k : active) {
-            CutState &c = cut[k];
-            for (int l = l0; l <= lend && !c.done; l++) {
-                if (c.ismall > 20) {
-                    if (c.llog > l) c.llog = l;
-                    c.nl = l;
-                    c.done = true;
-                    break;
-                }
-                for (int m = -2; m <= 2; m++) {
-                    if (m == 0 || std::abs(m) > l) continue;
-                    const double a = (std::abs(m) == 2) ? amp[(size_t)k * LC + l] : 0.0;
-                    calamp(c, a, l, s.ratl, true);
-                }
+            HIPCHK(h, hipMemcpyAsync(pc + S, h->d_kstart.p, sizeof(int) * nblocks, hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(h, hipMemcpyAsync(pc, cut_lstop, sizeof(int) * S, hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(h, hipEventRecord(h->ev_chunk[c & 1], h->stream));
+        if (c > 0) {
+            HIPCHK(h, hipEventSynchronize(h->ev_chunk[(c - 1) & 1]));
+            if (all_cut(c - 1)) {
+                last = c - 1;
+                c++;
+                break;
             }
-            if (!c.done && lend >= s.maxlmax) {
-                c.nl = s.maxlmax + 1;
-                c.done = true;
-            }
-            if (!c.done) still.push_back(k);
         }
-        active.swap(still);
-        l0 = lend + 1;
     }
-    std::vector<int> nl(S), all(S);
+    const int nqueued = c;
+    if (last < 0) {  // the last chunk ends at lmax: its state holds every cut-off
+        last = nqueued - 1;
+        HIPCHK(h, hipEventSynchronize(h->ev_chunk[last & 1]));
+    }
+    std::vector<int> nl(S);
     for (int k = 0; k < S; k++) {
-        nl[k] = cut[k].nl;
-        all[k] = k;
-        h->llog[freqs[k] - h->ipos0] = cut[k].llog - 1;
+        nl[k] = pin[per_chunk * last + k];
+        h->llog[freqs[k] - h->ipos0] = std::min(s.maxlmax, nl[k]) - 1;
         h->done[freqs[k] - h->ipos0] = 1;
         h->logged[freqs[k] - h->ipos0] = 1;
     }
+    // systems solved: every degree of the chunks up to the one whose state
+    // found the slot's cut-off (that chunk is solved whole); rows per system
+    // from the depth cut's first row of its block of 64 degrees
+    for (int k = 0; k < S; k++) {
+        const int ck = std::min(nl[k], s.maxlmax) / h->lchunk;
+        const int nsys = std::min(s.maxlmax, (ck + 1) * h->lchunk - 1) + 1;
+        h->stats.systems += nsys;
+        // algorithmic bytes: band read once, 14 nonzero right-hand sides'
+        // (value, derivative) at every receiver written once (w = 2)
+        h->stats.solve_bytes += (double)nsys * 16.0 * (2.0 * nn + 4.0 * 14 * R);
+        // algorithmic bytes of the reference's systems l = 1 .. nl-1: the
+        // band (w = 2) of the rows solved and the (value, derivative) of 14
+        // right-hand sides at every receiver
+        for (int lb0 = 0; lb0 < nl[k]; lb0 += 64) {
+            const int la = std::max(1, lb0), lb = std::min(nl[k], lb0 + 64);
+            if (lb <= la) continue;
+            int ne = nn;
+            if (h->dcut > 0.0) {
+                const int cc = lb0 / h->lchunk, nlbc = (std::min((cc + 1) * h->lchunk - 1, s.maxlmax) - cc * h->lchunk + 64) / 64;
+                ne = nn - pin[per_chunk * cc + S + (size_t)k * nlbc + (lb0 - cc * h->lchunk) / 64] + 1;
+            }
+            h->stats.solve_alg_bytes += (double)(lb - la) * 16.0 * (2.0 * ne + 4.0 * 14 * R);
+        }
+    }
     HIPCHK(h, hipMemcpyAsync(h->d_nl.p, nl.data(), sizeof(int) * S, hipMemcpyHostToDevice, h->stream));
-    HIPCHK(h, hipMemcpyAsync(h->d_slots.p, all.data(), sizeof(int) * S, hipMemcpyHostToDevice, h->stream));
     TimedLaunch tz;
     ev_begin(h, tz, 3);
     // (the bf16 split writes the zeros of the receiver rows of the BF16
@@ -1136,11 +1166,6 @@ static int sh_run_batch(dsm_sgt_t *h, const ShPrep &p)
     // 4 flops per (complex coefficient x real basis) per (r,theta); 36 rows
     // per receiver (26 without RSGT)
     h->stats.synth_flops += 4.0 * sumL * s.theta_n * (double)h->syn_rows * R;
-    // algorithmic bytes of the reference's systems l = 1 .. nl-1: the band
-    // (w = 2) of the rows solved and the (value, derivative) of 14
-    // right-hand sides at every receiver
-    for (int k = 0; k < S; k++)
-        for (int l = 1; l < nl[k]; l++) h->stats.solve_alg_bytes += 16.0 * (2.0 * neff[(size_t)k * LC + l] + 4.0 * 14 * R);
     HIPCHK(h, hipStreamSynchronize(h->stream));
     ev_collect(h);
     return DSM_SGT_OK;
@@ -2053,7 +2078,9 @@ int dsm_sgt_create_tuned(const dsm_sgt_input_t *in, int32_t mode, int32_t precis
             hipEventCreateWithFlags(&h->ev_pack, hipEventDisableTiming) != hipSuccess ||
             hipEventCreateWithFlags(&h->ev_l0in, hipEventDisableTiming) != hipSuccess ||
             hipEventCreateWithFlags(&h->ev_l0out, hipEventDisableTiming) != hipSuccess ||
-            hipEventCreateWithFlags(&h->ev_tab, hipEventDisableTiming) != hipSuccess) {
+            hipEventCreateWithFlags(&h->ev_tab, hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&h->ev_chunk[0], hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&h->ev_chunk[1], hipEventDisableTiming) != hipSuccess) {
             rc = fail(h, DSM_SGT_EHIP, "hipStreamCreate failed");
             break;
         }
@@ -3066,7 +3093,6 @@ void dsm_sgt_destroy(dsm_sgt_t *h)
     if (h->stream_io) (void)hipStreamDestroy(h->stream_io);
     if (h->ev_pack) (void)hipEventDestroy(h->ev_pack);
     if (h->pin) (void)hipHostFree(h->pin);
-    if (h->pin_amp) (void)hipHostFree(h->pin_amp);
     if (h->stream) {
         (void)hipSetDevice(h->device);
         (void)hipStreamSynchronize(h->stream);
@@ -3079,7 +3105,10 @@ void dsm_sgt_destroy(dsm_sgt_t *h)
         (void)hipStreamSynchronize(h->stream2);
         (void)hipStreamDestroy(h->stream2);
     }
+    if (h->pin_amp) (void)hipHostFree(h->pin_amp);  // (after the streams: copies into it are stream work)
     if (h->ev_tab) (void)hipEventDestroy(h->ev_tab);
+    for (hipEvent_t e : h->ev_chunk)
+        if (e) (void)hipEventDestroy(e);
     if (h->ev_l0in) (void)hipEventDestroy(h->ev_l0in);
     if (h->ev_l0out) {
         (void)hipEventDestroy(h->ev_l0out);

@@ -13,6 +13,7 @@ namespace dsm {
 void launch_sh_solve(const ShSolveArgs &a, int nblocks, hipStream_t s);
 void launch_sh_coef(const ShCoefArgs &a, int nblocks, hipStream_t s);
 void launch_sh_depth(const ShDepthArgs &a, hipStream_t s);
+void launch_sh_cut(const ShCutArgs &a, int nslots, hipStream_t s);
 
 // P-SV lane bodies and argument structs: psv_lane.h
 // excitation g of every (frequency, l) into a.gexc, then the band solve;

@@ -39,6 +39,7 @@ struct ShSolveArgs {
     const ShSrcConst *src;
     const int *jstart;     // mixed forms, depth cut: first row solved per block of the launch (null: row 1)
     unsigned long long *small_pivots;  // count of pivots with |d| <= 1e-14 (may be null)
+    const int *lstop;      // [slot] degrees l >= lstop are not solved (the slot's l cut-off; null: none)
 };
 // Depth cut of the SH systems (mixed forms): per block of 64 degrees the
 // first row whose decay below the source, integrated from the source down,
@@ -50,6 +51,7 @@ struct ShDepthArgs {
     int nslots, nlb, l0, k;     // k: first source node (1-based)
     double dcut, dcap;
     int *jstart;                // [active slot][nlb]
+    const int *lstop;           // [slot] blocks from lstop on are skipped (null: none)
 };
 constexpr int SH_NSOL = 11;  // 10 sources + surface force
 
@@ -69,6 +71,20 @@ struct ShCoefArgs {
     double2 *E;            // [slot][36][r_n][lcap]
     double2 *Es;           // [slot][5][lcap]
     double *amp;           // [slot][lcap]
+    const int *lstop;      // [slot] degrees l >= lstop are skipped (null: none)
+};
+// The l cut-off of SH replayed on the device, one chunk [l0, lend] of
+// degrees at a time, its state carried per slot from chunk to chunk
+// (sh_cut_kernel).  lstop stays INT_MAX until the slot's cut-off is known,
+// then holds it (maxlmax + 1 when the series runs to lmax).
+struct ShCutArgs {
+    const double *amp;   // [slot][lcap] |g0(nn)| from the coefficient kernel
+    const int *lsuf;     // [slot]
+    int lcap, l0, lend, maxlmax;
+    double ratl;
+    double *carry;       // [slot] running maximum of the amplitudes before l0
+    int *count;          // [slot] count of small entries before l0
+    int *lstop;          // [slot]
 };
 
 
@@ -174,7 +190,7 @@ __host__ __device__ inline int sh_solve_lane(const ShSolveArgs &a, int b, int tx
 {
     const int sl = a.slot_list[b / a.nlb];
     const int l = a.l0 + (b % a.nlb) * 64 + tx;
-    if (l > a.lend) return 0;
+    if (l > a.lend || (a.lstop && l >= a.lstop[sl])) return 0;
     const long gid = (long)b * 64 + tx;
     const double l2 = (double)l * (double)(l + 1);
     const int nn = a.nn, k = a.k;
@@ -275,7 +291,7 @@ __host__ __device__ inline void sh_coef_lane(const ShCoefArgs &a, int b, int tx)
 {
     const int sl = a.slot_list[b / a.nlb];
     const int l = a.l0 + (b % a.nlb) * 64 + tx;
-    if (l > a.lend) return;
+    if (l > a.lend || (a.lstop && l >= a.lstop[sl])) return;
     const int R = a.r_n, LC = a.lcap;
     double2 *E = a.E + (size_t)sl * 36 * R * LC + l;
     double2 *Es = a.Es + (size_t)sl * 5 * LC + l;
