@@ -2166,8 +2166,10 @@ int dsm_sgt_autotune(const dsm_sgt_input_t *in, int32_t mode, int32_t precision,
     } else if (psv) {
         nslots = {48, 32, 16};
     } else {
-        nslots = {64, 32, 16};
-        lchunks = {1024, 512, 256};
+        // (SH's solve and coefficient kernels hold one wave per 64 degrees of
+        // a slot: small batches and chunks leave the device under-occupied)
+        nslots = {128, 64, 32};
+        lchunks = {4096, 2048, 1024};
     }
     // the probe: the whole frequency range when it holds at most 1024
     // frequencies (the plan that wins on the top frequencies alone was
@@ -2180,6 +2182,7 @@ int dsm_sgt_autotune(const dsm_sgt_input_t *in, int32_t mode, int32_t precision,
     if (ihi < ilo) return DSM_SGT_EINVAL;
     double best_ms = -1.0;
     int last_rc = DSM_SGT_OK;
+    std::vector<std::pair<dsm_sgt_tuning_t, double>> tried;
     // time plan `t` (tile grouping and graph mode set on the live engine):
     // two runs, the faster counted; a later plan must beat the best by more
     // than 1% (timing noise), and the candidates come simplest first
@@ -2194,6 +2197,7 @@ int dsm_sgt_autotune(const dsm_sgt_input_t *in, int32_t mode, int32_t precision,
             const double dt = ms_since(t0);
             if (ms < 0.0 || dt < ms) ms = dt;
         }
+        tried.emplace_back(t, ms);
         if (best_ms < 0.0 || ms < 0.99 * best_ms) {
             best_ms = ms;
             *best = t;
@@ -2227,6 +2231,22 @@ int dsm_sgt_autotune(const dsm_sgt_input_t *in, int32_t mode, int32_t precision,
             if (rc) last_rc = rc;
             dsm_sgt_destroy(h);
         }
+    // SH: the smallest batch x chunk among the plans within 3% of the
+    // fastest.  The solve's scratch grows with both, and a plan that is only
+    // as fast on the probe can take most of the device (on the 2.5 s
+    // multi-depth grid, 128 x 4096 ran no faster per depth than 64 x 1024
+    // and its blocks had to be allocated afresh after other engines).
+    if (!psv && !tried.empty()) {
+        double fast = tried[0].second;
+        for (const auto &x : tried) fast = std::min(fast, x.second);
+        const std::pair<dsm_sgt_tuning_t, double> *pick = nullptr;
+        auto foot = [](const dsm_sgt_tuning_t &t) { return (double)t.nslot * t.lchunk; };
+        for (const auto &x : tried)
+            if (x.second <= 1.03 * fast && (!pick || foot(x.first) < foot(pick->first))) pick = &x;
+        *best = pick->first;
+        best->catalog_window = 0;
+        best_ms = pick->second;
+    }
     // stage 2: graph mode for the winner
     if (best_ms >= 0.0 && graphs.size() > 1) {
         dsm_sgt_tuning_t t = *best;
