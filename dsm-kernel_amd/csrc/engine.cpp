@@ -91,6 +91,15 @@ public:
             free_.erase(last);
         }
     }
+    // bytes of the device's cached blocks (free for the next engine)
+    size_t cached(int dev)
+    {
+        std::lock_guard<std::mutex> g(m_);
+        size_t n = 0;
+        for (const auto &b : free_)
+            if (b.first.first == dev) n += b.first.second;
+        return n;
+    }
     void flush(int dev)
     {
         std::lock_guard<std::mutex> g(m_);
@@ -119,6 +128,12 @@ struct DevBuf {
     ~DevBuf()
     {
         if (p) buf_cache().give(dev, p, n);
+    }
+    void swap(DevBuf &o)
+    {
+        std::swap(p, o.p);
+        std::swap(n, o.n);
+        std::swap(dev, o.dev);
     }
     hipError_t ensure(size_t bytes)
     {
@@ -196,6 +211,16 @@ struct dsm_sgt {
     // SH: the l chunks of a batch, recorded after each chunk's cut-off state
     // is copied to the host (two in flight)
     hipEvent_t ev_chunk[2] = {nullptr, nullptr};
+    // overlap plan (BF16 P-SV forms): the synthesis of batch b runs on
+    // stream_syn beside the solves of batch b+1.  The buffers between the
+    // two stages are held twice (the *_alt set) and swapped before every
+    // batch; ev_ready (engine stream) releases a batch's synthesis, and
+    // ev_syn[set] marks the end of the synthesis that last read a set, which
+    // the engine stream waits for before it writes that set again.
+    bool ovl = false;
+    int bset = 0;
+    hipStream_t stream_syn = nullptr;
+    hipEvent_t ev_ready = nullptr, ev_syn[2] = {nullptr, nullptr};
     std::string err;
     dsm_sgt_input_t in{};
     ShSetup sh;
@@ -235,6 +260,19 @@ struct dsm_sgt {
     int mirror = 0, nh = 0, nhp = 0;
     DevBuf d_tb, d_eb, d_stash, d_nlr, d_tiles;
     DevBuf d_kpart;  // split-K partial sums of the SYNN synthesis
+    // overlap plan: the second set of the buffers the synthesis reads
+    DevBuf d_eb_alt, d_Es_alt, d_nlr_alt, d_nl_alt, d_slotout_alt, d_tiles_alt, d_kpart_alt;
+    void swap_sets()
+    {
+        d_eb.swap(d_eb_alt);
+        d_Es.swap(d_Es_alt);
+        d_nlr.swap(d_nlr_alt);
+        d_nl.swap(d_nl_alt);
+        d_slotout.swap(d_slotout_alt);
+        d_tiles.swap(d_tiles_alt);
+        d_kpart.swap(d_kpart_alt);
+        bset ^= 1;
+    }
     DevBuf d_diag;  // [0] small pivots, [1] non-finite catalog values of the current run
     std::vector<int> tiles_main, tiles_acc;  // host copies of the tile orders (launch sizes)
     int tiles_main_n = 0, tiles_acc_n = 0;
@@ -398,8 +436,9 @@ static hipError_t wait_tables(dsm_sgt_t *h)
 // Synthesis launch in the engine's precision; Eb is the bf16 split of
 // ga.E (BF16 forms).
 static void launch_synth(dsm_sgt_t *h, const SynthArgs &ga, const DevBuf &Eb, const int *nlr = nullptr,
-                         const int *tiles = nullptr, int ntiles = 0)
+                         const int *tiles = nullptr, int ntiles = 0, hipStream_t st = nullptr)
 {
+    if (!st) st = h->stream;
     if (h->ns) {
         SynthB16Args b{};
         b.Eb = Eb.as<unsigned short>();
@@ -421,11 +460,11 @@ static void launch_synth(dsm_sgt_t *h, const SynthArgs &ga, const DevBuf &Eb, co
         b.tiles = tiles;
         b.ntiles = ntiles;
         b.nonfinite = ga.nonfinite;
-        launch_synth_b16(b, h->ns, h->stream);
+        launch_synth_b16(b, h->ns, st);
     } else if (h->precision == DSM_SGT_FP32) {
-        launch_synth_f32(ga, h->d_tab32.as<float>(), h->stream);
+        launch_synth_f32(ga, h->d_tab32.as<float>(), st);
     } else {
-        launch_synth_f64(ga, h->stream);
+        launch_synth_f64(ga, st);
     }
 }
 
@@ -1358,6 +1397,18 @@ static int psv_create(dsm_sgt_t *h)
     // coefficient rows, l = 0 rows, cut-off tails, zero rows below the cut
     rc = alloc_b16(h, S, PSV_NQ, R, s.maxlmax + 1);
     if (rc) return rc;
+    // overlap plan (BF16 forms, eager launches): the second set of the
+    // buffers the synthesis reads, initialised as the first
+    h->ovl = h->tune.overlap == 1 && h->ns && !h->graph;
+    if (h->ovl) {
+        h->swap_sets();
+        HIPCHK(h, h->d_Es.ensure(h->d_Es_alt.n));
+        HIPCHK(h, h->d_nl.ensure(h->d_nl_alt.n));
+        HIPCHK(h, h->d_slotout.ensure(h->d_slotout_alt.n));
+        rc = alloc_b16(h, S, PSV_NQ, R, s.maxlmax + 1);
+        h->swap_sets();
+        if (rc) return rc;
+    }
     if (h->ns) {
         HIPCHK(h, h->d_l0row.ensure(sizeof(unsigned short) * PSV_NQ * h->ns * (size_t)S * R * 2));
         HIPCHK(h, h->d_l0slots.ensure(sizeof(int) * S));
@@ -1529,6 +1580,9 @@ static int psv_run_batch(dsm_sgt_t *h, const PsvPrep &p, bool full_range)
         cut[k].llog = s.maxlmax;
         slotout[k] = h->catpos[freqs[k] - h->ipos0];  // catalog position
     }
+    // overlap plan: this batch's set of synthesis buffers was last read by
+    // the synthesis two batches back
+    if (h->ovl) HIPCHK(h, hipStreamWaitEvent(h->stream, h->ev_syn[h->bset], 0));
     {
         const int rc = psv_device_band(h, p.scal, S, 0);
         if (rc) return rc;
@@ -1910,19 +1964,26 @@ static int psv_run_batch(dsm_sgt_t *h, const PsvPrep &p, bool full_range)
     gs.jobs = h->d_jobs.as<SynthJob>() + h->njobs_main + h->njobs_acc;
     gs.njobs = h->njobs_syn;
     if (h->njobs_syn) HIPCHK(h, synn_split(h, gs));
+    // overlap plan: the synthesis on its own stream, once the engine stream
+    // has written this batch's rows, extents and tile order
+    const hipStream_t ss = h->ovl ? h->stream_syn : h->stream;
+    if (h->ovl) {
+        HIPCHK(h, hipEventRecord(h->ev_ready, h->stream));
+        HIPCHK(h, hipStreamWaitEvent(ss, h->ev_ready, 0));
+    }
     TimedLaunch tg;
     if (gmode) {
         rc = begin_capture(h, cg);
         if (rc) return rc;
     } else {
-        ev_begin(h, tg, 2);
+        ev_begin(h, tg, 2, ss);
     }
-    launch_synth(h, ga, h->d_eb, h->d_nlr.as<int>(), tiled ? h->d_tiles.as<int>() : nullptr, h->tiles_main_n);
+    launch_synth(h, ga, h->d_eb, h->d_nlr.as<int>(), tiled ? h->d_tiles.as<int>() : nullptr, h->tiles_main_n, ss);
     HIPCHK(h, hipGetLastError());
     ga.jobs = h->d_jobs.as<SynthJob>() + h->njobs_main;
     ga.njobs = h->njobs_acc;
     launch_synth(h, ga, h->d_eb, h->d_nlr.as<int>(), tiled ? h->d_tiles.as<int>() + h->tiles_main_n : nullptr,
-                 h->tiles_acc_n);
+                 h->tiles_acc_n, ss);
     HIPCHK(h, hipGetLastError());
     if (gmode) {  // synthesis graph: the main and the accumulate launch
         rc = end_capture_launch(h, cg, &tg, 2);
@@ -1932,11 +1993,12 @@ static int psv_run_batch(dsm_sgt_t *h, const PsvPrep &p, bool full_range)
     }
     if (h->njobs_syn) {
         TimedLaunch tg2;
-        ev_begin(h, tg2, 3);
-        launch_synth_f64(gs, h->stream);
+        ev_begin(h, tg2, 3, ss);
+        launch_synth_f64(gs, ss);
         HIPCHK(h, hipGetLastError());
         ev_end(h, tg2);
     }
+    if (h->ovl) HIPCHK(h, hipEventRecord(h->ev_syn[h->bset], ss));
     h->stats.synth_launches += 2;
     std::vector<int> nlr_dev;
     if (h->ns) {  // the l extents the synthesis used, for its issued-flop count
@@ -2080,7 +2142,11 @@ int dsm_sgt_create_tuned(const dsm_sgt_input_t *in, int32_t mode, int32_t precis
             hipEventCreateWithFlags(&h->ev_l0out, hipEventDisableTiming) != hipSuccess ||
             hipEventCreateWithFlags(&h->ev_tab, hipEventDisableTiming) != hipSuccess ||
             hipEventCreateWithFlags(&h->ev_chunk[0], hipEventDisableTiming) != hipSuccess ||
-            hipEventCreateWithFlags(&h->ev_chunk[1], hipEventDisableTiming) != hipSuccess) {
+            hipEventCreateWithFlags(&h->ev_chunk[1], hipEventDisableTiming) != hipSuccess ||
+            hipStreamCreateWithFlags(&h->stream_syn, hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&h->ev_ready, hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&h->ev_syn[0], hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&h->ev_syn[1], hipEventDisableTiming) != hipSuccess) {
             rc = fail(h, DSM_SGT_EHIP, "hipStreamCreate failed");
             break;
         }
@@ -2113,6 +2179,7 @@ int dsm_sgt_get_tuning(dsm_sgt_t *h, dsm_sgt_tuning_t *t)
     t->xcd_group = h->xg;
     t->graph = h->graph;
     t->catalog_window = h->ncat < h->npos ? h->ncat : 0;
+    t->overlap = h->ovl ? 1 : 0;
     return DSM_SGT_OK;
 }
 
@@ -2135,19 +2202,24 @@ int dsm_sgt_autotune(const dsm_sgt_input_t *in, int32_t mode, int32_t precision,
     mix(&mode, sizeof(mode));
     mix(&precision, sizeof(precision));
     mix(prop.gcnArchName, std::strlen(prop.gcnArchName));
+    // (the key's suffix names the line format: nslot lchunk xcd_group graph
+    // overlap, then the probe time; lines of other formats never match)
     char key[32];
-    std::snprintf(key, sizeof(key), "%016llx", (unsigned long long)hv);
+    std::snprintf(key, sizeof(key), "%016llx-o", (unsigned long long)hv);
     if (cache_path) {
         if (FILE *f = std::fopen(cache_path, "r")) {
-            char k[64];
-            dsm_sgt_tuning_t t{};
-            double ms = 0.0;
+            char line[256], k[64];
             bool hit = false;
-            while (std::fscanf(f, "%63s %d %d %d %d %lf", k, &t.nslot, &t.lchunk, &t.xcd_group, &t.graph, &ms) == 6)
-                if (std::strcmp(k, key) == 0) {
+            while (std::fgets(line, sizeof(line), f)) {
+                dsm_sgt_tuning_t t{};
+                double ms = 0.0;
+                if (std::sscanf(line, "%63s %d %d %d %d %d %lf", k, &t.nslot, &t.lchunk, &t.xcd_group, &t.graph,
+                                &t.overlap, &ms) == 7 &&
+                    std::strcmp(k, key) == 0) {
                     *best = t;  // the last entry of a key wins
                     hit = true;
                 }
+            }
             std::fclose(f);
             if (hit) return DSM_SGT_OK;
         }
@@ -2180,13 +2252,23 @@ int dsm_sgt_autotune(const dsm_sgt_input_t *in, int32_t mode, int32_t precision,
     const int nprobe = (ihi - first + 1 <= 1024) ? ihi - first + 1 : 16 + 2 * nslots[0];
     const int ilo = std::max(first, ihi - nprobe + 1);
     if (ihi < ilo) return DSM_SGT_EINVAL;
-    double best_ms = -1.0;
+    double best_ms = -1.0, tight_ms = -1.0;
+    dsm_sgt_tuning_t tight{};
     int last_rc = DSM_SGT_OK;
     std::vector<std::pair<dsm_sgt_tuning_t, double>> tried;
+    // headroom a plan must leave: the device's free memory plus the cached
+    // blocks of earlier engines, with the plan's engine alive after its
+    // runs.  Catalogs produced in windows build one engine after another
+    // (several source depths), and plans that left the device nearly full
+    // had their blocks freed and allocated afresh between those engines
+    // (seconds each); such plans are taken only when no other runs.
+    hipDeviceProp_t dp{};
+    size_t room = (size_t)64 << 30;
+    if (hipGetDeviceProperties(&dp, device) == hipSuccess) room = std::min(room, dp.totalGlobalMem / 5);
     // time plan `t` (tile grouping and graph mode set on the live engine):
     // two runs, the faster counted; a later plan must beat the best by more
     // than 1% (timing noise), and the candidates come simplest first
-    auto probe = [&](dsm_sgt_t *h, const dsm_sgt_tuning_t &t) -> int {
+    auto probe = [&](dsm_sgt_t *h, const dsm_sgt_tuning_t &t, double need = 0.99) -> int {
         h->xg = t.xcd_group;
         h->graph = t.graph;
         double ms = -1.0;
@@ -2197,8 +2279,18 @@ int dsm_sgt_autotune(const dsm_sgt_input_t *in, int32_t mode, int32_t precision,
             const double dt = ms_since(t0);
             if (ms < 0.0 || dt < ms) ms = dt;
         }
+        size_t fr = 0, tot = 0;
+        const bool roomy = hipMemGetInfo(&fr, &tot) != hipSuccess || fr + buf_cache().cached(device) >= room;
+        if (!roomy) {
+            if (tight_ms < 0.0 || ms < need * tight_ms) {
+                tight_ms = ms;
+                tight = t;
+                tight.catalog_window = 0;
+            }
+            return DSM_SGT_OK;
+        }
         tried.emplace_back(t, ms);
-        if (best_ms < 0.0 || ms < 0.99 * best_ms) {
+        if (best_ms < 0.0 || ms < need * best_ms) {
             best_ms = ms;
             *best = t;
             best->catalog_window = 0;
@@ -2257,12 +2349,34 @@ int dsm_sgt_autotune(const dsm_sgt_input_t *in, int32_t mode, int32_t precision,
         if (rc) last_rc = rc;
         dsm_sgt_destroy(h);
     }
+    // stage 3: the synthesis beside the next batch's solves, for an eager
+    // winner of the BF16 P-SV forms; it holds a second set of the synthesis
+    // buffers and must win by 3% (on the 5 s grid it ran 1.5% faster, the
+    // synthesis launches 47% longer beside the solves)
+    if (best_ms >= 0.0 && psv && bf16 && best->graph == 0) {
+        dsm_sgt_tuning_t t = *best;
+        t.overlap = 1;
+        dsm_sgt_t *h = nullptr;
+        int rc = engine(t, &h);
+        if (!rc) rc = probe(h, t, 0.97);
+        if (rc) last_rc = rc;
+        dsm_sgt_destroy(h);
+    }
+    // the probe engines' blocks, of every candidate's sizes, are freed: left
+    // in the cache they split the device between sizes the chosen plan's
+    // engines do not all reuse, and engines built one after another then
+    // had blocks freed and allocated afresh between them (seconds each)
+    buf_cache().flush(device);
+    if (best_ms < 0.0 && tight_ms >= 0.0) {  // every plan left the device nearly full
+        *best = tight;
+        best_ms = tight_ms;
+    }
     if (best_ms < 0.0) return last_rc ? last_rc : DSM_SGT_EHIP;
     if (probe_ms) *probe_ms = best_ms;
     if (cache_path) {
         if (FILE *f = std::fopen(cache_path, "a")) {
-            std::fprintf(f, "%s %d %d %d %d %.3f\n", key, best->nslot, best->lchunk, best->xcd_group, best->graph,
-                         best_ms);
+            std::fprintf(f, "%s %d %d %d %d %d %.3f\n", key, best->nslot, best->lchunk, best->xcd_group, best->graph,
+                         best->overlap, best_ms);
             std::fclose(f);
         }
     }
@@ -2349,6 +2463,7 @@ static int run_batches(dsm_sgt_t *h, const std::vector<int> &all)
         next.get();
         h->stats.prep_wait_ms += ms_since(tw);
         if (b + 1 < nb) next = std::async(std::launch::async, prepare, b + 1);
+        if (h->ovl) h->swap_sets();  // the other set of synthesis buffers
         int rc = psv_run_batch(h, h->prep[b & 1], false);
         if (rc == PSV_RETRY_FULL) {
             h->stats.solve_retries++;
@@ -2370,7 +2485,11 @@ static int run_frequencies(dsm_sgt_t *h, const std::vector<int> &all)
     if (hipSetDevice(h->device) != hipSuccess) return fail(h, DSM_SGT_EHIP, "hipSetDevice failed");
     HIPCHK(h, hipMemsetAsync(h->d_diag.p, 0, h->d_diag.n, h->stream));
     const int rc = run_batches(h, all);
+    // (overlap plan: the last batches' synthesis is still on its stream)
+    const hipError_t es = hipStreamSynchronize(h->stream_syn);
     if (rc) return rc;
+    HIPCHK(h, es);
+    ev_collect(h);
     unsigned long long diag[2] = {0, 0};
     HIPCHK(h, hipMemcpyAsync(diag, h->d_diag.p, sizeof(diag), hipMemcpyDeviceToHost, h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));
@@ -3125,6 +3244,12 @@ void dsm_sgt_destroy(dsm_sgt_t *h)
         (void)hipStreamSynchronize(h->stream2);
         (void)hipStreamDestroy(h->stream2);
     }
+    if (h->stream_syn) {
+        (void)hipStreamSynchronize(h->stream_syn);
+        (void)hipStreamDestroy(h->stream_syn);
+    }
+    for (hipEvent_t e : {h->ev_ready, h->ev_syn[0], h->ev_syn[1]})
+        if (e) (void)hipEventDestroy(e);
     if (h->pin_amp) (void)hipHostFree(h->pin_amp);  // (after the streams: copies into it are stream work)
     if (h->ev_tab) (void)hipEventDestroy(h->ev_tab);
     for (hipEvent_t e : h->ev_chunk)

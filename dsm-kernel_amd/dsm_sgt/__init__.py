@@ -41,11 +41,11 @@ class Stats(C.Structure):
 class Tuning(C.Structure):
     """Launch plan (dsm_sgt_tuning_t): 0 in a field takes the default."""
     _fields_ = [("nslot", C.c_int32), ("lchunk", C.c_int32), ("xcd_group", C.c_int32), ("graph", C.c_int32),
-                ("catalog_window", C.c_int32), ("reserved", C.c_int32 * 3)]
+                ("catalog_window", C.c_int32), ("overlap", C.c_int32), ("reserved", C.c_int32 * 2)]
 
     def as_dict(self):
         return {"nslot": self.nslot, "lchunk": self.lchunk, "xcd_group": self.xcd_group, "graph": self.graph,
-                "catalog_window": self.catalog_window}
+                "catalog_window": self.catalog_window, "overlap": self.overlap}
 
 
 class KmParams(C.Structure):

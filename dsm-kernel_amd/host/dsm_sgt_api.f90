@@ -25,7 +25,8 @@ module dsm_sgt_api
   ! dsm_sgt_tuning_t (launch plan; 0 = default)
   type, bind(c) :: dsm_sgt_tuning
      integer(c_int32_t) :: nslot = 0, lchunk = 0, xcd_group = 0, graph = 0, catalog_window = 0
-     integer(c_int32_t) :: reserved(3) = 0
+     integer(c_int32_t) :: overlap = 0
+     integer(c_int32_t) :: reserved(2) = 0
   end type dsm_sgt_tuning
 
   ! dsm_km_params_t (include/dsm_km.h): KernelMaker's parameters of the catalog stage

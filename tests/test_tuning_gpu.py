@@ -220,3 +220,28 @@ def test_sh_depth_cut(depth_km):
     worst = max(float(component_errors(x, y).max()) for i in freqs for x, y in zip(got[i], want[i]))
     assert worst <= 1e-10, worst
     print(f"SH depth cut, source {depth_km} km: worst component difference {worst:.1e}")
+
+
+@pytest.mark.parametrize("plan", [{"overlap": 1, "nslot": 8}, {"overlap": 1, "nslot": 16, "catalog_window": 20}])
+def test_overlap_plan_gives_the_same_catalog(plan):
+    """The overlap plan (the synthesis of a batch on its own stream beside
+    the next batch's solves, with two sets of the buffers between them):
+    every frequency's records bit-identical to the one-stream plan's, over
+    two runs (with a catalog window the second run replaces the first's
+    frequencies; without, it recomputes them in place)."""
+    import dsm_sgt
+    inp = _input()
+    imax = GRID["imax"]
+    freqs = tuple(range(1, imax + 1))
+    with dsm_sgt.Engine(inp, mode=dsm_sgt.PSV, precision=dsm_sgt.BF16X6, tuning={"nslot": plan["nslot"]}) as ref:
+        ref.run(1, imax)
+        want = _catalog(ref, freqs)
+        lc = [ref.lcut(i) for i in freqs]
+    with dsm_sgt.Engine(inp, mode=dsm_sgt.PSV, precision=dsm_sgt.BF16X6, tuning=plan) as e:
+        assert e.tuning().overlap == 1 and e.tuning().nslot == plan["nslot"]
+        runs = [(1, 20), (21, imax)] if plan.get("catalog_window") else [(1, imax), (1, imax)]
+        for a, b in runs:
+            e.run(a, b)
+            _same(_catalog(e, range(a, b + 1)), {i: want[i] for i in range(a, b + 1)})
+        assert [e.lcut(i) for i in freqs] == lc
+    print(f"overlap plan {plan}: {imax} frequencies bit-identical to the one-stream plan")
