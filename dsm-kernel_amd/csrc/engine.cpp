@@ -876,7 +876,7 @@ static int sh_create(dsm_sgt_t *h)
     HIPCHK(h, h->d_nl.ensure(sizeof(int) * S));
     HIPCHK(h, h->d_slotout.ensure(sizeof(int) * S));
     const size_t nthreads = (size_t)S * h->lchunk;
-    HIPCHK(h, h->d_scratch.ensure(sizeof(double2) * (size_t)(nn + 10 * (nn - k + 1)) * nthreads));
+    HIPCHK(h, h->d_scratch.ensure(sizeof(double2) * (size_t)(nn + 20 + (nn - k - 1)) * nthreads));
     HIPCHK(h, h->d_xn.ensure(sizeof(double2) * (size_t)S * h->nneed * SH_NSOL * h->lcap));
     HIPCHK(h, h->d_E.ensure(sizeof(double2) * (size_t)S * 36 * R * h->lcap));
     HIPCHK(h, h->d_Es.ensure(sizeof(double2) * (size_t)S * 5 * h->lcap));

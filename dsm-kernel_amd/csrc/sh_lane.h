@@ -31,7 +31,7 @@ struct ShSolveArgs {
     int nn, k;             // band rows, first source node (1-based)
     int nneed;
     const int *need;       // stencil nodes, descending, 1-based
-    double2 *scratch;      // [nn + 10*(nn-k+1)][nthreads]
+    double2 *scratch;      // [nn + 20 + (nn-k-1)][nthreads]: L, the sources at rows k and k+1, c_j dr_j above
     long sstride;          // nthreads of the launch
     double2 *xn;           // [slot][nneed][SH_NSOL][lcap]
     int lcap;
@@ -198,8 +198,15 @@ __host__ __device__ inline int sh_solve_lane(const ShSolveArgs &a, int b, int tx
     const double2 *a0d = a.band + (size_t)(sl * 4 + 1) * nn;
     const double2 *a2o = a.band + (size_t)(sl * 4 + 2) * nn;
     const double2 *a2d = a.band + (size_t)(sl * 4 + 3) * nn;
-    double2 *Ls = a.scratch;                           // L_j at row j-1
-    double2 *Zs = a.scratch + (size_t)nn * a.sstride;  // scaled z of the sources, rows j >= k
+    // scratch: L_j at row j-1; then the scaled forward values of the 10
+    // sources at rows k and k+1 (z dr, 20 entries), then per row j >= k+2
+    // one entry q_j = c_j dr_j.  Above row k+1 the sources' right-hand sides
+    // are zero and their forward values follow one recurrence,
+    // z(j) = -L_j z(j-1), so z_s(j) = c_j z_s(k+1) with the common
+    // c_j = prod_{i=k+2..j} (-L_i): one value per row instead of ten
+    // (the scratch traffic of a deep source's rows above it).
+    double2 *Ls = a.scratch;
+    double2 *Zs = a.scratch + (size_t)nn * a.sstride;
     const long ss = a.sstride;
     double2 *xo = a.xn + (size_t)sl * a.nneed * SH_NSOL * a.lcap + l;
     if (l == 0) {  // no m with 0 < |m| <= l: nothing is solved (SGTsh-MPI.f90:453)
@@ -217,7 +224,8 @@ __host__ __device__ inline int sh_solve_lane(const ShSolveArgs &a, int b, int tx
     // than e^-dcut from the source)
     const int jlo = a.jstart ? a.jstart[b] : 1;
     double2 drp = cz();
-    double2 z[10];
+    double2 z[10];                       // z_s(k), then z_s(k+1)
+    double2 c = make_double2(1.0, 0.0);  // c_j
     int small = 0;
     for (int j = jlo; j <= nn; j++) {
         const double2 dg = caxpy(a0d[j - 1], l2, a2d[j - 1]);
@@ -233,27 +241,33 @@ __host__ __device__ inline int sh_solve_lane(const ShSolveArgs &a, int b, int tx
         const double2 drj = crecip(pv);
         small += !(pv.x * pv.x + pv.y * pv.y > 1e-28);
         Ls[(size_t)(j - 1) * ss + gid] = Lj;
-        if (j >= k) {
+        if (j == k || j == k + 1) {
             for (int s = 0; s < 10; s++) {
-                if (j == k) z[s] = ex[s].a;
-                else if (j == k + 1) z[s] = csub(ex[s].b, cmul(Lj, z[s]));
-                else z[s] = cneg(cmul(Lj, z[s]));
+                z[s] = (j == k) ? ex[s].a : csub(ex[s].b, cmul(Lj, z[s]));
                 Zs[(size_t)(10 * (j - k) + s) * ss + gid] = cmul(z[s], drj);
             }
+        } else if (j > k + 1) {
+            c = cneg(cmul(Lj, c));
+            Zs[(size_t)(18 + j - k) * ss + gid] = cmul(c, drj);
         }
         drp = drj;
     }
+    // scaled forward value of source s at row j >= k (z[] holds z_s(k+1))
+    auto zsc = [&](int j, int s) {
+        return (j <= k + 1) ? Zs[(size_t)(10 * (j - k) + s) * ss + gid] : cmul(Zs[(size_t)(18 + j - k) * ss + gid], z[s]);
+    };
     // pass 2: backward substitution (dclib.f90:98-104), recording the
     // stencil nodes
     double2 x[10];
-    for (int s = 0; s < 10; s++) x[s] = Zs[(size_t)(10 * (nn - k) + s) * ss + gid];
+    for (int s = 0; s < 10; s++) x[s] = zsc(nn, s);
     double2 xs = cmul(gam, drp);  // surface force: z(nn) = gam * dr(nn)
     int p = 0;
     for (int j = nn; j >= jlo; j--) {
         if (j < nn) {
             const double2 Ln = Ls[(size_t)j * ss + gid];  // L_{j+1}
+            const double2 qj = (j > k + 1) ? Zs[(size_t)(18 + j - k) * ss + gid] : cz();
             for (int s = 0; s < 10; s++) {
-                const double2 zz = (j >= k) ? Zs[(size_t)(10 * (j - k) + s) * ss + gid] : cz();
+                const double2 zz = (j > k + 1) ? cmul(qj, z[s]) : (j >= k) ? Zs[(size_t)(10 * (j - k) + s) * ss + gid] : cz();
                 x[s] = csub(zz, cmul(Ln, x[s]));
             }
             xs = cneg(cmul(Ln, xs));
