@@ -193,7 +193,10 @@ def run_c5(args):
         # warm-up (untimed, as the warmup steps of the other workloads): one
         # engine of the plan at the deepest source runs the last window, so
         # that the device blocks the timed engines reuse are allocated once
+        # (the blocks cached from the other mode's engines are released first:
+        # their sizes are not this mode's)
         t0 = time.perf_counter()
+        dsm_sgt.release_cache()
         p = dict(C5, imax=imax, r0=6371.0 - float(depths[deepest]))
         with dsm_sgt.Engine(make_input(p), mode=mode, precision=prec, tuning=plans[name]) as warm:
             warm.run(max(1, imax - C5_WINDOW + 1), imax)

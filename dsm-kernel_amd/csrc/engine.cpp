@@ -876,7 +876,7 @@ static int sh_create(dsm_sgt_t *h)
     HIPCHK(h, h->d_nl.ensure(sizeof(int) * S));
     HIPCHK(h, h->d_slotout.ensure(sizeof(int) * S));
     const size_t nthreads = (size_t)S * h->lchunk;
-    HIPCHK(h, h->d_scratch.ensure(sizeof(double2) * (size_t)(nn + 20 + (nn - k - 1)) * nthreads));
+    HIPCHK(h, h->d_scratch.ensure(sizeof(double2) * (size_t)(2 * ((nn + SH_SEG - 1) / SH_SEG) + 20) * nthreads));
     HIPCHK(h, h->d_xn.ensure(sizeof(double2) * (size_t)S * h->nneed * SH_NSOL * h->lcap));
     HIPCHK(h, h->d_E.ensure(sizeof(double2) * (size_t)S * 36 * R * h->lcap));
     HIPCHK(h, h->d_Es.ensure(sizeof(double2) * (size_t)S * 5 * h->lcap));
@@ -2883,6 +2883,13 @@ int dsm_sgt_host_alloc(size_t bytes, void **ptr)
 void dsm_sgt_host_free(void *ptr)
 {
     if (ptr) (void)hipHostFree(ptr);
+}
+
+int dsm_sgt_release_cache(int32_t device)
+{
+    if (device < 0) return DSM_SGT_EINVAL;
+    buf_cache().flush(device);
+    return DSM_SGT_OK;
 }
 
 int dsm_sgt_write_catalog(dsm_sgt_t *h, int32_t i, const char *output_dir, const char *model_name)

@@ -48,7 +48,7 @@ extern "C" int dsm_hostcheck_sh(const dsm_sgt_input_t *in, int32_t i, double *ts
     }
     const int nneed = (int)need.size();
     const int nlb = (L + 64) / 64, nthr = nlb * 64;
-    std::vector<double2> scratch((size_t)(nn + 20 + (nn - s.ns - 1)) * nthr);
+    std::vector<double2> scratch((size_t)(2 * ((nn + SH_SEG - 1) / SH_SEG) + 20) * nthr);
     std::vector<double2> xn((size_t)nneed * SH_NSOL * lcap);
     std::vector<double2> E((size_t)SH_NQ * R * lcap), Es((size_t)SH_NQS * lcap);
     std::vector<double> amp(lcap), rinv(R);

@@ -31,7 +31,7 @@ struct ShSolveArgs {
     int nn, k;             // band rows, first source node (1-based)
     int nneed;
     const int *need;       // stencil nodes, descending, 1-based
-    double2 *scratch;      // [nn + 20 + (nn-k-1)][nthreads]: L, the sources at rows k and k+1, c_j dr_j above
+    double2 *scratch;      // [2 ceil(nn/SH_SEG) + 20][nthreads]: segment checkpoints, the sources at rows k, k+1
     long sstride;          // nthreads of the launch
     double2 *xn;           // [slot][nneed][SH_NSOL][lcap]
     int lcap;
@@ -54,6 +54,7 @@ struct ShDepthArgs {
     const int *lstop;           // [slot] blocks from lstop on are skipped (null: none)
 };
 constexpr int SH_NSOL = 11;  // 10 sources + surface force
+constexpr int SH_SEG = 8;    // rows per checkpoint segment of the SH solve
 
 // ---- SH coefficients --------------------------------------------------------
 // One lane per (omega,l): source excitation (calga/calg2,
@@ -198,15 +199,17 @@ __host__ __device__ inline int sh_solve_lane(const ShSolveArgs &a, int b, int tx
     const double2 *a0d = a.band + (size_t)(sl * 4 + 1) * nn;
     const double2 *a2o = a.band + (size_t)(sl * 4 + 2) * nn;
     const double2 *a2d = a.band + (size_t)(sl * 4 + 3) * nn;
-    // scratch: L_j at row j-1; then the scaled forward values of the 10
-    // sources at rows k and k+1 (z dr, 20 entries), then per row j >= k+2
-    // one entry q_j = c_j dr_j.  Above row k+1 the sources' right-hand sides
-    // are zero and their forward values follow one recurrence,
-    // z(j) = -L_j z(j-1), so z_s(j) = c_j z_s(k+1) with the common
-    // c_j = prod_{i=k+2..j} (-L_i): one value per row instead of ten
-    // (the scratch traffic of a deep source's rows above it).
-    double2 *Ls = a.scratch;
-    double2 *Zs = a.scratch + (size_t)nn * a.sstride;
+    // scratch: per segment of SH_SEG rows from jlo, a checkpoint (dr and c
+    // of the row below the segment); then the scaled forward values of the
+    // 10 sources at rows k and k+1 (z dr, 20 entries).  Above row k+1 the
+    // sources' right-hand sides are zero and their forward values follow one
+    // recurrence, z(j) = -L_j z(j-1), so z_s(j) = c_j z_s(k+1) with the
+    // common c_j = prod_{i=k+2..j} (-L_i).  The backward pass recomputes each
+    // segment's L_j and c_j dr_j from its checkpoint (the forward pass's
+    // operations on the same values, so the same bits) instead of reading a
+    // value per row and source back.
+    double2 *Ck = a.scratch;
+    double2 *Zs = a.scratch + (size_t)2 * ((nn + SH_SEG - 1) / SH_SEG) * a.sstride;
     const long ss = a.sstride;
     double2 *xo = a.xn + (size_t)sl * a.nneed * SH_NSOL * a.lcap + l;
     if (l == 0) {  // no m with 0 < |m| <= l: nothing is solved (SGTsh-MPI.f90:453)
@@ -223,24 +226,32 @@ __host__ __device__ inline int sh_solve_lane(const ShSolveArgs &a, int b, int tx
     // depth cut the band starts at row jlo (rows below carry a decay of more
     // than e^-dcut from the source)
     const int jlo = a.jstart ? a.jstart[b] : 1;
-    double2 drp = cz();
-    double2 z[10];                       // z_s(k), then z_s(k+1)
-    double2 c = make_double2(1.0, 0.0);  // c_j
-    int small = 0;
-    for (int j = jlo; j <= nn; j++) {
+    // one factor step at row j from dr(j-1): L_j, the pivot and dr(j)
+    auto step = [&](int j, double2 drm, double2 &Lj, double2 &pv) {
         const double2 dg = caxpy(a0d[j - 1], l2, a2d[j - 1]);
-        double2 Lj, pv;
         if (j == jlo) {
             pv = dg;
             Lj = cz();
         } else {
             const double2 xx = caxpy(a0o[j - 1], l2, a2o[j - 1]);
-            Lj = cmul(xx, drp);
+            Lj = cmul(xx, drm);
             pv = csub(dg, cmul(xx, Lj));
         }
-        const double2 drj = crecip(pv);
+        return crecip(pv);
+    };
+    double2 drp = cz();
+    double2 z[10];                       // z_s(k), then z_s(k+1)
+    double2 c = make_double2(1.0, 0.0);  // c_j (1 up to row k+1)
+    int small = 0;
+    for (int j = jlo; j <= nn; j++) {
+        if ((j - jlo) % SH_SEG == 0) {
+            const int t = (j - jlo) / SH_SEG;
+            Ck[(size_t)(2 * t) * ss + gid] = drp;
+            Ck[(size_t)(2 * t + 1) * ss + gid] = c;
+        }
+        double2 Lj, pv;
+        const double2 drj = step(j, drp, Lj, pv);
         small += !(pv.x * pv.x + pv.y * pv.y > 1e-28);
-        Ls[(size_t)(j - 1) * ss + gid] = Lj;
         if (j == k || j == k + 1) {
             for (int s = 0; s < 10; s++) {
                 z[s] = (j == k) ? ex[s].a : csub(ex[s].b, cmul(Lj, z[s]));
@@ -248,35 +259,58 @@ __host__ __device__ inline int sh_solve_lane(const ShSolveArgs &a, int b, int tx
             }
         } else if (j > k + 1) {
             c = cneg(cmul(Lj, c));
-            Zs[(size_t)(18 + j - k) * ss + gid] = cmul(c, drj);
         }
         drp = drj;
     }
-    // scaled forward value of source s at row j >= k (z[] holds z_s(k+1))
-    auto zsc = [&](int j, int s) {
-        return (j <= k + 1) ? Zs[(size_t)(10 * (j - k) + s) * ss + gid] : cmul(Zs[(size_t)(18 + j - k) * ss + gid], z[s]);
-    };
-    // pass 2: backward substitution (dclib.f90:98-104), recording the
-    // stencil nodes
-    double2 x[10];
-    for (int s = 0; s < 10; s++) x[s] = zsc(nn, s);
-    double2 xs = cmul(gam, drp);  // surface force: z(nn) = gam * dr(nn)
+    // pass 2: backward substitution (dclib.f90:98-104) segment by segment
+    // from the top, recording the stencil nodes
+    double2 x[10], xs = cz();
+    double2 Lup = cz();  // L of the row above the segment
     int p = 0;
-    for (int j = nn; j >= jlo; j--) {
-        if (j < nn) {
-            const double2 Ln = Ls[(size_t)j * ss + gid];  // L_{j+1}
-            const double2 qj = (j > k + 1) ? Zs[(size_t)(18 + j - k) * ss + gid] : cz();
-            for (int s = 0; s < 10; s++) {
-                const double2 zz = (j > k + 1) ? cmul(qj, z[s]) : (j >= k) ? Zs[(size_t)(10 * (j - k) + s) * ss + gid] : cz();
-                x[s] = csub(zz, cmul(Ln, x[s]));
+    bool fin = false;
+    for (int t = (nn - jlo) / SH_SEG; t >= 0 && !fin; t--) {
+        const int j0 = jlo + SH_SEG * t;
+        double2 L[SH_SEG], q[SH_SEG];
+        {
+            double2 dr = Ck[(size_t)(2 * t) * ss + gid], cc = Ck[(size_t)(2 * t + 1) * ss + gid];
+#pragma unroll
+            for (int u = 0; u < SH_SEG; u++) {
+                L[u] = q[u] = cz();
+                const int j = j0 + u;
+                if (j <= nn) {
+                    double2 pv;
+                    const double2 drj = step(j, dr, L[u], pv);
+                    if (j > k + 1) {
+                        cc = cneg(cmul(L[u], cc));
+                        q[u] = cmul(cc, drj);  // c_j dr_j
+                    }
+                    dr = drj;
+                }
             }
-            xs = cneg(cmul(Ln, xs));
         }
-        if (p < a.nneed && a.need[p] == j) {
-            for (int s = 0; s < 10; s++) xo[(size_t)(SH_NSOL * p + s) * a.lcap] = x[s];
-            xo[(size_t)(SH_NSOL * p + 10) * a.lcap] = xs;
-            if (++p == a.nneed) break;  // (need is descending: no row below is recorded)
+#pragma unroll
+        for (int u = SH_SEG - 1; u >= 0; u--) {
+            const int j = j0 + u;
+            if (j > nn || fin) continue;
+            // scaled forward value of source s at row j (z[] holds z_s(k+1))
+            auto zz = [&](int s) {
+                return (j > k + 1) ? cmul(q[u], z[s]) : (j >= k) ? Zs[(size_t)(10 * (j - k) + s) * ss + gid] : cz();
+            };
+            if (j == nn) {
+                for (int s = 0; s < 10; s++) x[s] = zz(s);
+                xs = cmul(gam, drp);  // surface force: z(nn) = gam * dr(nn)
+            } else {
+                const double2 Ln = (u == SH_SEG - 1) ? Lup : L[u + 1];  // L_{j+1}
+                for (int s = 0; s < 10; s++) x[s] = csub(zz(s), cmul(Ln, x[s]));
+                xs = cneg(cmul(Ln, xs));
+            }
+            if (p < a.nneed && a.need[p] == j) {
+                for (int s = 0; s < 10; s++) xo[(size_t)(SH_NSOL * p + s) * a.lcap] = x[s];
+                xo[(size_t)(SH_NSOL * p + 10) * a.lcap] = xs;
+                if (++p == a.nneed) fin = true;  // (need is descending: no row below is recorded)
+            }
         }
+        Lup = L[0];
     }
     for (; p < a.nneed; p++)  // stencil rows below the cut
         for (int q = 0; q < SH_NSOL; q++) xo[(size_t)(SH_NSOL * p + q) * a.lcap] = cz();

@@ -68,7 +68,8 @@ EXPORTS = ("dsm_sgt_create", "dsm_sgt_get_info", "dsm_sgt_get_grid", "dsm_sgt_ru
            "dsm_sgt_reset_stats", "dsm_sgt_clear", "dsm_sgt_last_error", "dsm_sgt_destroy", "dsm_sgt_psv_band",
            "dsm_sgt_create_tuned", "dsm_sgt_get_tuning", "dsm_sgt_autotune", "dsm_sgt_run_list",
            "dsm_sgt_write_catalogs_async", "dsm_sgt_copy_catalog_async", "dsm_sgt_wait", "dsm_sgt_host_alloc",
-           "dsm_sgt_host_free", "dsm_km_points", "dsm_km_vectorfft", "dsm_km_vectorfft_noalloc")
+           "dsm_sgt_host_free", "dsm_sgt_release_cache", "dsm_km_points", "dsm_km_vectorfft",
+           "dsm_km_vectorfft_noalloc")
 
 
 def lib():
@@ -95,6 +96,7 @@ def lib():
         L.dsm_sgt_host_alloc.argtypes = [C.c_size_t, C.POINTER(C.c_void_p)]
         L.dsm_sgt_host_free.argtypes = [C.c_void_p]
         L.dsm_sgt_host_free.restype = None
+        L.dsm_sgt_release_cache.argtypes = [C.c_int32]
         L.dsm_km_points.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.POINTER(KmParams), C.c_int32, C.c_void_p,
                                     C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         L.dsm_km_vectorfft.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_void_p,
@@ -315,6 +317,14 @@ class HostBuffer:
             self.close()
         except Exception:
             pass
+
+
+def release_cache(device=0):
+    """Free the device blocks destroyed engines left cached for later
+    engines (dsm_sgt_release_cache)."""
+    rc = lib().dsm_sgt_release_cache(device)
+    if rc:
+        raise DsmSgtError(rc, "release_cache failed")
 
 
 def autotune(inp, mode=PSV, precision=BF16X6, device=0, cache_path=None):

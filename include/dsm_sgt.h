@@ -240,6 +240,12 @@ int dsm_sgt_wait(dsm_sgt_t *h);
 /* Pinned host memory for dsm_sgt_copy_catalog_async (hipHostMalloc). */
 int dsm_sgt_host_alloc(size_t bytes, void **ptr);
 void dsm_sgt_host_free(void *ptr);
+/* Free the device blocks that destroyed engines left cached on HIP device
+ * `device` for the process's later engines (an engine holds ~100 GB; the
+ * next engine of similar sizes takes them over instead of allocating).  No
+ * counterpart in the reference; for callers switching to engines of other
+ * sizes (another mode, another grid) or handing the memory to other code. */
+int dsm_sgt_release_cache(int32_t device);
 
 int dsm_sgt_get_stats(dsm_sgt_t *h, dsm_sgt_stats_t *stats);
 int dsm_sgt_reset_stats(dsm_sgt_t *h);
