@@ -922,7 +922,10 @@ static int sh_create(dsm_sgt_t *h)
             for (int b = 0; b < 6; b++) par[(mu - 1) * 6 + b] = fpar[b] * ((mu & 1) ? -1 : 1);
         set_mirror(h, par);
     }
-    rc = alloc_b16(h, S, 36, R, -1);
+    // the split of every batch writes each degree a tile reads (the terms
+    // below each cut-off, zeros up to the longest cut-off + 128): only the
+    // padding degrees above lmax are zeroed here
+    rc = alloc_b16(h, S, 36, R, s.maxlmax + 1);
     if (rc) return rc;
     HIPCHK(h, h->d_jobs.ensure(sizeof(SynthJob) * h->jobs.size()));
     HIPCHK(h, hipMemcpy(h->d_jobs.p, h->jobs.data(), sizeof(SynthJob) * h->jobs.size(), hipMemcpyHostToDevice));
