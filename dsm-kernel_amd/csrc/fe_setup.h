@@ -106,6 +106,11 @@ struct PsvSetup {
 // Returns 0 or a DSM_SGT_E* code with *err set.
 int psv_setup(const dsm_sgt_input_t &in, PsvSetup &s, std::string &err);
 
+// The input's distances in degrees, as both set-ups take them: the list, or
+// thetamin + (i - 1) thetadelta for i = 1 .. (thetamax - thetamin) / thetadelta + 1
+// (SGTpsv-MPI.f90 / SGTsh-MPI.f90 theta grid); returns their count.
+int theta_grid(const dsm_sgt_input_t &in, std::vector<double> &deg);
+
 // Per-frequency data (SGTpsv-MPI.f90:566-625): the band pieces of every
 // column, the excitation's RK tables and the source-zone moduli.
 struct PsvOmega {

@@ -151,6 +151,15 @@ void mtrnp2(int nlayer, int p, int q, const double *hm1, double *hm2)
 
 }  // namespace
 
+int theta_grid(const dsm_sgt_input_t &in, std::vector<double> &deg)
+{
+    const bool tl = in.theta_list_n > 0;
+    const int n = tl ? in.theta_list_n : (int)((in.thetamax - in.thetamin) / in.thetadelta) + 1;
+    deg.resize(std::max(0, n));
+    for (int i = 1; i <= n; i++) deg[i - 1] = tl ? in.theta_list[i - 1] : in.thetamin + (double)(i - 1) * in.thetadelta;
+    return n;
+}
+
 int psv_setup(const dsm_sgt_input_t &in, PsvSetup &s, std::string &err)
 {
     s = PsvSetup{};
@@ -213,10 +222,7 @@ int psv_setup(const dsm_sgt_input_t &in, PsvSetup &s, std::string &err)
                 s.qkp[ir] = s.qkappa[iz];
             }
     }
-    s.theta_n = tl ? in.theta_list_n : (int)((in.thetamax - in.thetamin) / in.thetadelta) + 1;
-    s.theta_deg.resize(s.theta_n);
-    for (int i = 1; i <= s.theta_n; i++)
-        s.theta_deg[i - 1] = tl ? in.theta_list[i - 1] : in.thetamin + (double)(i - 1) * in.thetadelta;
+    s.theta_n = theta_grid(in, s.theta_deg);
     s.r0 = in.r0min;
     s.intir0 = (int)(s.r0 * 1.0e3);
 

@@ -160,14 +160,11 @@ int sh_setup(const dsm_sgt_input_t &in, ShSetup &s, std::string &err)
             err = "receiver above the surface";
             return DSM_SGT_EINVAL;
         }
-    s.theta_n = tl ? in.theta_list_n : (int)((in.thetamax - in.thetamin) / in.thetadelta) + 1;
+    s.theta_n = theta_grid(in, s.theta_deg);
     if (s.theta_n < 1) {
         err = "empty distance grid";
         return DSM_SGT_EINVAL;
     }
-    s.theta_deg.resize(s.theta_n);
-    for (int i = 1; i <= s.theta_n; i++)
-        s.theta_deg[i - 1] = tl ? in.theta_list[i - 1] : in.thetamin + (double)(i - 1) * in.thetadelta;
     s.r0 = in.r0min;
     s.intir0 = (int)(s.r0 * 1.0e3);
     if (s.r0 < s.rmin || s.r0 > s.rmax) {
