@@ -1218,6 +1218,30 @@ static int sh_run_batch(dsm_sgt_t *h, const ShPrep &p)
 // ------------------------------------------------------------------------
 static int psv_create(dsm_sgt_t *h)
 {
+    // the FP64 distance table first, on stream2: it needs only the distances
+    // and the degree range, and runs beside the host set-up below (whose
+    // distances are the same theta_grid)
+    bool tab_early = false;
+    if ((h->in.theta_list_n > 0 || h->in.thetadelta > 0) && h->in.maxlmax >= 1) {
+        std::vector<double> deg;
+        const int tn = theta_grid(h->in, deg);
+        if (tn >= 1 && tn <= (1 << 20)) {
+            const int LC = ((h->in.maxlmax + 1 + 63) / 64) * 64, tp = ((tn + 127) / 128) * 128;
+            std::vector<double> th(tn);
+            for (int i = 0; i < tn; i++) th[i] = deg[i] / 180.0 * PI;
+            HIPCHK(h, h->d_theta.ensure(sizeof(double) * tn));
+            HIPCHK(h, h->d_tab.ensure(sizeof(double) * PSV_NPLANES * (size_t)LC * tp));
+            HIPCHK(h, h->d_plm.ensure(sizeof(double) * 4 * (size_t)LC * tp));
+            HIPCHK(h, hipMemcpy(h->d_theta.p, th.data(), sizeof(double) * tn, hipMemcpyHostToDevice));
+            TimedLaunch t;
+            ev_begin(h, t, 1, h->stream2);
+            launch_legendre_psv(h->d_theta.as<double>(), tn, tp, h->in.maxlmax, LC, h->d_plm.as<double>(),
+                                h->d_tab.as<double>(), h->stream2);
+            HIPCHK(h, hipGetLastError());
+            ev_end(h, t);
+            tab_early = true;
+        }
+    }
     int rc = psv_setup(h->in, h->psv, h->err);
     if (rc) return rc;
     const PsvSetup &s = h->psv;
@@ -1276,7 +1300,7 @@ static int psv_create(dsm_sgt_t *h)
     HIPCHK(h, h->d_noderow.ensure(sizeof(int) * N0));
     HIPCHK(h, h->d_nodec1.ensure(sizeof(int) * N0));
     HIPCHK(h, h->d_jsta0.ensure(sizeof(int) * R));
-    HIPCHK(h, hipMemcpy(h->d_theta.p, th.data(), sizeof(double) * s.theta_n, hipMemcpyHostToDevice));
+    if (!tab_early) HIPCHK(h, hipMemcpy(h->d_theta.p, th.data(), sizeof(double) * s.theta_n, hipMemcpyHostToDevice));
     HIPCHK(h, hipMemcpy(h->d_need.p, g.need.data(), sizeof(int) * h->nneed, hipMemcpyHostToDevice));
     HIPCHK(h, hipMemcpy(h->d_recslot.p, g.rec_row.data(), sizeof(int) * 6 * R, hipMemcpyHostToDevice));
     HIPCHK(h, hipMemcpy(h->d_w0.p, s.w0.data(), sizeof(double) * 3 * R, hipMemcpyHostToDevice));
@@ -1287,7 +1311,7 @@ static int psv_create(dsm_sgt_t *h)
     HIPCHK(h, hipMemcpy(h->d_noderow.p, s.node_row.data(), sizeof(int) * N0, hipMemcpyHostToDevice));
     HIPCHK(h, hipMemcpy(h->d_nodec1.p, s.node_c1.data(), sizeof(int) * N0, hipMemcpyHostToDevice));
     HIPCHK(h, hipMemcpy(h->d_jsta0.p, s.jsta0.data(), sizeof(int) * R, hipMemcpyHostToDevice));
-    {
+    if (!tab_early) {
         // the FP64 table on stream2, beside the rest of the set-up
         TimedLaunch t;
         ev_begin(h, t, 1, h->stream2);

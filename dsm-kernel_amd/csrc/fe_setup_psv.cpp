@@ -3,6 +3,9 @@
 // (SGTpsv-MPI.f90:566-625).  Built with -ffp-contract=off in the
 // reference's operation order; see fe_setup.h.
 #include <algorithm>
+#include <atomic>
+#include <functional>
+#include <thread>
 #include <cmath>
 #include <cstring>
 #include <limits>
@@ -15,6 +18,22 @@ namespace dsm {
 static const double PI = 3.1415926535897932;
 
 namespace {
+
+// Runs independent host tasks on up to 16 threads and waits for them.
+void run_tasks(std::vector<std::function<void()>> &tasks)
+{
+    std::atomic<size_t> next{0};
+    auto worker = [&] {
+        for (size_t i; (i = next.fetch_add(1)) < tasks.size();) tasks[i]();
+    };
+    const unsigned hc = std::max(1u, std::thread::hardware_concurrency());
+    const size_t nt = std::min<size_t>({tasks.size(), (size_t)16, (size_t)hc});
+    std::vector<std::thread> pool;
+    for (size_t t = 1; t < nt; t++) pool.emplace_back(worker);
+    worker();
+    for (auto &t : pool) t.join();
+}
+
 
 // value of the zone polynomial coef[0..3] at r (x = r/rmax powers built
 // incrementally, others.f90:590-682)
@@ -401,99 +420,112 @@ int psv_setup(const dsm_sgt_input_t &in, PsvSetup &s, std::string &err)
         rhoinv[i] = 1.0 / rho[i];
         kappainv[i] = 1.0 / kappa[i];
     }
-    // element integrals per zone (SGTpsv-MPI.f90:395-435)
+    // element integrals per zone (SGTpsv-MPI.f90:395-435): the matrices of
+    // every zone are independent, so they run as host tasks on a few
+    // threads (each matrix computed by the same operations as in one
+    // sequence), then the zones' derived operators in turn
     s.fe.resize(nz);
+    std::vector<std::function<void()>> tasks;
+    const double *V = vra.data();
     for (int iz = 0; iz < nz; iz++) {
         PsvZoneFE &z = s.fe[iz];
         const int nl = s.nlayer[iz];
         z.nlayer = nl;
         z.solid = s.iphase[iz] == 1;
         const double *rai = &s.ra[s.isp[iz] - 1];
-        std::vector<double> work(4 * nl);
         auto vec = [&](std::vector<double> &v) { v.assign(4 * nl, 0.0); };
+        // a matrix with its lumped part (calhl/caltl, then calt)
+        auto lumped = [&, nl, rai](const std::vector<double> *con, std::vector<double> *dst, int rpow, bool hl) {
+            tasks.push_back([=, &vnp] {
+                std::vector<double> w(4 * nl);
+                calmatc_psv(nl, vnp, V, con->data(), rpow, 0, 0, rai, dst->data());
+                if (hl) calhl(nl, vnp, V, con->data(), rai, w.data());
+                else caltl(nl, vnp, V, con->data(), rai, w.data());
+                calt(nl, dst->data(), w.data(), dst->data());
+            });
+        };
+        auto plain = [&, nl, rai](const std::vector<double> *con, std::vector<double> *dst, int rpow, int w1dn, int w2dn) {
+            tasks.push_back([=, &vnp] { calmatc_psv(nl, vnp, V, con->data(), rpow, w1dn, w2dn, rai, dst->data()); });
+        };
         if (z.solid) {
             for (auto *v : {&z.t, &z.h1x, &z.h1y, &z.h1z, &z.h2L, &z.h2N, &z.h3ax, &z.h3ay, &z.h3az, &z.h4aL, &z.h4aN,
                             &z.h5ax, &z.h5ay, &z.h5az, &z.h6aL, &z.h6aN, &z.h3x, &z.h3y, &z.h3z, &z.h4L, &z.h4N,
                             &z.h5x, &z.h5y, &z.h5z, &z.h6L, &z.h6N, &z.h7x, &z.h7y, &z.h7z, &z.h8L, &z.h8N})
                 vec(*v);
-            const double *V = vra.data();
-            calmatc_psv(nl, vnp, V, rho.data(), 2, 0, 0, rai, z.t.data());
-            caltl(nl, vnp, V, rho.data(), rai, work.data());
-            calt(nl, z.t.data(), work.data(), z.t.data());
-            struct {
-                const std::vector<double> *con;
-                std::vector<double> *dst;
-            } lumped[] = {{&ecKx, &z.h1x}, {&ecKy, &z.h1y}, {&ecKz, &z.h1z}, {&ecL, &z.h2L}, {&ecN, &z.h2N}};
-            for (auto &e : lumped) {
-                calmatc_psv(nl, vnp, V, e.con->data(), 0, 0, 0, rai, e.dst->data());
-                calhl(nl, vnp, V, e.con->data(), rai, work.data());
-                calt(nl, e.dst->data(), work.data(), e.dst->data());
-            }
-            calmatc_psv(nl, vnp, V, ecKx.data(), 1, 0, 1, rai, z.h5ax.data());
-            calmatc_psv(nl, vnp, V, ecKy.data(), 1, 0, 1, rai, z.h5ay.data());
-            calmatc_psv(nl, vnp, V, ecKz.data(), 1, 0, 1, rai, z.h5az.data());
-            calmatc_psv(nl, vnp, V, ecL.data(), 1, 0, 1, rai, z.h6aL.data());
-            calmatc_psv(nl, vnp, V, ecN.data(), 1, 0, 1, rai, z.h6aN.data());
-            mtrnp(nl, z.h5ax.data(), z.h3ax.data());
-            mtrnp(nl, z.h5ay.data(), z.h3ay.data());
-            mtrnp(nl, z.h5az.data(), z.h3az.data());
-            mtrnp(nl, z.h6aL.data(), z.h4aL.data());
-            mtrnp(nl, z.h6aN.data(), z.h4aN.data());
-            calmatc_psv(nl, vnp, V, ecKx.data(), 2, 1, 1, rai, z.h7x.data());
-            calmatc_psv(nl, vnp, V, ecKy.data(), 2, 1, 1, rai, z.h7y.data());
-            calmatc_psv(nl, vnp, V, ecKz.data(), 2, 1, 1, rai, z.h7z.data());
-            calmatc_psv(nl, vnp, V, ecL.data(), 2, 1, 1, rai, z.h8L.data());
-            calmatc_psv(nl, vnp, V, ecN.data(), 2, 1, 1, rai, z.h8N.data());
-            // modified first-derivative operator (SGTpsv-MPI.f90:438-478);
-            // caltstg recomputes the same structure values
-            const int jt = s.isp[iz] + iz - 1;  // zone's first structure node, 0-based
-            const double *vz = &vra[jt];
-            calh5(nl, vz, &ecKx[jt], rai, work.data());
-            for (int k = 0; k < 4 * nl; k++) z.h5x[k] = z.h5ax[k] - work[k];
-            calh5(nl, vz, &ecKy[jt], rai, work.data());
-            for (int k = 0; k < 4 * nl; k++) z.h5y[k] = z.h5ay[k] - work[k];
-            calh5(nl, vz, &ecKz[jt], rai, work.data());
-            for (int k = 0; k < 4 * nl; k++) z.h5z[k] = z.h5az[k] - work[k];
-            calh5(nl, vz, &ecL[jt], rai, work.data());
-            for (int k = 0; k < 4 * nl; k++) z.h6L[k] = z.h6aL[k] - work[k];
-            calh5(nl, vz, &ecN[jt], rai, work.data());
-            for (int k = 0; k < 4 * nl; k++) z.h6N[k] = z.h6aN[k] - work[k];
-            mtrnp(nl, z.h5x.data(), z.h3x.data());
-            mtrnp(nl, z.h5y.data(), z.h3y.data());
-            mtrnp(nl, z.h5z.data(), z.h3z.data());
-            mtrnp(nl, z.h6L.data(), z.h4L.data());
-            mtrnp(nl, z.h6N.data(), z.h4N.data());
-            const int nh = 4 * (nl + 1);
-            for (auto *v : {&z.h3mx, &z.h3my, &z.h3mz, &z.h5mx, &z.h5my, &z.h5mz, &z.h4m1L, &z.h4m1N, &z.h4m2L,
-                            &z.h4m2N, &z.h6m1L, &z.h6m1N, &z.h6m2L, &z.h6m2N})
-                v->assign(nh, 0.0);
-            calhm1(nl, vz, &ecKx[jt], rai, z.h5mx.data());
-            calhm1(nl, vz, &ecKy[jt], rai, z.h5my.data());
-            calhm1(nl, vz, &ecKz[jt], rai, z.h5mz.data());
-            calhm1(nl, vz, &ecL[jt], rai, z.h6m1L.data());
-            calhm1(nl, vz, &ecN[jt], rai, z.h6m1N.data());
-            calhm2(nl, vz, &ecL[jt], rai, z.h6m2L.data());
-            calhm2(nl, vz, &ecN[jt], rai, z.h6m2N.data());
-            mtrnp2(nl, 1, 2, z.h5mx.data(), z.h3mx.data());
-            mtrnp2(nl, 1, 2, z.h5my.data(), z.h3my.data());
-            mtrnp2(nl, 1, 2, z.h5mz.data(), z.h3mz.data());
-            mtrnp2(nl, 1, 2, z.h6m1L.data(), z.h4m2L.data());
-            mtrnp2(nl, 1, 2, z.h6m1N.data(), z.h4m2N.data());
-            mtrnp2(nl, 2, 1, z.h6m2L.data(), z.h4m1L.data());
-            mtrnp2(nl, 2, 1, z.h6m2N.data(), z.h4m1N.data());
+            lumped(&rho, &z.t, 2, false);
+            lumped(&ecKx, &z.h1x, 0, true);
+            lumped(&ecKy, &z.h1y, 0, true);
+            lumped(&ecKz, &z.h1z, 0, true);
+            lumped(&ecL, &z.h2L, 0, true);
+            lumped(&ecN, &z.h2N, 0, true);
+            plain(&ecKx, &z.h5ax, 1, 0, 1);
+            plain(&ecKy, &z.h5ay, 1, 0, 1);
+            plain(&ecKz, &z.h5az, 1, 0, 1);
+            plain(&ecL, &z.h6aL, 1, 0, 1);
+            plain(&ecN, &z.h6aN, 1, 0, 1);
+            plain(&ecKx, &z.h7x, 2, 1, 1);
+            plain(&ecKy, &z.h7y, 2, 1, 1);
+            plain(&ecKz, &z.h7z, 2, 1, 1);
+            plain(&ecL, &z.h8L, 2, 1, 1);
+            plain(&ecN, &z.h8N, 2, 1, 1);
         } else {
             vec(z.p1);
             vec(z.p2);
             vec(z.p3);
-            const double *V = vra.data();
-            calmatc_psv(nl, vnp, V, rhoinv.data(), 2, 1, 1, rai, z.p1.data());
-            calmatc_psv(nl, vnp, V, rhoinv.data(), 0, 0, 0, rai, z.p2.data());
-            calhl(nl, vnp, V, rhoinv.data(), rai, work.data());
-            calt(nl, z.p2.data(), work.data(), z.p2.data());
-            calmatc_psv(nl, vnp, V, kappainv.data(), 2, 0, 0, rai, z.p3.data());
-            caltl(nl, vnp, V, kappainv.data(), rai, work.data());
-            calt(nl, z.p3.data(), work.data(), z.p3.data());
+            plain(&rhoinv, &z.p1, 2, 1, 1);
+            lumped(&rhoinv, &z.p2, 0, true);
+            lumped(&kappainv, &z.p3, 2, false);
         }
+    }
+    run_tasks(tasks);
+    for (int iz = 0; iz < nz; iz++) {
+        PsvZoneFE &z = s.fe[iz];
+        if (!z.solid) continue;
+        const int nl = z.nlayer;
+        const double *rai = &s.ra[s.isp[iz] - 1];
+        std::vector<double> work(4 * nl);
+        mtrnp(nl, z.h5ax.data(), z.h3ax.data());
+        mtrnp(nl, z.h5ay.data(), z.h3ay.data());
+        mtrnp(nl, z.h5az.data(), z.h3az.data());
+        mtrnp(nl, z.h6aL.data(), z.h4aL.data());
+        mtrnp(nl, z.h6aN.data(), z.h4aN.data());
+        // modified first-derivative operator (SGTpsv-MPI.f90:438-478);
+        // caltstg recomputes the same structure values
+        const int jt = s.isp[iz] + iz - 1;  // zone's first structure node, 0-based
+        const double *vz = &vra[jt];
+        calh5(nl, vz, &ecKx[jt], rai, work.data());
+        for (int k = 0; k < 4 * nl; k++) z.h5x[k] = z.h5ax[k] - work[k];
+        calh5(nl, vz, &ecKy[jt], rai, work.data());
+        for (int k = 0; k < 4 * nl; k++) z.h5y[k] = z.h5ay[k] - work[k];
+        calh5(nl, vz, &ecKz[jt], rai, work.data());
+        for (int k = 0; k < 4 * nl; k++) z.h5z[k] = z.h5az[k] - work[k];
+        calh5(nl, vz, &ecL[jt], rai, work.data());
+        for (int k = 0; k < 4 * nl; k++) z.h6L[k] = z.h6aL[k] - work[k];
+        calh5(nl, vz, &ecN[jt], rai, work.data());
+        for (int k = 0; k < 4 * nl; k++) z.h6N[k] = z.h6aN[k] - work[k];
+        mtrnp(nl, z.h5x.data(), z.h3x.data());
+        mtrnp(nl, z.h5y.data(), z.h3y.data());
+        mtrnp(nl, z.h5z.data(), z.h3z.data());
+        mtrnp(nl, z.h6L.data(), z.h4L.data());
+        mtrnp(nl, z.h6N.data(), z.h4N.data());
+        const int nh = 4 * (nl + 1);
+        for (auto *v : {&z.h3mx, &z.h3my, &z.h3mz, &z.h5mx, &z.h5my, &z.h5mz, &z.h4m1L, &z.h4m1N, &z.h4m2L,
+                        &z.h4m2N, &z.h6m1L, &z.h6m1N, &z.h6m2L, &z.h6m2N})
+            v->assign(nh, 0.0);
+        calhm1(nl, vz, &ecKx[jt], rai, z.h5mx.data());
+        calhm1(nl, vz, &ecKy[jt], rai, z.h5my.data());
+        calhm1(nl, vz, &ecKz[jt], rai, z.h5mz.data());
+        calhm1(nl, vz, &ecL[jt], rai, z.h6m1L.data());
+        calhm1(nl, vz, &ecN[jt], rai, z.h6m1N.data());
+        calhm2(nl, vz, &ecL[jt], rai, z.h6m2L.data());
+        calhm2(nl, vz, &ecN[jt], rai, z.h6m2N.data());
+        mtrnp2(nl, 1, 2, z.h5mx.data(), z.h3mx.data());
+        mtrnp2(nl, 1, 2, z.h5my.data(), z.h3my.data());
+        mtrnp2(nl, 1, 2, z.h5mz.data(), z.h3mz.data());
+        mtrnp2(nl, 1, 2, z.h6m1L.data(), z.h4m2L.data());
+        mtrnp2(nl, 1, 2, z.h6m1N.data(), z.h4m2N.data());
+        mtrnp2(nl, 2, 1, z.h6m2L.data(), z.h4m1L.data());
+        mtrnp2(nl, 2, 1, z.h6m2N.data(), z.h4m1N.data());
     }
     // calspdr (others.f90:870-898), ksta and nn (SGTpsv-MPI.f90:627-635)
     s.kkdr.assign(nz, 1);
