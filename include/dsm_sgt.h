@@ -147,7 +147,7 @@ typedef struct dsm_sgt_tuning {
     int32_t overlap;    /* 1 (BF16 P-SV forms, eager launches): the synthesis of batch b runs on a
                            stream of its own beside the solves of batch b+1, with the buffers between
                            them held twice (about 47 GB more for 128 frequencies of the 5 s grid);
-                           0: one stream */
+                           0: one stream.  Not chosen by dsm_sgt_autotune */
     int32_t reserved[2];
 } dsm_sgt_tuning_t;
 
@@ -162,8 +162,8 @@ int dsm_sgt_get_tuning(dsm_sgt_t *h, dsm_sgt_tuning_t *tuning);
  * plan run twice, the faster run counted, and returns the fastest plan in
  * `best` and its probe time in `probe_ms`.  Candidates: the batch size,
  * the l chunk (SH, FP64 P-SV) and the synthesis tile grouping (BF16
- * forms); then graph mode and the overlap of synthesis and solves for the
- * winner (batched P-SV forms).  With `cache_path` (may be NULL) the result is looked up in and
+ * forms); then graph mode for the winner (batched P-SV forms).  The overlap
+ * plan is not a candidate (a caller sets it).  With `cache_path` (may be NULL) the result is looked up in and
  * appended to that text file, keyed by a hash of the input, mode,
  * precision and device name; a cached plan returns with probe_ms = 0.
  * The probe engines' catalogs are discarded and their device memory freed
